@@ -1,0 +1,168 @@
+/*
+ * alto_hip.h -- C ABI of the MI355X (gfx950) ALTO sparse CP-decomposition
+ * library, libalto_amd.so.
+ *
+ * Plain pointers, sizes and opaque handles only.  Every call returns an int
+ * status (ALTO_OK = 0, negative on error) instead of asserting or exiting as
+ * the reference does (alto.hpp:995, matrix.cpp:347, cpd.hpp:471).
+ * `stream` arguments are hipStream_t values passed as void* (NULL = the
+ * library's default stream).
+ *
+ * Each entry point names the reference interface it replaces
+ * (paths relative to the jeewhanchoi/ALTO-stream repository root).
+ */
+#ifndef ALTO_HIP_H_
+#define ALTO_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ALTO_MAX_MODES 8
+
+/* Status codes. */
+#define ALTO_OK 0
+#define ALTO_ERR_ARG -1          /* bad argument (null pointer, bad mode, rank, ...) */
+#define ALTO_ERR_KEY_WIDTH -2    /* coordinates need more bits than the key holds */
+#define ALTO_ERR_HIP -3          /* HIP runtime error (details: alto_hip_last_error) */
+#define ALTO_ERR_NOMEM -4        /* device allocation failed */
+#define ALTO_ERR_NO_DEVICE -5    /* no usable GPU */
+#define ALTO_ERR_UNSUPPORTED -6  /* configuration not built (e.g. rank > 128) */
+#define ALTO_ERR_NUMERIC -7      /* Cholesky failure where the reference exits */
+#define ALTO_ERR_EMPTY_SLICE -8  /* CP-stream time slice with no nonzeros */
+
+/* Arithmetic of the MTTKRP / factor storage. */
+#define ALTO_PREC_F64 0   /* FP64 values, factors and output (plain path) */
+#define ALTO_PREC_F32 1   /* FP32 values and factors, FP32 products */
+#define ALTO_PREC_BF16 2  /* FP32 values, BF16-stored factors widened to FP32 */
+
+/* Fit convention (cpd.hpp:193-275, see DESIGN notes in the library). */
+#define ALTO_FIT_EXACT 0           /* ||M||^2 from the symmetric Grams */
+#define ALTO_FIT_REFERENCE_DIAG 1  /* ||M||^2 from the Gram lower triangles as
+                                      the reference reads them (zero off-diagonal) */
+
+typedef struct alto_hip_tensor_s *alto_hip_tensor;
+
+/* A COO sparse tensor; mirrors SparseTensor (include/sptensor.hpp:8-25):
+ * per-mode index arrays cidx[m][i] (0-based, uint64) and FP64 values. */
+typedef struct {
+    int nmode;
+    const uint64_t *dims;            /* host array, nmode entries */
+    uint64_t nnz;
+    const uint64_t *const *cidx;     /* host array of nmode pointers */
+    const double *vals;
+    int on_device;                   /* 1: cidx[m] and vals are device pointers */
+} alto_hip_coo;
+
+/* ---- runtime ---- */
+const char *alto_hip_error_string(int status);
+const char *alto_hip_last_error(void);
+int alto_hip_init(int device);
+int alto_hip_device_name(char *buf, size_t len);
+int alto_hip_malloc(void **ptr, size_t bytes);
+int alto_hip_free(void *ptr);
+int alto_hip_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream);
+int alto_hip_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream);
+int alto_hip_memset(void *ptr, int value, size_t bytes, void *stream);
+int alto_hip_stream_create(void **stream);
+int alto_hip_stream_destroy(void *stream);
+int alto_hip_stream_sync(void *stream);
+/* Converts n elements between precisions (F64 <-> F32 <-> BF16 bits). */
+int alto_hip_convert(void *dst, int dst_prec, const void *src, int src_prec,
+                     size_t n, void *stream);
+/* Event timing on `stream`: start records, stop records and synchronises,
+ * returning the elapsed milliseconds. */
+int alto_hip_timer_start(void **timer, void *stream);
+int alto_hip_timer_stop(void *timer, void *stream, float *ms);
+
+/* ---- ALTO tensor: create_alto / destroy_alto (include/alto.hpp:436-604) ----
+ * Masks (setup_packed_alto, LSB_FIRST + SHORT_FIRST, alto.hpp:954-1057),
+ * linearisation (alto.hpp:515-531), sort (alto.hpp:1059-1090) and the
+ * nprtn line partitions with per-mode intervals (prtn_alto,
+ * alto.hpp:1128-1226) are computed on the GPU.  key_words: 0 = smallest
+ * that fits (64-bit when the coordinates need <= 64 bits), 1 = 64-bit
+ * (ALTO_MASK_LENGTH 64), 2 = 128-bit. */
+int alto_hip_tensor_create(const alto_hip_coo *coo, int nprtn, int key_words,
+                           void *stream, alto_hip_tensor *out);
+int alto_hip_tensor_destroy(alto_hip_tensor t);
+int alto_hip_tensor_info(alto_hip_tensor t, int *nmode, uint64_t *nnz,
+                         int *key_words, int *nprtn, int *total_bits);
+int alto_hip_tensor_dims(alto_hip_tensor t, uint64_t *dims);
+/* Getters for the bit-exact checks: masks (nmode*key_words words), sorted
+ * keys (nnz*key_words words, word 0 = low 64 bits), sorted FP64 values,
+ * the sort permutation (sorted position -> input index), partitions. */
+int alto_hip_tensor_masks(alto_hip_tensor t, uint64_t *masks);
+int alto_hip_tensor_keys(alto_hip_tensor t, uint64_t *keys);
+int alto_hip_tensor_vals(alto_hip_tensor t, double *vals);
+int alto_hip_tensor_perm(alto_hip_tensor t, uint32_t *perm);
+int alto_hip_tensor_partitions(alto_hip_tensor t, uint64_t *prtn_ptr,
+                               uint64_t *intervals);
+/* Device pointers of the sorted keys / FP64 values (for device-side users). */
+int alto_hip_tensor_device_arrays(alto_hip_tensor t, const uint64_t **keys,
+                                  const double **vals);
+
+/* ---- MTTKRP: mttkrp_alto_par (include/alto.hpp:902-908) ----
+ * out[target] += MTTKRP.  factors: host array of nmode DEVICE pointers,
+ * row-major I_m x rank, element type by `prec` (double / float / bf16 bits).
+ * out: device, dims[target] x rank, double when out_f64 != 0 (or prec is
+ * F64), else float.  The output is accumulated, as the reference does, so
+ * callers zero it first (cpd.hpp:110). */
+int alto_hip_mttkrp(alto_hip_tensor t, int target, int prec, int rank,
+                    const void *const *factors, void *out, int out_f64,
+                    void *stream);
+/* Same, selecting the kernel explicitly (for tests / the autotuner):
+ * algo 0 = automatic, 1 = ALTO-order segments (atomic flush),
+ * 2 = output-ordered segments (per-mode row order), 3 = LDS-privatised rows. */
+int alto_hip_mttkrp_algo(alto_hip_tensor t, int target, int prec, int rank,
+                         const void *const *factors, void *out, int out_f64,
+                         int algo, void *stream);
+/* Reference COO MTTKRP, mttkrp (src/mttkrp.cpp:122-156), on the GPU:
+ * out += MTTKRP over a device COO tensor, FP64. */
+int alto_hip_mttkrp_coo(const alto_hip_coo *coo, int target, int rank,
+                        const double *const *factors, double *out, void *stream);
+
+/* ---- CP-ALS: cpd_alto (include/cpd.hpp:47-191) with pseudo_inverse
+ * (cpd.hpp:377-480), update_gram (src/gram.cpp:29-42), KruskalModelNorm
+ * (src/kruskal_model.cpp:319-353) and cpd_fit_alto (cpd.hpp:193-275). */
+typedef struct {
+    int rank;
+    int max_iters;
+    double epsilon;       /* stop when |fit - prev_fit| < epsilon, it > 0 */
+    int prec;             /* ALTO_PREC_F64 / F32 / BF16 */
+    int fit_mode;         /* ALTO_FIT_REFERENCE_DIAG (default printout) or EXACT */
+    int verbose;          /* 1: print the reference's per-iteration lines */
+} alto_hip_als_opts;
+
+/* factors: host, nmode pointers to I_m x rank FP64 (in: initial model, out:
+ * final model); lambda: host rank doubles (out).  fits / fits_exact: host
+ * arrays of max_iters (may be NULL).  timings (may be NULL): 8 doubles,
+ * seconds: total, MTTKRP, pseudo-inverse, copy, normalise, Gram, fit, and
+ * per-iteration total.  Returns the iteration count in *iters. */
+int alto_hip_cpd_als(alto_hip_tensor t, const alto_hip_als_opts *opts,
+                     double *const *factors, double *lambda, double *fits,
+                     double *fits_exact, int *iters, double *timings,
+                     void *stream);
+
+/* ---- CP-stream: cpstream + cpstream_alto_iter (src/streaming_cpd.cpp:26-366,
+ * 1198-1519) with admm_cf_ (src/admm.cpp:216-429), CPSTREAM_ALTO model,
+ * timeslice limit 1.  init_factors: host, nmode arrays of dims[m] x rank
+ * rows used when the model grows (row order = first-appearance labels).
+ * Outputs (host, may be NULL): final factors (dims[m] x rank each), final
+ * dims, the time row s_t per slice, outer iterations per slice, ADMM
+ * iterations per slice, final Grams (nmode*rank*rank).  timings (may be
+ * NULL): 4 doubles, seconds: total, tensor build, MTTKRP, ADMM+dense.
+ * Returns the number of slices processed, or a negative status. */
+int alto_hip_cpstream(const alto_hip_coo *coo, int stream_mode, int rank,
+                      int max_iters, double epsilon, int nonneg, int max_slices,
+                      const double *const *init_factors, double *const *out_factors,
+                      uint64_t *out_dims, double *out_time_rows,
+                      int *out_outer_iters, double *out_admm_iters,
+                      double *out_grams, double *timings, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,88 @@
+/*
+ * alto_oracle.h -- CPU FP64 restatement of the ALTO-stream sparse CP hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the oracle: the parity tests,
+ * the smoke check and the CPU-baseline leg of bench.py load it as a checker.
+ * The product library (alto-stream_amd/) never links, loads or calls it.
+ *
+ * Keys are arrays of uint64 words, kw words per key (kw = 1 for the 64-bit
+ * ALTO mask, 2 for the 128-bit mask), little-endian word order (word 0 holds
+ * key bits 0..63).  Masks are stored the same way, nmode*kw words.
+ * Factor matrices are row-major I_n x R doubles, as in the reference's
+ * KruskalModel (include/kruskal_model.hpp:9-16).
+ */
+#ifndef ALTO_ORACLE_H_
+#define ALTO_ORACLE_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MAX_MODES 8
+
+/* MTTKRP variant chosen by the reference dispatcher (alto.hpp:835-846). */
+enum { ORC_VAR_ATOMIC = 0, ORC_VAR_STREAM = 1, ORC_VAR_DA_MEM_PULL = 2 };
+
+/* Fit convention for cpd_fit_alto (cpd.hpp:193-275). */
+enum { ORC_FIT_EXACT = 0, ORC_FIT_REFERENCE_DIAG = 1 };
+
+void orc_set_threads(int nthreads);
+int orc_get_threads(void);
+
+int orc_num_bits(uint64_t dim);
+int orc_alto_masks(int nmode, const uint64_t *dims, int kw, uint64_t *masks);
+void orc_pdep(uint64_t x, const uint64_t *mask, int kw, uint64_t *out);
+uint64_t orc_pext(const uint64_t *key, const uint64_t *mask, int kw);
+void orc_linearize(int nmode, uint64_t nnz, const uint64_t *const *cidx,
+                   const uint64_t *masks, int kw, uint64_t *keys);
+void orc_sort_alto(uint64_t nnz, int kw, uint64_t *keys, double *vals,
+                   uint64_t *perm);
+void orc_prtn_alto(int nmode, const uint64_t *dims, uint64_t nnz, int kw,
+                   const uint64_t *keys, const uint64_t *masks, int nprtn,
+                   uint64_t *prtn_ptr, uint64_t *intervals);
+int orc_get_num_ptrn(uint64_t nnz, int max_threads);
+
+void orc_mttkrp_coo(int nmode, uint64_t nnz, const uint64_t *const *cidx,
+                    const double *vals, double *const *factors, int rank,
+                    int mode, double *out);
+int orc_mttkrp_variant(int nmode, const uint64_t *dims, uint64_t nnz,
+                       int stream_mode, int target);
+int orc_mttkrp_alto(int nmode, const uint64_t *dims, uint64_t nnz, int kw,
+                    const uint64_t *keys, const double *vals,
+                    const uint64_t *masks, int nprtn, const uint64_t *prtn_ptr,
+                    const uint64_t *intervals, int stream_mode,
+                    double *const *factors, int rank, int target, double *out);
+
+void orc_gram_upper(uint64_t rows, int rank, const double *U, double *G);
+int orc_pseudo_inverse_als(int nmode, int rank, double *const *grams,
+                           int mode, uint64_t rows, double *U);
+void orc_kruskal_norm(uint64_t rows, int rank, double *U, double *lambda,
+                      int which);
+double orc_cpd_fit(int nmode, const uint64_t *dims, int rank,
+                   double *const *factors, const double *lambda,
+                   double *const *grams, const double *last_mttkrp,
+                   double normX, int fit_mode);
+int orc_cpd_als(int nmode, const uint64_t *dims, uint64_t nnz, int kw,
+                const uint64_t *keys, const double *vals,
+                const uint64_t *masks, int nprtn, const uint64_t *prtn_ptr,
+                const uint64_t *intervals, int rank, int max_iters,
+                double eps, int fit_mode, double *const *factors,
+                double *lambda, double *fits, double *fits_exact);
+
+double orc_admm_cf(uint64_t rows, int rank, int nmode, int mode,
+                   double *const *grams, int nonneg, double *primal,
+                   double *dual, double *mttkrp);
+int orc_cpstream(int nmode, const uint64_t *dims, uint64_t nnz,
+                 const uint64_t *const *cidx, const double *vals,
+                 int stream_mode, int rank, int max_iters, double eps,
+                 int nonneg, int max_slices, double *const *init_factors,
+                 double *const *out_factors, uint64_t *out_dims,
+                 double *out_time_rows, int *out_outer_iters,
+                 double *out_admm_iters, double *out_grams);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

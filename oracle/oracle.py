@@ -1,0 +1,276 @@
+"""ctypes binding of the CPU oracle (oracle/alto_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker or the timed CPU port,
+never as the product path.
+
+Arrays are numpy; keys are uint64 arrays of shape (nnz, kw).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+
+VAR_ATOMIC, VAR_STREAM, VAR_DA_MEM_PULL = 0, 1, 2
+FIT_EXACT, FIT_REFERENCE_DIAG = 0, 1
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_lib = None
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+        L = _lib
+        L.orc_num_bits.restype = ctypes.c_int
+        L.orc_num_bits.argtypes = [ctypes.c_uint64]
+        L.orc_alto_masks.restype = ctypes.c_int
+        L.orc_pext.restype = ctypes.c_uint64
+        L.orc_get_num_ptrn.restype = ctypes.c_int
+        L.orc_get_num_ptrn.argtypes = [ctypes.c_uint64, ctypes.c_int]
+        L.orc_mttkrp_variant.restype = ctypes.c_int
+        L.orc_mttkrp_alto.restype = ctypes.c_int
+        L.orc_pseudo_inverse_als.restype = ctypes.c_int
+        L.orc_cpd_fit.restype = ctypes.c_double
+        L.orc_cpd_als.restype = ctypes.c_int
+        L.orc_admm_cf.restype = ctypes.c_double
+        L.orc_cpstream.restype = ctypes.c_int
+        L.orc_get_threads.restype = ctypes.c_int
+    return _lib
+
+
+def _p64(a):
+    return a.ctypes.data_as(_u64p)
+
+
+def _pf(a):
+    return a.ctypes.data_as(_f64p)
+
+
+def _ptr_array(arrs, kind):
+    T = _u64p if kind == "u" else _f64p
+    arr = (T * len(arrs))()
+    for i, a in enumerate(arrs):
+        arr[i] = a.ctypes.data_as(T) if a is not None else None
+    return arr
+
+
+def set_threads(n):
+    lib().orc_set_threads(ctypes.c_int(int(n)))
+
+
+def num_bits(dim):
+    return lib().orc_num_bits(ctypes.c_uint64(int(dim)))
+
+
+def key_words(dims):
+    total = sum(num_bits(d) for d in dims)
+    return 1 if total <= 64 else 2
+
+
+def alto_masks(dims, kw=None):
+    dims = np.ascontiguousarray(dims, dtype=np.uint64)
+    if kw is None:
+        kw = key_words(dims)
+    masks = np.zeros((len(dims), kw), dtype=np.uint64)
+    total = lib().orc_alto_masks(ctypes.c_int(len(dims)), _p64(dims), ctypes.c_int(kw), _p64(masks))
+    if total < 0:
+        raise ValueError("ALTO key needs more than %d bits" % (64 * kw))
+    return masks, total
+
+
+def pdep(x, mask):
+    mask = np.ascontiguousarray(mask, dtype=np.uint64)
+    out = np.zeros(len(mask), dtype=np.uint64)
+    lib().orc_pdep(ctypes.c_uint64(int(x)), _p64(mask), ctypes.c_int(len(mask)), _p64(out))
+    return out
+
+
+def pext(key, mask):
+    key = np.ascontiguousarray(key, dtype=np.uint64)
+    mask = np.ascontiguousarray(mask, dtype=np.uint64)
+    return int(lib().orc_pext(_p64(key), _p64(mask), ctypes.c_int(len(mask))))
+
+
+def linearize(cidx, masks):
+    cidx = [np.ascontiguousarray(c, dtype=np.uint64) for c in cidx]
+    nnz = len(cidx[0])
+    kw = masks.shape[1]
+    keys = np.zeros((nnz, kw), dtype=np.uint64)
+    lib().orc_linearize(ctypes.c_int(len(cidx)), ctypes.c_uint64(nnz), _ptr_array(cidx, "u"),
+                        _p64(np.ascontiguousarray(masks)), ctypes.c_int(kw), _p64(keys))
+    return keys
+
+
+def sort_alto(keys, vals):
+    keys = np.ascontiguousarray(keys, dtype=np.uint64).copy()
+    vals = np.ascontiguousarray(vals, dtype=np.float64).copy()
+    nnz, kw = keys.shape
+    perm = np.zeros(nnz, dtype=np.uint64)
+    lib().orc_sort_alto(ctypes.c_uint64(nnz), ctypes.c_int(kw), _p64(keys), _pf(vals), _p64(perm))
+    return keys, vals, perm
+
+
+def prtn_alto(dims, keys, masks, nprtn):
+    dims = np.ascontiguousarray(dims, dtype=np.uint64)
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    nnz, kw = keys.shape
+    ptr = np.zeros(nprtn + 1, dtype=np.uint64)
+    iv = np.zeros((nprtn, len(dims), 2), dtype=np.uint64)
+    lib().orc_prtn_alto(ctypes.c_int(len(dims)), _p64(dims), ctypes.c_uint64(nnz), ctypes.c_int(kw),
+                        _p64(keys), _p64(np.ascontiguousarray(masks)), ctypes.c_int(nprtn),
+                        _p64(ptr), _p64(iv))
+    return ptr, iv
+
+
+def get_num_ptrn(nnz, max_threads):
+    return lib().orc_get_num_ptrn(ctypes.c_uint64(int(nnz)), ctypes.c_int(int(max_threads)))
+
+
+def mttkrp_coo(cidx, vals, factors, mode, out=None):
+    cidx = [np.ascontiguousarray(c, dtype=np.uint64) for c in cidx]
+    vals = np.ascontiguousarray(vals, dtype=np.float64)
+    factors = [np.ascontiguousarray(f, dtype=np.float64) for f in factors]
+    rank = factors[0].shape[1]
+    if out is None:
+        out = np.zeros_like(factors[mode])
+    lib().orc_mttkrp_coo(ctypes.c_int(len(cidx)), ctypes.c_uint64(len(vals)), _ptr_array(cidx, "u"),
+                         _pf(vals), _ptr_array(factors, "f"), ctypes.c_int(rank), ctypes.c_int(mode),
+                         _pf(out))
+    return out
+
+
+class AltoTensor:
+    """Oracle-side ALTO tensor: masks, sorted keys/values, partitions."""
+
+    def __init__(self, dims, cidx, vals, nprtn, kw=None):
+        self.dims = np.ascontiguousarray(dims, dtype=np.uint64)
+        self.nmode = len(dims)
+        self.masks, self.bits = alto_masks(self.dims, kw)
+        self.kw = self.masks.shape[1]
+        keys = linearize(cidx, self.masks)
+        self.keys, self.vals, self.perm = sort_alto(keys, vals)
+        self.nnz = len(self.vals)
+        self.nprtn = nprtn
+        self.prtn_ptr, self.intervals = prtn_alto(self.dims, self.keys, self.masks, nprtn)
+
+    def variant(self, target, stream_mode=-1):
+        return lib().orc_mttkrp_variant(ctypes.c_int(self.nmode), _p64(self.dims),
+                                        ctypes.c_uint64(self.nnz), ctypes.c_int(stream_mode),
+                                        ctypes.c_int(target))
+
+    def mttkrp(self, factors, target, stream_mode=-1, out=None):
+        factors = [np.ascontiguousarray(f, dtype=np.float64) for f in factors]
+        rank = factors[0].shape[1]
+        if out is None:
+            out = np.zeros((int(self.dims[target]), rank), dtype=np.float64)
+        var = lib().orc_mttkrp_alto(
+            ctypes.c_int(self.nmode), _p64(self.dims), ctypes.c_uint64(self.nnz), ctypes.c_int(self.kw),
+            _p64(self.keys), _pf(self.vals), _p64(self.masks), ctypes.c_int(self.nprtn),
+            _p64(self.prtn_ptr), _p64(self.intervals), ctypes.c_int(stream_mode),
+            _ptr_array(factors, "f"), ctypes.c_int(rank), ctypes.c_int(target), _pf(out))
+        return out, var
+
+    def cpd_als(self, factors, rank, max_iters, eps=1e-30, fit_mode=FIT_REFERENCE_DIAG):
+        factors = [np.ascontiguousarray(f, dtype=np.float64).copy() for f in factors]
+        lam = np.ones(rank, dtype=np.float64)
+        fits = np.zeros(max_iters, dtype=np.float64)
+        fits_exact = np.zeros(max_iters, dtype=np.float64)
+        n = lib().orc_cpd_als(
+            ctypes.c_int(self.nmode), _p64(self.dims), ctypes.c_uint64(self.nnz), ctypes.c_int(self.kw),
+            _p64(self.keys), _pf(self.vals), _p64(self.masks), ctypes.c_int(self.nprtn),
+            _p64(self.prtn_ptr), _p64(self.intervals), ctypes.c_int(rank), ctypes.c_int(max_iters),
+            ctypes.c_double(eps), ctypes.c_int(fit_mode), _ptr_array(factors, "f"), _pf(lam),
+            _pf(fits), _pf(fits_exact))
+        return factors, lam, fits[:n], fits_exact[:n]
+
+
+def gram_upper(U):
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    R = U.shape[1]
+    G = np.zeros((R, R), dtype=np.float64)
+    lib().orc_gram_upper(ctypes.c_uint64(U.shape[0]), ctypes.c_int(R), _pf(U), _pf(G))
+    return G
+
+
+def kruskal_norm(U, which):
+    U = np.ascontiguousarray(U, dtype=np.float64).copy()
+    lam = np.zeros(U.shape[1], dtype=np.float64)
+    lib().orc_kruskal_norm(ctypes.c_uint64(U.shape[0]), ctypes.c_int(U.shape[1]), _pf(U), _pf(lam),
+                           ctypes.c_int(which))
+    return U, lam
+
+
+def pseudo_inverse_als(grams, mode, U):
+    grams = [np.ascontiguousarray(g, dtype=np.float64).copy() for g in grams]
+    U = np.ascontiguousarray(U, dtype=np.float64).copy()
+    R = U.shape[1]
+    fb = lib().orc_pseudo_inverse_als(ctypes.c_int(len(grams)), ctypes.c_int(R), _ptr_array(grams, "f"),
+                                      ctypes.c_int(mode), ctypes.c_uint64(U.shape[0]), _pf(U))
+    return U, grams, fb
+
+
+def admm_cf(grams, mode, primal, dual, mttkrp, nonneg=True):
+    grams = [np.ascontiguousarray(g, dtype=np.float64) for g in grams]
+    primal = np.ascontiguousarray(primal, dtype=np.float64).copy()
+    dual = np.ascontiguousarray(dual, dtype=np.float64).copy()
+    mttkrp = np.ascontiguousarray(mttkrp, dtype=np.float64).copy()
+    R = primal.shape[1]
+    it = lib().orc_admm_cf(ctypes.c_uint64(primal.shape[0]), ctypes.c_int(R), ctypes.c_int(len(grams)),
+                           ctypes.c_int(mode), _ptr_array(grams, "f"), ctypes.c_int(1 if nonneg else 0),
+                           _pf(primal), _pf(dual), _pf(mttkrp))
+    return primal, dual, it
+
+
+def cpstream(dims, cidx, vals, stream_mode, rank, max_iters, eps, nonneg, init_factors,
+             max_slices=0):
+    dims = np.ascontiguousarray(dims, dtype=np.uint64)
+    N = len(dims)
+    cidx = [np.ascontiguousarray(c, dtype=np.uint64) for c in cidx]
+    vals = np.ascontiguousarray(vals, dtype=np.float64)
+    init = [np.ascontiguousarray(f, dtype=np.float64) for f in init_factors]
+    nslices_max = int(dims[stream_mode])
+    outs = [np.zeros((int(dims[m]) if m != stream_mode else 1, rank)) for m in range(N)]
+    out_dims = np.zeros(N, dtype=np.uint64)
+    trows = np.zeros((nslices_max, rank))
+    outer = np.zeros(nslices_max, dtype=np.int32)
+    admm = np.zeros(nslices_max)
+    grams = np.zeros((N, rank, rank))
+    n = lib().orc_cpstream(
+        ctypes.c_int(N), _p64(dims), ctypes.c_uint64(len(vals)), _ptr_array(cidx, "u"), _pf(vals),
+        ctypes.c_int(stream_mode), ctypes.c_int(rank), ctypes.c_int(max_iters), ctypes.c_double(eps),
+        ctypes.c_int(1 if nonneg else 0), ctypes.c_int(max_slices), _ptr_array(init, "f"),
+        _ptr_array(outs, "f"), _p64(out_dims), _pf(trows),
+        outer.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _pf(admm), _pf(grams))
+    outs = [o[: int(out_dims[m])] for m, o in enumerate(outs)]
+    return dict(factors=outs, dims=out_dims, time_rows=trows[:n], outer_iters=outer[:n],
+                admm_iters=admm[:n], grams=grams, slices=n)
+
+
+def read_tns(path, zero_based=False):
+    """Reads a FROSTT .tns file like ImportSparseTensor (sptensor.cpp:74-158):
+    '#' lines skipped; dims = max index (+1 if zero based)."""
+    rows = []
+    with open(path) as f:
+        for line in f:
+            if len(line) > 1 and line[0] != "#":
+                rows.append(line.split())
+    nmode = len(rows[0]) - 1
+    idx = np.array([[int(t) for t in r[:nmode]] for r in rows], dtype=np.uint64)
+    vals = np.array([float(r[nmode]) for r in rows], dtype=np.float64)
+    off = 0 if zero_based else 1
+    cidx = [idx[:, m] - np.uint64(off) for m in range(nmode)]
+    dims = [int(idx[:, m].max()) + (1 if zero_based else 0) for m in range(nmode)]
+    return dims, cidx, vals
