@@ -1,0 +1,345 @@
+"""Python binding of libalto_amd.so (the C ABI in include/alto_hip.h).
+
+Mirrors the reference's host interface names so that callers and tests read
+like the reference's own code:
+  create_alto        include/alto.hpp:436-582   -> AltoTensor
+  mttkrp_alto_par    include/alto.hpp:902-908   -> mttkrp_alto_par()
+  mttkrp (COO)       src/mttkrp.cpp:122-156     -> mttkrp_coo()
+  cpd_alto           include/cpd.hpp:47-191     -> cpd_alto()
+  cpstream           src/streaming_cpd.cpp:26-366 -> cpstream()
+
+There is no CPU fallback: if the shared library is missing or no GPU is
+usable, every call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libalto_amd.so")
+
+OK = 0
+PREC_F64, PREC_F32, PREC_BF16 = 0, 1, 2
+FIT_EXACT, FIT_REFERENCE_DIAG = 0, 1
+_PREC_DTYPE = {PREC_F64: np.float64, PREC_F32: np.float32, PREC_BF16: np.uint16}
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+_lib = None
+
+
+class AltoError(RuntimeError):
+    pass
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE, "-j8"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise AltoError("libalto_amd.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        L.alto_hip_error_string.restype = ctypes.c_char_p
+        L.alto_hip_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def check(status, what=""):
+    if status != OK:
+        L = lib()
+        raise AltoError("%s failed: %s (%d): %s" % (
+            what, L.alto_hip_error_string(status).decode(), status, L.alto_hip_last_error().decode()))
+    return status
+
+
+def init(device=0):
+    check(lib().alto_hip_init(ctypes.c_int(device)), "alto_hip_init")
+
+
+def device_name():
+    buf = ctypes.create_string_buffer(256)
+    check(lib().alto_hip_device_name(buf, ctypes.c_size_t(256)), "device_name")
+    return buf.value.decode()
+
+
+def synchronize(stream=None):
+    check(lib().alto_hip_stream_sync(_vp(stream)), "stream_sync")
+
+
+class DeviceArray:
+    """A device allocation with numpy-like shape/dtype, freed on close/GC."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = _vp()
+        check(lib().alto_hip_malloc(ctypes.byref(p), ctypes.c_size_t(max(self.nbytes, 16))), "malloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, a, stream=None):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        if d.nbytes:
+            check(lib().alto_hip_memcpy_h2d(_vp(d.ptr), a.ctypes.data_as(_vp), ctypes.c_size_t(d.nbytes),
+                                            _vp(stream)), "h2d")
+        return d
+
+    @classmethod
+    def zeros(cls, shape, dtype, stream=None):
+        d = cls(shape, dtype)
+        check(lib().alto_hip_memset(_vp(d.ptr), 0, ctypes.c_size_t(max(d.nbytes, 16)), _vp(stream)), "memset")
+        return d
+
+    def zero_(self, stream=None):
+        check(lib().alto_hip_memset(_vp(self.ptr), 0, ctypes.c_size_t(max(self.nbytes, 16)), _vp(stream)),
+              "memset")
+        return self
+
+    def numpy(self, stream=None):
+        a = np.empty(self.shape, dtype=self.dtype)
+        if self.nbytes:
+            check(lib().alto_hip_memcpy_d2h(a.ctypes.data_as(_vp), _vp(self.ptr), ctypes.c_size_t(self.nbytes),
+                                            _vp(stream)), "d2h")
+        return a
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            lib().alto_hip_free(_vp(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def to_prec(a, prec, stream=None):
+    """Device copy of a host FP64 array in the given precision."""
+    src = DeviceArray.from_numpy(np.ascontiguousarray(a, dtype=np.float64), stream)
+    if prec == PREC_F64:
+        return src
+    dst = DeviceArray(src.shape, _PREC_DTYPE[prec])
+    check(lib().alto_hip_convert(_vp(dst.ptr), ctypes.c_int(prec), _vp(src.ptr), ctypes.c_int(PREC_F64),
+                                 ctypes.c_size_t(int(np.prod(src.shape))), _vp(stream)), "convert")
+    src.close()
+    return dst
+
+
+def convert(dst, dst_prec, src, src_prec, n, stream=None):
+    check(lib().alto_hip_convert(_vp(dst.ptr), ctypes.c_int(dst_prec), _vp(src.ptr), ctypes.c_int(src_prec),
+                                 ctypes.c_size_t(n), _vp(stream)), "convert")
+
+
+class _Coo(ctypes.Structure):
+    _fields_ = [("nmode", ctypes.c_int), ("dims", _u64p), ("nnz", ctypes.c_uint64),
+                ("cidx", ctypes.POINTER(_u64p)), ("vals", _f64p), ("on_device", ctypes.c_int)]
+
+
+def _coo(dims, cidx, vals, on_device):
+    dims_a = np.ascontiguousarray(dims, dtype=np.uint64)
+    N = len(dims_a)
+    arr = (_u64p * N)()
+    keep = [dims_a]
+    if on_device:
+        for m in range(N):
+            arr[m] = ctypes.cast(_vp(cidx[m].ptr), _u64p)
+        vp = ctypes.cast(_vp(vals.ptr), _f64p)
+        nnz = vals.shape[0]
+    else:
+        cs = [np.ascontiguousarray(c, dtype=np.uint64) for c in cidx]
+        vv = np.ascontiguousarray(vals, dtype=np.float64)
+        keep += cs + [vv]
+        for m in range(N):
+            arr[m] = cs[m].ctypes.data_as(_u64p)
+        vp = vv.ctypes.data_as(_f64p)
+        nnz = len(vv)
+    c = _Coo(N, dims_a.ctypes.data_as(_u64p), nnz, arr, vp, 1 if on_device else 0)
+    return c, keep + [arr]
+
+
+class AltoTensor:
+    """Device ALTO tensor: create_alto (include/alto.hpp:436-582)."""
+
+    def __init__(self, dims, cidx, vals, nprtn=1, key_words=0, on_device=False, stream=None):
+        c, keep = _coo(dims, cidx, vals, on_device)
+        h = _vp()
+        check(lib().alto_hip_tensor_create(ctypes.byref(c), ctypes.c_int(nprtn), ctypes.c_int(key_words),
+                                           _vp(stream), ctypes.byref(h)), "create_alto")
+        del keep
+        self.handle = h.value
+        nm, nnz, kw, npr, tb = ctypes.c_int(), ctypes.c_uint64(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib().alto_hip_tensor_info(_vp(self.handle), ctypes.byref(nm), ctypes.byref(nnz), ctypes.byref(kw),
+                                         ctypes.byref(npr), ctypes.byref(tb)), "tensor_info")
+        self.nmode, self.nnz, self.kw, self.nprtn, self.total_bits = nm.value, nnz.value, kw.value, npr.value, tb.value
+        self.dims = np.ascontiguousarray(dims, dtype=np.uint64).copy()
+
+    def masks(self):
+        m = np.zeros((self.nmode, self.kw), dtype=np.uint64)
+        check(lib().alto_hip_tensor_masks(_vp(self.handle), m.ctypes.data_as(_u64p)), "masks")
+        return m
+
+    def keys(self):
+        k = np.zeros((self.nnz, self.kw), dtype=np.uint64)
+        check(lib().alto_hip_tensor_keys(_vp(self.handle), k.ctypes.data_as(_u64p)), "keys")
+        return k
+
+    def vals(self):
+        v = np.zeros(self.nnz, dtype=np.float64)
+        check(lib().alto_hip_tensor_vals(_vp(self.handle), v.ctypes.data_as(_f64p)), "vals")
+        return v
+
+    def perm(self):
+        p = np.zeros(self.nnz, dtype=np.uint32)
+        check(lib().alto_hip_tensor_perm(_vp(self.handle), p.ctypes.data_as(_vp)), "perm")
+        return p
+
+    def partitions(self):
+        ptr = np.zeros(self.nprtn + 1, dtype=np.uint64)
+        iv = np.zeros((self.nprtn, self.nmode, 2), dtype=np.uint64)
+        check(lib().alto_hip_tensor_partitions(_vp(self.handle), ptr.ctypes.data_as(_u64p),
+                                               iv.ctypes.data_as(_u64p)), "partitions")
+        return ptr, iv
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().alto_hip_tensor_destroy(_vp(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def create_alto(dims, cidx, vals, nprtn=1, key_words=0, on_device=False, stream=None):
+    return AltoTensor(dims, cidx, vals, nprtn, key_words, on_device, stream)
+
+
+def _ptrs(devs):
+    arr = (_vp * len(devs))()
+    for i, d in enumerate(devs):
+        arr[i] = d.ptr if d is not None else None
+    return arr
+
+
+def mttkrp_alto_par(target, factors, rank, at, out, prec=PREC_F64, out_f64=True, algo=0, stream=None):
+    """out += MTTKRP(target) on device arrays (mttkrp_alto_par, alto.hpp:902-908)."""
+    check(lib().alto_hip_mttkrp_algo(_vp(at.handle), ctypes.c_int(target), ctypes.c_int(prec), ctypes.c_int(rank),
+                                     _ptrs(factors), _vp(out.ptr), ctypes.c_int(1 if out_f64 else 0),
+                                     ctypes.c_int(algo), _vp(stream)), "mttkrp_alto_par")
+
+
+def mttkrp_coo(dims, cidx_dev, vals_dev, target, rank, factors, out, stream=None):
+    c, keep = _coo(dims, cidx_dev, vals_dev, True)
+    check(lib().alto_hip_mttkrp_coo(ctypes.byref(c), ctypes.c_int(target), ctypes.c_int(rank), _ptrs(factors),
+                                    _vp(out.ptr), _vp(stream)), "mttkrp_coo")
+
+
+class _AlsOpts(ctypes.Structure):
+    _fields_ = [("rank", ctypes.c_int), ("max_iters", ctypes.c_int), ("epsilon", ctypes.c_double),
+                ("prec", ctypes.c_int), ("fit_mode", ctypes.c_int), ("verbose", ctypes.c_int)]
+
+
+def cpd_alto(at, factors, rank, max_iters, eps=1e-30, prec=PREC_F64, fit_mode=FIT_REFERENCE_DIAG,
+             verbose=False, stream=None):
+    """CP-ALS (cpd_alto, include/cpd.hpp:47-191).  factors: host FP64 initial
+    model.  Returns (factors, lambda, fits, fits_exact, timings)."""
+    fs = [np.ascontiguousarray(f, dtype=np.float64).copy() for f in factors]
+    arr = (_f64p * len(fs))()
+    for i, f in enumerate(fs):
+        arr[i] = f.ctypes.data_as(_f64p)
+    lam = np.zeros(rank)
+    fits = np.zeros(max(max_iters, 1))
+    fits_exact = np.zeros(max(max_iters, 1))
+    tim = np.zeros(8)
+    iters = ctypes.c_int()
+    o = _AlsOpts(rank, max_iters, eps, prec, fit_mode, 1 if verbose else 0)
+    check(lib().alto_hip_cpd_als(_vp(at.handle), ctypes.byref(o), arr, lam.ctypes.data_as(_f64p),
+                                 fits.ctypes.data_as(_f64p), fits_exact.ctypes.data_as(_f64p),
+                                 ctypes.byref(iters), tim.ctypes.data_as(_f64p), _vp(stream)), "cpd_alto")
+    n = iters.value
+    return fs, lam, fits[:n], fits_exact[:n], tim
+
+
+def cpstream(dims, cidx, vals, stream_mode, rank, max_iters, eps, nonneg, init_factors, max_slices=0,
+             stream=None):
+    """CP-stream (cpstream, src/streaming_cpd.cpp:26-366), host COO input."""
+    c, keep = _coo(dims, cidx, vals, False)
+    dims = np.ascontiguousarray(dims, dtype=np.uint64)
+    N = len(dims)
+    init = [np.ascontiguousarray(f, dtype=np.float64) for f in init_factors]
+    iarr = (_f64p * N)()
+    for i, f in enumerate(init):
+        iarr[i] = f.ctypes.data_as(_f64p)
+    nsl = int(dims[stream_mode])
+    outs = [np.zeros((int(dims[m]) if m != stream_mode else 1, rank)) for m in range(N)]
+    oarr = (_f64p * N)()
+    for i, f in enumerate(outs):
+        oarr[i] = f.ctypes.data_as(_f64p)
+    out_dims = np.zeros(N, dtype=np.uint64)
+    trows = np.zeros((nsl, rank))
+    outer = np.zeros(nsl, dtype=np.int32)
+    admm = np.zeros(nsl)
+    grams = np.zeros((N, rank, rank))
+    tim = np.zeros(4)
+    n = lib().alto_hip_cpstream(ctypes.byref(c), ctypes.c_int(stream_mode), ctypes.c_int(rank),
+                                ctypes.c_int(max_iters), ctypes.c_double(eps), ctypes.c_int(1 if nonneg else 0),
+                                ctypes.c_int(max_slices), iarr, oarr, out_dims.ctypes.data_as(_u64p),
+                                trows.ctypes.data_as(_f64p), outer.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                admm.ctypes.data_as(_f64p), grams.ctypes.data_as(_f64p),
+                                tim.ctypes.data_as(_f64p), _vp(stream))
+    if n < 0:
+        check(n, "cpstream")
+    outs = [o[: int(out_dims[m])] for m, o in enumerate(outs)]
+    return dict(factors=outs, dims=out_dims, time_rows=trows[:n], outer_iters=outer[:n],
+                admm_iters=admm[:n], grams=grams, slices=n, timings=tim)
+
+
+def generate(dims, nnz, seed, mode_dist, zipf_s=None, value_dist=0, p1=0.0, p2=1.0, block_frac=0.0,
+             block=0, stream=None):
+    """Synthetic tensor in device memory -> (cidx device arrays, vals device array)."""
+    dims = np.ascontiguousarray(dims, dtype=np.uint64)
+    N = len(dims)
+    cidx = [DeviceArray((nnz,), np.uint64) for _ in range(N)]
+    vals = DeviceArray((nnz,), np.float64)
+    md = np.ascontiguousarray(mode_dist, dtype=np.int32)
+    zs = np.ascontiguousarray(zipf_s if zipf_s is not None else [1.1] * N, dtype=np.float64)
+    carr = (_vp * N)()
+    for m in range(N):
+        carr[m] = cidx[m].ptr
+    check(lib().alto_hip_generate(ctypes.c_int(N), dims.ctypes.data_as(_u64p), ctypes.c_uint64(nnz),
+                                  ctypes.c_uint64(seed), md.ctypes.data_as(_vp), zs.ctypes.data_as(_f64p),
+                                  ctypes.c_int(value_dist), ctypes.c_double(p1), ctypes.c_double(p2),
+                                  ctypes.c_double(block_frac), ctypes.c_uint64(block), carr, _vp(vals.ptr),
+                                  _vp(stream)), "generate")
+    return cidx, vals
+
+
+class Timer:
+    """HIP-event timer on a stream (milliseconds)."""
+
+    def __init__(self, stream=None):
+        self.stream = stream
+
+    def __enter__(self):
+        t = _vp()
+        check(lib().alto_hip_timer_start(ctypes.byref(t), _vp(self.stream)), "timer_start")
+        self.t = t.value
+        return self
+
+    def __exit__(self, *exc):
+        ms = ctypes.c_float()
+        check(lib().alto_hip_timer_stop(_vp(self.t), _vp(self.stream), ctypes.byref(ms)), "timer_stop")
+        self.ms = ms.value
+        return False

@@ -1,0 +1,238 @@
+// CP-ALS on the GPU: cpd_alto (include/cpd.hpp:47-191).
+//
+// Per iteration and mode n (all on one stream, no host round trip except the
+// fit scalar the convergence test needs):
+//   out = 0; out += MTTKRP_n (FP64 accumulation of FP32/BF16/FP64 products)
+//   vinv = (Hadamard_{m != n} G_m + 1e-3 I)^-1      (FP64 Cholesky, 1 WG)
+//   u = out * vinv, column partials (sum u^2, max u, sum u*out)
+//   lambda = 2-norm (iteration 0) or max-norm (later iterations)
+//   U_n = u / lambda stored in the factor precision, G_n = U_n^T U_n (FP64)
+// then the fit (cpd_fit_alto, cpd.hpp:193-275) in FP64.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+#include "dense.hpp"
+#include "tensor.hpp"
+
+namespace alto {
+
+int run_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const *factors,
+               void *out, int out_f64, int algo, uint64_t per_group, hipStream_t s);
+
+// inner = sum over blocks/columns of u*M; ||M||^2 exact and diagonal-only.
+__global__ __launch_bounds__(256) void k_fit(const double *__restrict__ grams, int nmode, int R,
+                                             const double *__restrict__ lambda,
+                                             const double *__restrict__ partials, int nblk,
+                                             const double *__restrict__ normX, int fit_mode,
+                                             double *__restrict__ fit_out,
+                                             double *__restrict__ fit_exact_out) {
+    extern __shared__ double sh[];  // R*R + R
+    double *tg = sh, *ipc = sh + R * R;
+    const int t = threadIdx.x, RR = R * R;
+    for (int e = t; e < RR; e += blockDim.x) {
+        int i = e / R, j = e % R;
+        double v = lambda[i] * lambda[j];
+        for (int m = 0; m < nmode; ++m) v *= grams[(size_t)m * RR + e];
+        tg[e] = v;
+    }
+    for (int c = t; c < R; c += blockDim.x) {
+        double s = 0.0;
+        for (int b = 0; b < nblk; ++b) s += partials[(size_t)b * 3 * R + 2 * R + c];
+        ipc[c] = s;
+    }
+    __syncthreads();
+    if (t == 0) {
+        double inner = 0.0;
+        for (int c = 0; c < R; ++c) inner += ipc[c];
+        double nx = *normX;
+        double ue = 0.0, ud = 0.0;
+        for (int e = 0; e < RR; ++e) ue += tg[e];
+        for (int i = 0; i < R; ++i) ud += tg[i * R + i];
+        double f[2];
+        double nu[2] = {fabs(ue), fabs(ud)};
+        for (int k = 0; k < 2; ++k) {
+            double r = nx + nu[k] - 2.0 * inner;
+            if (r > 0.0) r = sqrt(r);
+            f[k] = r / sqrt(nx);
+        }
+        *fit_out = fit_mode == ALTO_FIT_EXACT ? f[0] : f[1];
+        *fit_exact_out = f[0];
+    }
+}
+
+static size_t prec_size(int prec) { return prec == ALTO_PREC_F64 ? 8 : prec == ALTO_PREC_F32 ? 4 : 2; }
+
+struct Events {
+    std::vector<hipEvent_t> ev;
+    ~Events() {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+    hipEvent_t make() {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        ev.push_back(e);
+        return e;
+    }
+};
+
+static float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, a, b);
+    return ms;
+}
+
+}  // namespace alto
+
+using namespace alto;
+
+extern "C" int alto_hip_cpd_als(alto_hip_tensor th, const alto_hip_als_opts *o,
+                                double *const *factors, double *lambda, double *fits,
+                                double *fits_exact, int *iters, double *timings, void *stream) {
+    if (!th || !o || !factors) return ALTO_ERR_ARG;
+    TensorImpl *t = &th->impl;
+    const int R = o->rank, N = t->nmode, prec = o->prec;
+    if (R < 1 || R > kMaxRank || o->max_iters < 0) return ALTO_ERR_ARG;
+    if (prec != ALTO_PREC_F64 && prec != ALTO_PREC_F32 && prec != ALTO_PREC_BF16) return ALTO_ERR_ARG;
+    for (int m = 0; m < N; ++m)
+        if (!factors[m]) return ALTO_ERR_ARG;
+    ALTO_TRY(ensure_init());
+    hipStream_t s = to_stream(stream);
+
+    uint64_t maxI = 1;
+    for (int m = 0; m < N; ++m) maxI = t->dims[m] > maxI ? t->dims[m] : maxI;
+    DevBuf Uf[kMaxModes], mt, grams, vinv, flag, part, gpart, lam, normX, fitd, tmp64;
+    const void *fptr[kMaxModes];
+    for (int m = 0; m < N; ++m) {
+        ALTO_TRY(Uf[m].alloc(prec_size(prec) * t->dims[m] * R));
+        fptr[m] = Uf[m].p;
+    }
+    ALTO_TRY(mt.alloc(sizeof(double) * maxI * R));
+    ALTO_TRY(grams.alloc(sizeof(double) * N * R * R));
+    ALTO_TRY(vinv.alloc(sizeof(double) * R * R));
+    ALTO_TRY(flag.alloc(sizeof(int) * 4));
+    ALTO_TRY(part.alloc(sizeof(double) * kMaxBlocks * 3 * R));
+    ALTO_TRY(gpart.alloc(sizeof(double) * kMaxBlocks * R * R));
+    ALTO_TRY(lam.alloc(sizeof(double) * R));
+    ALTO_TRY(normX.alloc(sizeof(double)));
+    int maxit = o->max_iters > 0 ? o->max_iters : 1;
+    ALTO_TRY(fitd.alloc(sizeof(double) * 2 * maxit));
+
+    // Initial model -> factor storage; initial Grams (init_grams, gram.cpp:11-27).
+    for (int m = 0; m < N; ++m) {
+        uint64_t n = t->dims[m] * R;
+        ALTO_HIP_TRY(hipMemcpyAsync(mt.p, factors[m], sizeof(double) * n, hipMemcpyHostToDevice, s));
+        int nb = row_blocks(t->dims[m]);
+        ALTO_TRY(launch_scale_gram(mt.as<double>(), nullptr, t->dims[m], R, Uf[m].p, prec,
+                                   gpart.as<double>(), nb, s));
+        ALTO_TRY(launch_gram_reduce(gpart.as<double>(), nb, R, grams.as<double>() + (size_t)m * R * R, s));
+    }
+    ALTO_TRY(launch_sumsq(t->vals64.as<double>(), t->nnz, normX.as<double>(), s));
+    // Build the per-mode output-ordered schedules outside the timed loop
+    // (tensor preparation, like create_alto).
+    for (int m = 0; m < N; ++m) ALTO_TRY(ensure_schedule(t, m, prec == ALTO_PREC_F64, s));
+    ALTO_HIP_TRY(hipStreamSynchronize(s));
+
+    if (o->verbose)
+        printf("Running ALTO CP-ALS with %d max iterations and %.2e epsilon\n", o->max_iters,
+               o->epsilon);
+    Events evs;
+    double t_tot = 0, t_mttkrp = 0, t_pinv = 0, t_copy = 0, t_norm = 0, t_upd = 0, t_fit = 0;
+    double fit = 0.0, prev = 0.0;
+    int done = o->max_iters;
+    hipEvent_t e_it0 = evs.make(), e_it1 = evs.make();
+    std::vector<hipEvent_t> em(6 * N + 2);
+    for (auto &e : em) e = evs.make();
+    for (int it = 0; it < o->max_iters; ++it) {
+        ALTO_HIP_TRY(hipEventRecord(e_it0, s));
+        for (int n = 0; n < N; ++n) {
+            uint64_t I = t->dims[n];
+            int nb = row_blocks(I);
+            hipEvent_t *E = &em[6 * n];
+            ALTO_HIP_TRY(hipEventRecord(E[0], s));
+            ALTO_TRY(launch_fill_zero(mt.as<double>(), I * R, s));
+            ALTO_HIP_TRY(hipEventRecord(E[1], s));
+            ALTO_TRY(run_mttkrp(t, n, prec, R, fptr, mt.p, 1, 2, 0, s));
+            ALTO_HIP_TRY(hipEventRecord(E[2], s));
+            ALTO_TRY(launch_chol_inv(grams.as<double>(), N, n, R, 1e-3, true, vinv.as<double>(),
+                                     flag.as<int>(), nullptr, s));
+            ALTO_TRY(launch_solve(mt.as<double>(), I, R, vinv.as<double>(), part.as<double>(), nb, s));
+            ALTO_HIP_TRY(hipEventRecord(E[3], s));
+            ALTO_TRY(launch_lambda(part.as<double>(), nb, R, it == 0 ? 0 : 1, lam.as<double>(), s));
+            ALTO_HIP_TRY(hipEventRecord(E[4], s));
+            ALTO_TRY(launch_scale_gram(mt.as<double>(), lam.as<double>(), I, R, Uf[n].p, prec,
+                                       gpart.as<double>(), nb, s));
+            ALTO_TRY(launch_gram_reduce(gpart.as<double>(), nb, R,
+                                        grams.as<double>() + (size_t)n * R * R, s));
+            ALTO_HIP_TRY(hipEventRecord(E[5], s));
+        }
+        hipEvent_t ef0 = em[6 * N], ef1 = em[6 * N + 1];
+        ALTO_HIP_TRY(hipEventRecord(ef0, s));
+        {
+            int nb = row_blocks(t->dims[N - 1]);
+            size_t lds = sizeof(double) * ((size_t)R * R + R);
+            k_fit<<<1, 256, lds, s>>>(grams.as<double>(), N, R, lam.as<double>(), part.as<double>(),
+                                      nb, normX.as<double>(), o->fit_mode,
+                                      fitd.as<double>() + 2 * it, fitd.as<double>() + 2 * it + 1);
+            ALTO_LAUNCH_CHECK();
+        }
+        ALTO_HIP_TRY(hipEventRecord(ef1, s));
+        ALTO_HIP_TRY(hipEventRecord(e_it1, s));
+        double fpair[2];
+        ALTO_HIP_TRY(hipMemcpyAsync(fpair, fitd.as<double>() + 2 * it, sizeof(fpair),
+                                    hipMemcpyDeviceToHost, s));
+        ALTO_HIP_TRY(hipStreamSynchronize(s));
+        double it_s = elapsed(e_it0, e_it1) * 1e-3, mt_s = 0;
+        for (int n = 0; n < N; ++n) {
+            hipEvent_t *E = &em[6 * n];
+            t_copy += elapsed(E[0], E[1]) * 1e-3;
+            mt_s += elapsed(E[1], E[2]) * 1e-3;
+            t_pinv += elapsed(E[2], E[3]) * 1e-3;
+            t_norm += elapsed(E[3], E[4]) * 1e-3;
+            t_upd += elapsed(E[4], E[5]) * 1e-3;
+        }
+        t_fit += elapsed(ef0, ef1) * 1e-3;
+        t_mttkrp += mt_s;
+        t_tot += it_s;
+        fit = fpair[0];
+        if (fits) fits[it] = fpair[0];
+        if (fits_exact) fits_exact[it] = fpair[1];
+        if (o->verbose)
+            printf("it: %d\t fit: %g\t fit-delta: %g\ttime(for MTTKRP): %.4f s (%.4f s)\n", it, fit,
+                   fabs(prev - fit), it_s, mt_s);
+        if (it > 0 && fabs(prev - fit) < o->epsilon) {
+            done = it + 1;
+            break;
+        }
+        prev = fit;
+    }
+    int di = done > 0 ? done : 1;
+    if (o->verbose) {
+        printf("Total time (for MTTKRP):\t %.4f s (%.4f s)\n", t_tot, t_mttkrp);
+        printf("Total     MTTKRP    PseudoInv MemCopy   Normalize Update    Fit\n");
+        printf("%07.4f   %07.4f   %07.4f   %07.4f   %07.4f   %07.4f   %07.4f\n", t_tot, t_mttkrp,
+               t_pinv, t_copy, t_norm, t_upd, t_fit);
+        printf("Per iteration\n%07.4f   %07.4f   %07.4f   %07.4f   %07.4f   %07.4f   %07.4f\n",
+               t_tot / di, t_mttkrp / di, t_pinv / di, t_copy / di, t_norm / di, t_upd / di,
+               t_fit / di);
+    }
+    if (timings) {
+        double tv[8] = {t_tot, t_mttkrp, t_pinv, t_copy, t_norm, t_upd, t_fit, t_tot / di};
+        memcpy(timings, tv, sizeof(tv));
+    }
+    if (iters) *iters = o->max_iters > 0 ? done : 0;
+    // Final model back to the host as FP64.
+    for (int m = 0; m < N; ++m) {
+        uint64_t n = t->dims[m] * R;
+        ALTO_TRY(alto_hip_convert(mt.p, ALTO_PREC_F64, Uf[m].p, prec, n, s));
+        ALTO_HIP_TRY(hipMemcpyAsync(factors[m], mt.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        ALTO_HIP_TRY(hipStreamSynchronize(s));
+    }
+    if (lambda) {
+        ALTO_HIP_TRY(hipMemcpyAsync(lambda, lam.p, sizeof(double) * R, hipMemcpyDeviceToHost, s));
+        ALTO_HIP_TRY(hipStreamSynchronize(s));
+    }
+    return ALTO_OK;
+}

@@ -1,0 +1,389 @@
+// Dense FP64 kernels of the ALS / CP-stream step (gfx950).
+//
+// Reference operations: hadamard_product (src/matrix.cpp:269-294), the Gram
+// Hadamard in pseudo_inverse (include/cpd.hpp:383-411), dpotrf/dpotrs
+// (cpd.hpp:423-433, matrix.cpp:333-372), the dgelsy fallback (cpd.hpp:438-476),
+// KruskalModelNorm (src/kruskal_model.cpp:227-353), update_gram
+// (src/gram.cpp:29-42).
+//
+// Every reduction is two-level with a fixed order (per-block partials, then
+// one block summing the partials in block order), so results do not depend
+// on scheduling.
+#include <float.h>
+#include <math.h>
+
+#include "dense.hpp"
+
+namespace alto {
+
+__global__ void k_fill_zero(double *p, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+int launch_fill_zero(double *p, uint64_t n, hipStream_t s) {
+    if (!n) return ALTO_OK;
+    k_fill_zero<<<cdiv(n, 256), 256, 0, s>>>(p, n);
+    ALTO_LAUNCH_CHECK();
+    return ALTO_OK;
+}
+
+int row_blocks(uint64_t rows) {
+    uint64_t b = (rows + 31) / 32;
+    if (b < 1) b = 1;
+    if (b > (uint64_t)kMaxBlocks) b = kMaxBlocks;
+    return (int)b;
+}
+
+// One workgroup.  Phi = Hadamard of G_m (m != mode, in mode order); A = Phi +
+// diag_add I; Cholesky A = L L^T (lower, right-looking); L^{-1} in place
+// (column sweep from the right, as LAPACK dtrti2); vinv = L^{-T} L^{-1}.
+__global__ __launch_bounds__(256) void k_chol_inv(const double *__restrict__ grams, int nmode,
+                                                  int mode, int R, double diag_add,
+                                                  double *__restrict__ vinv,
+                                                  int *__restrict__ flag,
+                                                  double *__restrict__ phi_out) {
+    extern __shared__ double lds[];
+    double *A = lds;              // R*R
+    double *tmp = lds + R * R;    // R
+    __shared__ int fail;
+    __shared__ double dmax;
+    const int t = threadIdx.x;
+    const int RR = R * R;
+    for (int e = t; e < RR; e += blockDim.x) {
+        double v = 1.0;
+        for (int m = 0; m < nmode; ++m)
+            if (m != mode) v *= grams[(size_t)m * RR + e];
+        if (phi_out) phi_out[e] = v;
+        int i = e / R, j = e % R;
+        A[e] = v + (i == j ? diag_add : 0.0);
+    }
+    if (t == 0) fail = 0;
+    __syncthreads();
+    for (int j = 0; j < R; ++j) {
+        double d = A[j * R + j];
+        if (!(d > 0.0)) {
+            if (t == 0) fail = 1;
+            break;
+        }
+        double l = sqrt(d);
+        __syncthreads();
+        if (t == 0) A[j * R + j] = l;
+        for (int i = j + 1 + t; i < R; i += blockDim.x) A[i * R + j] /= l;
+        __syncthreads();
+        int n = R - j - 1;
+        for (int e = t; e < n * n; e += blockDim.x) {
+            int a = j + 1 + e / n, b = j + 1 + e % n;
+            if (a >= b) A[a * R + b] -= A[a * R + j] * A[b * R + j];
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    if (fail) {
+        // Reference fallback (cpd.hpp:438-476): the unregularised matrix,
+        // symmetrised from its lower triangle, which the reference never
+        // writes off the diagonal; minimum-norm solve of that diagonal.
+        if (t == 0) {
+            double mx = 0.0;
+            for (int r = 0; r < R; ++r) {
+                double v = 1.0;
+                for (int m = 0; m < nmode; ++m)
+                    if (m != mode) v *= grams[(size_t)m * RR + r * R + r];
+                mx = fabs(v) > mx ? fabs(v) : mx;
+            }
+            dmax = mx;
+        }
+        __syncthreads();
+        double tol = dmax * R * DBL_EPSILON;
+        for (int e = t; e < RR; e += blockDim.x) {
+            int i = e / R, j = e % R;
+            double v = 0.0;
+            if (i == j) {
+                double d = 1.0;
+                for (int m = 0; m < nmode; ++m)
+                    if (m != mode) d *= grams[(size_t)m * RR + e];
+                v = fabs(d) > tol ? 1.0 / d : 0.0;
+            }
+            vinv[e] = v;
+        }
+        if (t == 0 && flag) *flag = 1;
+        return;
+    }
+    // In-place inverse of the lower factor.
+    for (int j = R - 1; j >= 0; --j) {
+        if (t == 0) A[j * R + j] = 1.0 / A[j * R + j];
+        for (int k = j + 1 + t; k < R; k += blockDim.x) tmp[k] = A[k * R + j];
+        __syncthreads();
+        double ajj = A[j * R + j];
+        for (int i = j + 1 + t; i < R; i += blockDim.x) {
+            double s = 0.0;
+            for (int k = j + 1; k <= i; ++k) s += A[i * R + k] * tmp[k];
+            A[i * R + j] = -ajj * s;
+        }
+        __syncthreads();
+    }
+    for (int e = t; e < RR; e += blockDim.x) {
+        int a = e / R, b = e % R;
+        int k0 = a > b ? a : b;
+        double s = 0.0;
+        for (int k = k0; k < R; ++k) s += A[k * R + a] * A[k * R + b];
+        vinv[e] = s;
+    }
+    if (t == 0 && flag) *flag = 0;
+}
+
+int launch_chol_inv(const double *grams, int nmode, int mode, int R, double diag_add,
+                    bool copy_first, double *vinv, int *flag, double *phi_out, hipStream_t s) {
+    (void)copy_first;  // 1.0 * G == G exactly: both Hadamard orders agree bitwise
+    if (R < 1 || R > kMaxRank) return ALTO_ERR_UNSUPPORTED;
+    size_t lds = sizeof(double) * ((size_t)R * R + R);
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)k_chol_inv,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 256);
+        attr = true;
+    }
+    k_chol_inv<<<1, 256, lds, s>>>(grams, nmode, mode, R, diag_add, vinv, flag, phi_out);
+    ALTO_LAUNCH_CHECK();
+    return ALTO_OK;
+}
+
+static inline int pow2_at_least(int r) {
+    int p = 1;
+    while (p < r) p <<= 1;
+    return p;
+}
+
+// u = M * vinv (in place), with per-block column partials.
+__global__ __launch_bounds__(256) void k_solve(double *__restrict__ M, uint64_t rows, int R,
+                                               int RP, const double *__restrict__ vinv,
+                                               double *__restrict__ partials) {
+    extern __shared__ double lds[];
+    double *W = lds;              // R*R
+    double *T = lds + R * R;      // tile: TR rows x R
+    const int t = threadIdx.x;
+    const int lanes = 256 / RP;   // rows in flight per pass
+    const int TR = lanes * 4;
+    const int c = t % RP, rl = t / RP;
+    for (int e = t; e < R * R; e += 256) W[e] = vinv[e];
+    double ss = 0.0, mx = -INFINITY, ip = 0.0;
+    uint64_t ntiles = (rows + TR - 1) / TR;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        uint64_t r0 = tile * TR;
+        int nr = (int)((rows - r0) < (uint64_t)TR ? (rows - r0) : TR);
+        __syncthreads();
+        for (int e = t; e < nr * R; e += 256) T[e] = M[r0 * R + e];
+        __syncthreads();
+        if (c < R) {
+            for (int q = 0; q < 4; ++q) {
+                int rr = rl + q * lanes;
+                if (rr < nr) {
+                    const double *x = T + rr * R;
+                    double s = 0.0;
+                    for (int k = 0; k < R; ++k) s += x[k] * W[k * R + c];
+                    M[(r0 + rr) * R + c] = s;
+                    ss += s * s;
+                    mx = s > mx ? s : mx;
+                    ip += s * x[c];
+                }
+            }
+        }
+    }
+    __syncthreads();
+    double *red = T;  // reuse: 3 * 256 doubles
+    red[t] = ss;
+    red[256 + t] = mx;
+    red[512 + t] = ip;
+    __syncthreads();
+    if (rl == 0 && c < R) {
+        for (int l = 1; l < lanes; ++l) {
+            ss += red[l * RP + c];
+            double m2 = red[256 + l * RP + c];
+            mx = m2 > mx ? m2 : mx;
+            ip += red[512 + l * RP + c];
+        }
+        double *p = partials + (size_t)blockIdx.x * 3 * R;
+        p[c] = ss;
+        p[R + c] = mx;
+        p[2 * R + c] = ip;
+    }
+}
+
+int launch_solve(double *M, uint64_t rows, int R, const double *vinv, double *partials, int nblk,
+                 hipStream_t s) {
+    if (R > kMaxRank) return ALTO_ERR_UNSUPPORTED;
+    int RP = pow2_at_least(R);
+    int TR = (256 / RP) * 4;
+    size_t tile = (size_t)TR * R;
+    if (tile < 768) tile = 768;
+    size_t lds = sizeof(double) * ((size_t)R * R + tile);
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)k_solve, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024 - 256);
+        attr = true;
+    }
+    k_solve<<<nblk, 256, lds, s>>>(M, rows, R, RP, vinv, partials);
+    ALTO_LAUNCH_CHECK();
+    return ALTO_OK;
+}
+
+__global__ void k_lambda(const double *__restrict__ partials, int nblk, int R, int which,
+                         double *__restrict__ lambda) {
+    int c = threadIdx.x;
+    if (c >= R) return;
+    if (which == 0) {
+        double s = 0.0;
+        for (int b = 0; b < nblk; ++b) s += partials[(size_t)b * 3 * R + c];
+        lambda[c] = sqrt(s > 1e-12 ? s : 1e-12);
+    } else {
+        double m = 1.0;
+        for (int b = 0; b < nblk; ++b) {
+            double v = partials[(size_t)b * 3 * R + R + c];
+            m = v > m ? v : m;
+        }
+        lambda[c] = m;
+    }
+}
+
+int launch_lambda(const double *partials, int nblk, int R, int which, double *lambda,
+                  hipStream_t s) {
+    k_lambda<<<1, 256, 0, s>>>(partials, nblk, R, which, lambda);
+    ALTO_LAUNCH_CHECK();
+    return ALTO_OK;
+}
+
+template <int MODE>  // 0: scale f64 u by 1/lambda and store; 1: read stored factor
+__global__ __launch_bounds__(256) void k_scale_gram(const double *__restrict__ u,
+                                                    const double *__restrict__ lambda,
+                                                    const void *__restrict__ Uin,
+                                                    void *__restrict__ Uout, int prec,
+                                                    uint64_t rows, int R,
+                                                    double *__restrict__ gpart) {
+    extern __shared__ double T[];  // TR x R
+    constexpr int TR = 32;
+    const int t = threadIdx.x;
+    const int RR = R * R;
+    const int nq = (RR + 255) / 256;
+    double g[64];
+#pragma unroll
+    for (int q = 0; q < 64; ++q) g[q] = 0.0;
+    uint64_t ntiles = (rows + TR - 1) / TR;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        uint64_t r0 = tile * TR;
+        int nr = (int)((rows - r0) < (uint64_t)TR ? (rows - r0) : TR);
+        __syncthreads();
+        for (int e = t; e < nr * R; e += 256) {
+            uint64_t gi = r0 * R + e;
+            double v;
+            if (MODE == 0) {
+                double x = u[gi];
+                if (lambda) x = x / lambda[e % R];
+                if (prec == ALTO_PREC_F64) {
+                    ((double *)Uout)[gi] = x;
+                    v = x;
+                } else if (prec == ALTO_PREC_F32) {
+                    float f = (float)x;
+                    ((float *)Uout)[gi] = f;
+                    v = (double)f;
+                } else {
+                    uint16_t b = f32_to_bf16((float)x);
+                    ((uint16_t *)Uout)[gi] = b;
+                    v = (double)bf16_to_f32(b);
+                }
+            } else {
+                if (prec == ALTO_PREC_F64) v = ((const double *)Uin)[gi];
+                else if (prec == ALTO_PREC_F32) v = (double)((const float *)Uin)[gi];
+                else v = (double)bf16_to_f32(((const uint16_t *)Uin)[gi]);
+            }
+            T[e] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 64; ++q) {
+            if (q < nq) {
+                int e = t + q * 256;
+                if (e < RR) {
+                    int a = e / R, b = e % R;
+                    double s = g[q];
+                    for (int r = 0; r < nr; ++r) s += T[r * R + a] * T[r * R + b];
+                    g[q] = s;
+                }
+            }
+        }
+    }
+    double *p = gpart + (size_t)blockIdx.x * RR;
+#pragma unroll
+    for (int q = 0; q < 64; ++q) {
+        if (q < nq) {
+            int e = t + q * 256;
+            if (e < RR) p[e] = g[q];
+        }
+    }
+}
+
+int launch_scale_gram(const double *u, const double *lambda, uint64_t rows, int R, void *U,
+                      int prec, double *gpart, int nblk, hipStream_t s) {
+    if (R > kMaxRank) return ALTO_ERR_UNSUPPORTED;
+    size_t lds = sizeof(double) * 32 * R;
+    k_scale_gram<0><<<nblk, 256, lds, s>>>(u, lambda, nullptr, U, prec, rows, R, gpart);
+    ALTO_LAUNCH_CHECK();
+    return ALTO_OK;
+}
+
+int launch_gram(const void *U, int prec, uint64_t rows, int R, double *gpart, int nblk,
+                hipStream_t s) {
+    if (R > kMaxRank) return ALTO_ERR_UNSUPPORTED;
+    size_t lds = sizeof(double) * 32 * R;
+    k_scale_gram<1><<<nblk, 256, lds, s>>>(nullptr, nullptr, U, nullptr, prec, rows, R, gpart);
+    ALTO_LAUNCH_CHECK();
+    return ALTO_OK;
+}
+
+__global__ void k_gram_reduce(const double *__restrict__ gpart, int nblk, int RR,
+                              double *__restrict__ G) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= RR) return;
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += gpart[(size_t)b * RR + e];
+    G[e] = s;
+}
+
+int launch_gram_reduce(const double *gpart, int nblk, int R, double *G, hipStream_t s) {
+    int RR = R * R;
+    k_gram_reduce<<<cdiv(RR, 256), 256, 0, s>>>(gpart, nblk, RR, G);
+    ALTO_LAUNCH_CHECK();
+    return ALTO_OK;
+}
+
+__global__ __launch_bounds__(256) void k_sumsq_part(const double *__restrict__ v, uint64_t n,
+                                                    double *__restrict__ part) {
+    __shared__ double sh[4];
+    double s = 0.0;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
+        s += v[i] * v[i];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+__global__ void k_sum_final(const double *__restrict__ part, int n, double *__restrict__ out) {
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int b = 0; b < n; ++b) s += part[b];
+        *out = s;
+    }
+}
+
+int launch_sumsq(const double *v, uint64_t n, double *out, hipStream_t s) {
+    DevBuf part;
+    ALTO_TRY(part.alloc(sizeof(double) * 1024));
+    k_sumsq_part<<<1024, 256, 0, s>>>(v, n, part.as<double>());
+    k_sum_final<<<1, 64, 0, s>>>(part.as<double>(), 1024, out);
+    ALTO_LAUNCH_CHECK();
+    ALTO_HIP_TRY(hipStreamSynchronize(s));
+    return ALTO_OK;
+}
+
+}  // namespace alto

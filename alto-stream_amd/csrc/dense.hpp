@@ -1,0 +1,37 @@
+// Dense R x R / I x R kernels shared by CP-ALS and CP-stream.
+#pragma once
+#include "common.hpp"
+
+namespace alto {
+
+constexpr int kDenseThreads = 256;
+constexpr int kMaxBlocks = 512;   // row blocks per reduction (partials buffer size)
+constexpr int kMaxRank = 128;
+
+// Launch helpers (all FP64 R x R math, row-major).
+// grams: nmode consecutive R x R matrices.
+int launch_fill_zero(double *p, uint64_t n, hipStream_t s);
+// vinv <- inverse of (Hadamard_{m != mode} G_m + diag_add I) via Cholesky; on a
+// failed pivot, the diagonal minimum-norm inverse of the unregularised matrix
+// (reference fallback); flag[0] = 1 when the fallback ran.
+int launch_chol_inv(const double *grams, int nmode, int mode, int R, double diag_add,
+                    bool copy_first, double *vinv, int *flag, double *phi_out, hipStream_t s);
+int row_blocks(uint64_t rows);
+// u = M * vinv in place; partials[b][0..R) = sum u^2, [R..2R) = max u,
+// [2R..3R) = sum u * M (per row block b).
+int launch_solve(double *M, uint64_t rows, int R, const double *vinv, double *partials,
+                 int nblk, hipStream_t s);
+// lambda from partials: which 0 = 2-norm, 1 = max-norm.
+int launch_lambda(const double *partials, int nblk, int R, int which, double *lambda,
+                  hipStream_t s);
+// Stores U = u / lambda (or u when lambda == nullptr) in precision prec,
+// and writes Gram partials of the stored values.
+int launch_scale_gram(const double *u, const double *lambda, uint64_t rows, int R, void *U,
+                      int prec, double *gpart, int nblk, hipStream_t s);
+// Gram partials of an existing factor stored in precision prec.
+int launch_gram(const void *U, int prec, uint64_t rows, int R, double *gpart, int nblk,
+                hipStream_t s);
+int launch_gram_reduce(const double *gpart, int nblk, int R, double *G, hipStream_t s);
+int launch_sumsq(const double *v, uint64_t n, double *out, hipStream_t s);
+
+}  // namespace alto

@@ -1,0 +1,470 @@
+// ALTO tensor construction on the GPU: masks, linearisation, key sort,
+// line partitions, and the per-mode output-ordered schedules.
+//
+// Reference: create_alto (include/alto.hpp:436-582), setup_packed_alto
+// (954-1057), sort_alto (1059-1090), prtn_alto (1128-1226).
+#include <algorithm>
+#include <string.h>
+
+#include "tensor.hpp"
+
+namespace alto {
+
+int mode_bits(uint64_t dim) {
+    // util.hpp:50-59: 64 - clz(dim - 1); a size-1 mode gets 0 bits.
+    if (dim <= 1) return 0;
+    return 64 - __builtin_clzll(dim - 1);
+}
+
+int build_masks(int nmode, const uint64_t *dims, int kw, uint64_t *masks, BitPos *bp) {
+    int bits[kMaxModes], order[kMaxModes];
+    int total = 0, maxb = 0;
+    for (int n = 0; n < nmode; ++n) {
+        bits[n] = mode_bits(dims[n]);
+        total += bits[n];
+        maxb = std::max(maxb, bits[n]);
+        order[n] = n;
+    }
+    if (total > 64 * kw) return -1;
+    // Modes ordered by bit count, ties in mode order (SHORT_FIRST).
+    std::stable_sort(order, order + nmode, [&](int a, int b) { return bits[a] < bits[b]; });
+    memset(masks, 0, sizeof(uint64_t) * nmode * kw);
+    memset(bp, 0, sizeof(*bp));
+    bp->nmode = nmode;
+    int next = 0;  // next key bit, LSB first
+    for (int level = 0; level < maxb; ++level)
+        for (int k = 0; k < nmode; ++k) {
+            int m = order[k];
+            if (level < bits[m]) {
+                masks[m * kw + next / 64] |= 1ULL << (next % 64);
+                bp->pos[m][level] = (uint8_t)next;
+                ++next;
+            }
+        }
+    for (int m = 0; m < nmode; ++m) bp->bits[m] = bits[m];
+    return total;
+}
+
+struct IdxPtrs {
+    const uint64_t *p[kMaxModes];
+};
+
+// key = OR_m pdep(cidx[m][i], mask_m); flags out-of-range coordinates.
+__global__ __launch_bounds__(256) void k_linearize(IdxPtrs ci, uint64_t n, BitPos bp,
+                                                   DimsArg dims, uint64_t *__restrict__ k0,
+                                                   uint64_t *__restrict__ k1,
+                                                   uint32_t *__restrict__ bad) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t lo = 0, hi = 0;
+    bool oob = false;
+    for (int m = 0; m < bp.nmode; ++m) {
+        uint64_t x = ci.p[m][i];
+        oob |= x >= dims.d[m];
+        for (int b = 0; b < bp.bits[m]; ++b) {
+            if ((x >> b) & 1ULL) {
+                int p = bp.pos[m][b];
+                if (p < 64) lo |= 1ULL << p;
+                else hi |= 1ULL << (p - 64);
+            }
+        }
+    }
+    k0[i] = lo;
+    if (k1) k1[i] = hi;
+    if (oob) atomicOr(bad, 1u);
+}
+
+__device__ __forceinline__ uint64_t pext_pos(uint64_t lo, uint64_t hi, const BitPos &bp, int m) {
+    uint64_t x = 0;
+    for (int b = 0; b < bp.bits[m]; ++b) {
+        int p = bp.pos[m][b];
+        uint64_t bit = p < 64 ? (lo >> p) & 1ULL : (hi >> (p - 64)) & 1ULL;
+        x |= bit << b;
+    }
+    return x;
+}
+
+// Interleaved ALTO keys, packed keys and gathered FP64 values in sorted order.
+__global__ __launch_bounds__(256) void k_finish(const uint64_t *__restrict__ k0,
+                                                const uint64_t *__restrict__ k1,
+                                                const uint32_t *__restrict__ perm,
+                                                const double *__restrict__ vin, uint64_t n,
+                                                BitPos bp, PackInfo pk, int kw, int pkw,
+                                                uint64_t *__restrict__ keys,
+                                                uint64_t *__restrict__ pkeys,
+                                                double *__restrict__ vout) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t lo = k0[i], hi = k1 ? k1[i] : 0;
+    keys[i * kw] = lo;
+    if (kw == 2) keys[i * kw + 1] = hi;
+    uint64_t plo = 0, phi = 0;
+    for (int m = 0; m < bp.nmode; ++m) {
+        uint64_t x = pext_pos(lo, hi, bp, m);
+        int s = pk.shift[m], w = pk.width[m];
+        if (w == 0) continue;
+        if (s >= 64) {
+            phi |= x << (s - 64);
+        } else {
+            plo |= x << s;
+            if (s + w > 64) phi |= x >> (64 - s);
+        }
+    }
+    pkeys[i * pkw] = plo;
+    if (pkw == 2) pkeys[i * pkw + 1] = phi;
+    vout[i] = vin[perm[i]];
+}
+
+struct Chunk {
+    uint64_t start, end;
+    uint32_t part;
+};
+
+// Per-partition [min, max] of every mode index (prtn_alto, alto.hpp:1199-1226).
+__global__ __launch_bounds__(256) void k_intervals(const uint64_t *__restrict__ pkeys, int pkw,
+                                                   PackInfo pk, const Chunk *__restrict__ chunks,
+                                                   unsigned long long *__restrict__ iv) {
+    __shared__ unsigned long long smin[kMaxModes][4], smax[kMaxModes][4];
+    Chunk c = chunks[blockIdx.x];
+    unsigned long long lo[kMaxModes], hi[kMaxModes];
+    for (int m = 0; m < pk.nmode; ++m) {
+        lo[m] = ~0ULL;
+        hi[m] = 0;
+    }
+    for (uint64_t i = c.start + threadIdx.x; i < c.end; i += blockDim.x) {
+        uint64_t a = pkeys[i * pkw], b = pkw == 2 ? pkeys[i * pkw + 1] : 0;
+        for (int m = 0; m < pk.nmode; ++m) {
+            unsigned long long x = extract_field(a, b, pk.shift[m], pk.width[m]);
+            lo[m] = x < lo[m] ? x : lo[m];
+            hi[m] = x > hi[m] ? x : hi[m];
+        }
+    }
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int m = 0; m < pk.nmode; ++m) {
+        unsigned long long a = lo[m], b = hi[m];
+        for (int o = 32; o > 0; o >>= 1) {
+            unsigned long long a2 = __shfl_down(a, o, 64), b2 = __shfl_down(b, o, 64);
+            a = a2 < a ? a2 : a;
+            b = b2 > b ? b2 : b;
+        }
+        if (lane == 0) {
+            smin[m][wid] = a;
+            smax[m][wid] = b;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < (unsigned)pk.nmode) {
+        int m = threadIdx.x;
+        unsigned long long a = smin[m][0], b = smax[m][0];
+        for (int w = 1; w < 4; ++w) {
+            a = smin[m][w] < a ? smin[m][w] : a;
+            b = smax[m][w] > b ? smax[m][w] : b;
+        }
+        atomicMin(&iv[((uint64_t)c.part * pk.nmode + m) * 2], a);
+        atomicMax(&iv[((uint64_t)c.part * pk.nmode + m) * 2 + 1], b);
+    }
+}
+
+__global__ void k_init_intervals(unsigned long long *iv, uint64_t nprtn, int nmode) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nprtn * nmode) {
+        iv[2 * i] = ~0ULL;
+        iv[2 * i + 1] = 0;
+    }
+}
+
+__global__ void k_to_f32(float *d, const double *s, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = (float)s[i];
+}
+
+int ensure_vals32(TensorImpl *t, hipStream_t s) {
+    if (t->vals32.p) return ALTO_OK;
+    ALTO_TRY(t->vals32.alloc(sizeof(float) * t->nnz));
+    if (t->nnz) {
+        k_to_f32<<<cdiv(t->nnz, 256), 256, 0, s>>>(t->vals32.as<float>(), t->vals64.as<double>(),
+                                                  t->nnz);
+        ALTO_LAUNCH_CHECK();
+    }
+    return ALTO_OK;
+}
+
+__global__ __launch_bounds__(256) void k_mode_index(const uint64_t *__restrict__ pkeys, int pkw,
+                                                    PackInfo pk, int mode, uint64_t n,
+                                                    uint64_t *__restrict__ out) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t a = pkeys[i * pkw], b = pkw == 2 ? pkeys[i * pkw + 1] : 0;
+    out[i] = extract_field(a, b, pk.shift[mode], pk.width[mode]);
+}
+
+__global__ __launch_bounds__(256) void k_gather_sched(const uint64_t *__restrict__ pkeys, int pkw,
+                                                      const double *__restrict__ v64,
+                                                      const uint32_t *__restrict__ perm, uint64_t n,
+                                                      uint64_t *__restrict__ opk,
+                                                      float *__restrict__ o32,
+                                                      double *__restrict__ o64) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t j = perm[i];
+    opk[i * pkw] = pkeys[(uint64_t)j * pkw];
+    if (pkw == 2) opk[i * pkw + 1] = pkeys[(uint64_t)j * pkw + 1];
+    double v = v64[j];
+    o32[i] = (float)v;
+    if (o64) o64[i] = v;
+}
+
+// Output-ordered copy of the tensor for one mode: stable sort of the ALTO
+// order by that mode's index (the order the reference's op_sched variant
+// walks through oidx, alto.hpp:1708-1801).
+int ensure_schedule(TensorImpl *t, int mode, bool need64, hipStream_t s) {
+    ModeSchedule &sc = t->sched[mode];
+    if (sc.built && (!need64 || sc.vals64.p)) return ALTO_OK;
+    uint64_t n = t->nnz;
+    ALTO_TRY(sc.pkeys.alloc(sizeof(uint64_t) * t->pkw * n));
+    ALTO_TRY(sc.vals32.alloc(sizeof(float) * n));
+    if (need64) ALTO_TRY(sc.vals64.alloc(sizeof(double) * n));
+    if (n == 0) {
+        sc.built = true;
+        return ALTO_OK;
+    }
+    DevBuf idx, perm;
+    ALTO_TRY(idx.alloc(sizeof(uint64_t) * n));
+    ALTO_TRY(perm.alloc(sizeof(uint32_t) * n));
+    k_mode_index<<<cdiv(n, 256), 256, 0, s>>>(t->pkeys.as<uint64_t>(), t->pkw, t->pack, mode, n,
+                                              idx.as<uint64_t>());
+    ALTO_LAUNCH_CHECK();
+    ALTO_TRY(iota_u32(perm.as<uint32_t>(), n, s));
+    ALTO_TRY(radix_sort_pairs(idx.as<uint64_t>(), nullptr, perm.as<uint32_t>(), n,
+                              t->pack.width[mode], t->sort_ws, s));
+    k_gather_sched<<<cdiv(n, 256), 256, 0, s>>>(t->pkeys.as<uint64_t>(), t->pkw,
+                                                t->vals64.as<double>(), perm.as<uint32_t>(), n,
+                                                sc.pkeys.as<uint64_t>(), sc.vals32.as<float>(),
+                                                need64 ? sc.vals64.as<double>() : nullptr);
+    ALTO_LAUNCH_CHECK();
+    ALTO_HIP_TRY(hipStreamSynchronize(s));
+    sc.built = true;
+    return ALTO_OK;
+}
+
+}  // namespace alto
+
+using namespace alto;
+
+extern "C" {
+
+int alto_hip_tensor_create(const alto_hip_coo *coo, int nprtn, int key_words, void *stream,
+                           alto_hip_tensor *out) {
+    if (!coo || !out || !coo->dims || !coo->cidx || !coo->vals) return ALTO_ERR_ARG;
+    if (coo->nmode < 1 || coo->nmode > kMaxModes || nprtn < 1) return ALTO_ERR_ARG;
+    if (coo->nnz >= (1ULL << 32)) {
+        set_error("nnz %llu exceeds 2^32 - 1", (unsigned long long)coo->nnz);
+        return ALTO_ERR_UNSUPPORTED;
+    }
+    for (int m = 0; m < coo->nmode; ++m)
+        if (coo->dims[m] == 0 || (coo->nnz && !coo->cidx[m])) return ALTO_ERR_ARG;
+    ALTO_TRY(ensure_init());
+    hipStream_t s = to_stream(stream);
+
+    auto holder = new alto_hip_tensor_s;
+    TensorImpl *t = &holder->impl;
+    t->nmode = coo->nmode;
+    t->nnz = coo->nnz;
+    t->nprtn = nprtn;
+    memcpy(t->dims, coo->dims, sizeof(uint64_t) * t->nmode);
+    int total = 0;
+    for (int m = 0; m < t->nmode; ++m) total += mode_bits(t->dims[m]);
+    int kw = key_words == 0 ? (total <= 64 ? 1 : 2) : key_words;
+    if (kw != 1 && kw != 2) {
+        delete holder;
+        return ALTO_ERR_ARG;
+    }
+    if (build_masks(t->nmode, t->dims, kw, t->masks, &t->bp) < 0) {
+        set_error("ALTO key needs %d bits, key holds %d", total, 64 * kw);
+        delete holder;
+        return ALTO_ERR_KEY_WIDTH;
+    }
+    t->kw = kw;
+    t->total_bits = total;
+    t->pkw = total <= 64 ? 1 : 2;
+    t->pack.nmode = t->nmode;
+    int sh = 0;
+    for (int m = 0; m < t->nmode; ++m) {
+        t->pack.shift[m] = sh;
+        t->pack.width[m] = mode_bits(t->dims[m]);
+        sh += t->pack.width[m];
+    }
+    uint64_t n = t->nnz;
+    int st = ALTO_OK;
+    do {
+        if ((st = t->keys.alloc(sizeof(uint64_t) * kw * n))) break;
+        if ((st = t->pkeys.alloc(sizeof(uint64_t) * t->pkw * n))) break;
+        if ((st = t->perm.alloc(sizeof(uint32_t) * n))) break;
+        if ((st = t->vals64.alloc(sizeof(double) * n))) break;
+        DevBuf dci[kMaxModes], dvals, k0, k1, bad;
+        IdxPtrs ip{};
+        const double *vin = coo->vals;
+        if (!coo->on_device) {
+            for (int m = 0; m < t->nmode && !st; ++m) {
+                if ((st = dci[m].alloc(sizeof(uint64_t) * n))) break;
+                if (n) {
+                    hipError_t e = hipMemcpyAsync(dci[m].p, coo->cidx[m], sizeof(uint64_t) * n,
+                                                  hipMemcpyHostToDevice, s);
+                    if (e != hipSuccess) { set_error("H2D: %s", hipGetErrorString(e)); st = ALTO_ERR_HIP; }
+                }
+                ip.p[m] = dci[m].as<uint64_t>();
+            }
+            if (st) break;
+            if ((st = dvals.alloc(sizeof(double) * n))) break;
+            if (n && hipMemcpyAsync(dvals.p, coo->vals, sizeof(double) * n, hipMemcpyHostToDevice,
+                                    s) != hipSuccess) {
+                st = ALTO_ERR_HIP;
+                break;
+            }
+            vin = dvals.as<double>();
+        } else {
+            for (int m = 0; m < t->nmode; ++m) ip.p[m] = coo->cidx[m];
+        }
+        if ((st = k0.alloc(sizeof(uint64_t) * n))) break;
+        if (kw == 2 && (st = k1.alloc(sizeof(uint64_t) * n))) break;
+        if ((st = bad.alloc(sizeof(uint32_t)))) break;
+        if (hipMemsetAsync(bad.p, 0, sizeof(uint32_t), s) != hipSuccess) { st = ALTO_ERR_HIP; break; }
+        DimsArg da{};
+        for (int m = 0; m < t->nmode; ++m) da.d[m] = t->dims[m];
+        if (n) {
+            k_linearize<<<cdiv(n, 256), 256, 0, s>>>(ip, n, t->bp, da, k0.as<uint64_t>(),
+                                                     kw == 2 ? k1.as<uint64_t>() : nullptr,
+                                                     bad.as<uint32_t>());
+            if ((st = iota_u32(t->perm.as<uint32_t>(), n, s))) break;
+            if ((st = radix_sort_pairs(k0.as<uint64_t>(), kw == 2 ? k1.as<uint64_t>() : nullptr,
+                                       t->perm.as<uint32_t>(), n, total, t->sort_ws, s)))
+                break;
+            k_finish<<<cdiv(n, 256), 256, 0, s>>>(k0.as<uint64_t>(),
+                                                  kw == 2 ? k1.as<uint64_t>() : nullptr,
+                                                  t->perm.as<uint32_t>(), vin, n, t->bp, t->pack,
+                                                  kw, t->pkw, t->keys.as<uint64_t>(),
+                                                  t->pkeys.as<uint64_t>(), t->vals64.as<double>());
+            if (hipGetLastError() != hipSuccess) { st = ALTO_ERR_HIP; break; }
+        }
+        uint32_t hbad = 0;
+        if (hipMemcpyAsync(&hbad, bad.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess) {
+            set_error("ALTO encode failed");
+            st = ALTO_ERR_HIP;
+            break;
+        }
+        if (hbad) {
+            set_error("a coordinate is not below its mode size");
+            st = ALTO_ERR_ARG;
+            break;
+        }
+        // prtn_alto: nnz_ptrn = ceil(nnz / nprtn) (alto.hpp:1134-1161)
+        t->prtn_ptr.assign(nprtn + 1, 0);
+        uint64_t per = (n + nprtn - 1) / nprtn;
+        std::vector<Chunk> chunks;
+        const uint64_t CH = 1 << 16;
+        for (int p = 0; p < nprtn; ++p) {
+            uint64_t a = std::min<uint64_t>((uint64_t)p * per, n);
+            uint64_t b = std::min<uint64_t>((uint64_t)(p + 1) * per, n);
+            t->prtn_ptr[p + 1] = b;
+            for (uint64_t c = a; c < b; c += CH) chunks.push_back({c, std::min(c + CH, b), (uint32_t)p});
+        }
+        DevBuf div, dch;
+        if ((st = div.alloc(sizeof(uint64_t) * 2 * nprtn * t->nmode))) break;
+        k_init_intervals<<<cdiv((uint64_t)nprtn * t->nmode, 256), 256, 0, s>>>(
+            div.as<unsigned long long>(), nprtn, t->nmode);
+        if (!chunks.empty()) {
+            if ((st = dch.alloc(sizeof(Chunk) * chunks.size()))) break;
+            if (hipMemcpyAsync(dch.p, chunks.data(), sizeof(Chunk) * chunks.size(),
+                               hipMemcpyHostToDevice, s) != hipSuccess) { st = ALTO_ERR_HIP; break; }
+            k_intervals<<<(unsigned)chunks.size(), 256, 0, s>>>(t->pkeys.as<uint64_t>(), t->pkw,
+                                                                t->pack, dch.as<Chunk>(),
+                                                                div.as<unsigned long long>());
+        }
+        t->intervals.assign((size_t)2 * nprtn * t->nmode, 0);
+        if (hipMemcpyAsync(t->intervals.data(), div.p, sizeof(uint64_t) * t->intervals.size(),
+                           hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess) {
+            set_error("partition intervals failed");
+            st = ALTO_ERR_HIP;
+            break;
+        }
+        for (int p = 0; p < nprtn; ++p)
+            for (int m = 0; m < t->nmode; ++m) {
+                uint64_t *lo = &t->intervals[((size_t)p * t->nmode + m) * 2];
+                if (t->prtn_ptr[p] == t->prtn_ptr[p + 1]) {
+                    lo[0] = 0;  // empty partition: [0, 0] (alto.hpp:1220-1225)
+                    lo[1] = 0;
+                } else if (lo[0] > lo[1]) {
+                    lo[0] = lo[1];
+                }
+            }
+    } while (0);
+    if (st != ALTO_OK) {
+        delete holder;
+        return st;
+    }
+    *out = holder;
+    return ALTO_OK;
+}
+
+int alto_hip_tensor_destroy(alto_hip_tensor t) {
+    delete t;
+    return ALTO_OK;
+}
+
+int alto_hip_tensor_info(alto_hip_tensor t, int *nmode, uint64_t *nnz, int *key_words, int *nprtn,
+                         int *total_bits) {
+    if (!t) return ALTO_ERR_ARG;
+    if (nmode) *nmode = t->impl.nmode;
+    if (nnz) *nnz = t->impl.nnz;
+    if (key_words) *key_words = t->impl.kw;
+    if (nprtn) *nprtn = t->impl.nprtn;
+    if (total_bits) *total_bits = t->impl.total_bits;
+    return ALTO_OK;
+}
+
+int alto_hip_tensor_dims(alto_hip_tensor t, uint64_t *dims) {
+    if (!t || !dims) return ALTO_ERR_ARG;
+    memcpy(dims, t->impl.dims, sizeof(uint64_t) * t->impl.nmode);
+    return ALTO_OK;
+}
+
+int alto_hip_tensor_masks(alto_hip_tensor t, uint64_t *masks) {
+    if (!t || !masks) return ALTO_ERR_ARG;
+    memcpy(masks, t->impl.masks, sizeof(uint64_t) * t->impl.nmode * t->impl.kw);
+    return ALTO_OK;
+}
+
+int alto_hip_tensor_keys(alto_hip_tensor t, uint64_t *keys) {
+    if (!t || !keys) return ALTO_ERR_ARG;
+    return alto_hip_memcpy_d2h(keys, t->impl.keys.p, sizeof(uint64_t) * t->impl.kw * t->impl.nnz,
+                               nullptr);
+}
+
+int alto_hip_tensor_vals(alto_hip_tensor t, double *vals) {
+    if (!t || !vals) return ALTO_ERR_ARG;
+    return alto_hip_memcpy_d2h(vals, t->impl.vals64.p, sizeof(double) * t->impl.nnz, nullptr);
+}
+
+int alto_hip_tensor_perm(alto_hip_tensor t, uint32_t *perm) {
+    if (!t || !perm) return ALTO_ERR_ARG;
+    return alto_hip_memcpy_d2h(perm, t->impl.perm.p, sizeof(uint32_t) * t->impl.nnz, nullptr);
+}
+
+int alto_hip_tensor_partitions(alto_hip_tensor t, uint64_t *prtn_ptr, uint64_t *intervals) {
+    if (!t) return ALTO_ERR_ARG;
+    if (prtn_ptr) memcpy(prtn_ptr, t->impl.prtn_ptr.data(), sizeof(uint64_t) * t->impl.prtn_ptr.size());
+    if (intervals)
+        memcpy(intervals, t->impl.intervals.data(), sizeof(uint64_t) * t->impl.intervals.size());
+    return ALTO_OK;
+}
+
+int alto_hip_tensor_device_arrays(alto_hip_tensor t, const uint64_t **keys, const double **vals) {
+    if (!t) return ALTO_ERR_ARG;
+    if (keys) *keys = t->impl.keys.as<uint64_t>();
+    if (vals) *vals = t->impl.vals64.as<double>();
+    return ALTO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,363 @@
+// ALTO MTTKRP on gfx950.
+//
+// Reference: mttkrp_alto_par and its variants (include/alto.hpp:826-908,
+// 1349-1407 _atomic, 1544-1634 _da_mem_pull, 1636-1706 _stream,
+// 1708-1801 _op_sched).  Per nonzero: row = val * prod_{m != n}
+// A_m[idx_m(key)], accumulated into A_n[idx_n(key)] (out +=).
+//
+// GPU mapping: a lane group of L = R / V lanes owns one output row slice of
+// V elements per lane (V = 16 bytes / factor element size), so every factor
+// row gather is one 16-byte load per lane and a wave moves 64/L rows per
+// instruction.  Each group walks a contiguous range of nonzeros in batches
+// of B, issuing the B gathers of a mode back to back, and keeps a running
+// sum while the output index repeats; the sum is flushed with a hardware
+// atomic add only when the output index changes and at the end of the range.
+//   algo 1: nonzeros in ALTO order (runs come from duplicates / dense regions)
+//   algo 2: nonzeros in the mode's output order (ALTO order inside each row;
+//           the oidx order of the op_sched variant) -> one flush per row per
+//           group, atomics only where a row crosses a group boundary.
+#include <algorithm>
+
+#include "tensor.hpp"
+
+namespace alto {
+
+template <typename FT> struct FVec;
+template <> struct FVec<double> {
+    static constexpr int V = 2;
+    using CT = double;
+    __device__ __forceinline__ static void load(const double *p, double *o) {
+        double2 v = *(const double2 *)p;
+        o[0] = v.x;
+        o[1] = v.y;
+    }
+};
+template <> struct FVec<float> {
+    static constexpr int V = 4;
+    using CT = float;
+    __device__ __forceinline__ static void load(const float *p, float *o) {
+        float4 v = *(const float4 *)p;
+        o[0] = v.x;
+        o[1] = v.y;
+        o[2] = v.z;
+        o[3] = v.w;
+    }
+};
+template <> struct FVec<uint16_t> {  // BF16 storage, FP32 compute
+    static constexpr int V = 8;
+    using CT = float;
+    __device__ __forceinline__ static void load(const uint16_t *p, float *o) {
+        uint4 v = *(const uint4 *)p;
+        uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            o[2 * q] = __uint_as_float(w[q] << 16);
+            o[2 * q + 1] = __uint_as_float(w[q] & 0xffff0000u);
+        }
+    }
+};
+
+template <typename FT> struct FPtrs {
+    const FT *p[kMaxModes];
+};
+
+__device__ __forceinline__ void atomic_add_out(double *p, double v) { unsafeAtomicAdd(p, v); }
+__device__ __forceinline__ void atomic_add_out(float *p, float v) { unsafeAtomicAdd(p, v); }
+
+template <int PKW>
+__device__ __forceinline__ uint64_t field(uint64_t lo, uint64_t hi, int sh, int w) {
+    if (PKW == 1) return w >= 64 ? lo >> sh : (lo >> sh) & ((1ULL << w) - 1ULL);
+    return extract_field(lo, hi, sh, w);
+}
+
+template <typename VT, typename FT, typename OT, int R, int PKW, int B>
+__global__ __launch_bounds__(256) void k_mttkrp_seg(const uint64_t *__restrict__ pkeys,
+                                                    const VT *__restrict__ vals, uint64_t nnz,
+                                                    FPtrs<FT> F, OT *__restrict__ out,
+                                                    PackInfo pk, int target, uint64_t per_group) {
+    using Vec = FVec<FT>;
+    using CT = typename Vec::CT;
+    constexpr int V = Vec::V;
+    constexpr int L = R / V;
+    static_assert(R % V == 0 && L >= 1 && L <= 64 && (64 % L) == 0, "bad rank tiling");
+
+    const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t group = gtid / L;
+    const int lig = (int)(gtid % L);
+    uint64_t start = group * per_group;
+    if (start >= nnz) return;
+    uint64_t end = start + per_group < nnz ? start + per_group : nnz;
+
+    const int nmode = pk.nmode;
+    const int tsh = pk.shift[target], tw = pk.width[target];
+    OT *outp = out + lig * V;
+
+    uint64_t cur = ~0ULL;
+    CT acc[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) acc[v] = 0;
+
+    for (uint64_t base = start; base < end; base += B) {
+        uint64_t klo[B], khi[B];
+        CT prod[B][V];
+#pragma unroll
+        for (int k = 0; k < B; ++k) {
+            uint64_t i = base + k < end ? base + k : end - 1;
+            klo[k] = pkeys[i * PKW];
+            khi[k] = PKW == 2 ? pkeys[i * PKW + 1] : 0;
+            CT val = (CT)vals[i];
+#pragma unroll
+            for (int v = 0; v < V; ++v) prod[k][v] = val;
+        }
+        for (int m = 0; m < nmode; ++m) {
+            if (m == target) continue;
+            const int sh = pk.shift[m], w = pk.width[m];
+            const FT *fm = F.p[m] + lig * V;
+            CT f[B][V];
+#pragma unroll
+            for (int k = 0; k < B; ++k) {
+                uint64_t row = field<PKW>(klo[k], khi[k], sh, w);
+                Vec::load(fm + row * R, f[k]);
+            }
+#pragma unroll
+            for (int k = 0; k < B; ++k)
+#pragma unroll
+                for (int v = 0; v < V; ++v) prod[k][v] *= f[k][v];
+        }
+#pragma unroll
+        for (int k = 0; k < B; ++k) {
+            if (base + k < end) {
+                uint64_t row = field<PKW>(klo[k], khi[k], tsh, tw);
+                if (row != cur) {
+                    if (cur != ~0ULL) {
+#pragma unroll
+                        for (int v = 0; v < V; ++v) atomic_add_out(outp + cur * R + v, (OT)acc[v]);
+                    }
+                    cur = row;
+#pragma unroll
+                    for (int v = 0; v < V; ++v) acc[v] = prod[k][v];
+                } else {
+#pragma unroll
+                    for (int v = 0; v < V; ++v) acc[v] += prod[k][v];
+                }
+            }
+        }
+    }
+    if (cur != ~0ULL) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) atomic_add_out(outp + cur * R + v, (OT)acc[v]);
+    }
+}
+
+// Any rank: one wave per range, lane r covers ranks r, r+64, ...
+template <typename VT, typename FT, typename OT, int PKW>
+__global__ __launch_bounds__(256) void k_mttkrp_generic(const uint64_t *__restrict__ pkeys,
+                                                        const VT *__restrict__ vals, uint64_t nnz,
+                                                        FPtrs<FT> F, OT *__restrict__ out,
+                                                        PackInfo pk, int target, int R,
+                                                        uint64_t per_group) {
+    using CT = typename FVec<FT>::CT;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const int lane = threadIdx.x & 63;
+    uint64_t start = wave * per_group;
+    if (start >= nnz) return;
+    uint64_t end = start + per_group < nnz ? start + per_group : nnz;
+    const int nmode = pk.nmode;
+    for (int r0 = 0; r0 < R; r0 += 64) {
+        int r = r0 + lane;
+        bool act = r < R;
+        uint64_t cur = ~0ULL;
+        CT acc = 0;
+        for (uint64_t i = start; i < end; ++i) {
+            uint64_t lo = pkeys[i * PKW], hi = PKW == 2 ? pkeys[i * PKW + 1] : 0;
+            CT p = (CT)vals[i];
+            for (int m = 0; m < nmode; ++m) {
+                if (m == target) continue;
+                uint64_t row = field<PKW>(lo, hi, pk.shift[m], pk.width[m]);
+                if (act) {
+                    FT fv = F.p[m][row * R + r];
+                    CT c;
+                    if constexpr (sizeof(FT) == 2) c = bf16_to_f32(fv);
+                    else c = (CT)fv;
+                    p *= c;
+                }
+            }
+            uint64_t row = field<PKW>(lo, hi, pk.shift[target], pk.width[target]);
+            if (row != cur) {
+                if (cur != ~0ULL && act) atomic_add_out(out + cur * R + r, (OT)acc);
+                cur = row;
+                acc = p;
+            } else {
+                acc += p;
+            }
+        }
+        if (cur != ~0ULL && act) atomic_add_out(out + cur * R + r, (OT)acc);
+    }
+}
+
+struct MttkrpArgs {
+    const uint64_t *pkeys;
+    const void *vals;  // double for F64, float otherwise
+    uint64_t nnz;
+    const void *const *factors;
+    void *out;
+    PackInfo pk;
+    int pkw;
+    int target;
+    int rank;
+    int prec;
+    bool out64;
+    uint64_t per_group;
+};
+
+template <typename VT, typename FT, typename OT, int R, int PKW>
+static void launch_seg(const MttkrpArgs &a, hipStream_t s) {
+    constexpr int V = FVec<FT>::V;
+    constexpr int L = R / V;
+    FPtrs<FT> F{};
+    for (int m = 0; m < a.pk.nmode; ++m) F.p[m] = (const FT *)a.factors[m];
+    uint64_t groups = (a.nnz + a.per_group - 1) / a.per_group;
+    uint64_t threads = groups * L;
+    unsigned blocks = cdiv(threads, 256);
+    constexpr int B = 8;
+    k_mttkrp_seg<VT, FT, OT, R, PKW, B><<<blocks, 256, 0, s>>>(
+        a.pkeys, (const VT *)a.vals, a.nnz, F, (OT *)a.out, a.pk, a.target, a.per_group);
+}
+
+template <typename VT, typename FT, typename OT, int PKW>
+static void launch_generic(const MttkrpArgs &a, hipStream_t s) {
+    FPtrs<FT> F{};
+    for (int m = 0; m < a.pk.nmode; ++m) F.p[m] = (const FT *)a.factors[m];
+    uint64_t waves = (a.nnz + a.per_group - 1) / a.per_group;
+    unsigned blocks = cdiv(waves * 64, 256);
+    k_mttkrp_generic<VT, FT, OT, PKW><<<blocks, 256, 0, s>>>(
+        a.pkeys, (const VT *)a.vals, a.nnz, F, (OT *)a.out, a.pk, a.target, a.rank, a.per_group);
+}
+
+template <typename VT, typename FT, typename OT, int PKW>
+static int dispatch_rank(const MttkrpArgs &a, hipStream_t s) {
+    constexpr int V = FVec<FT>::V;
+    switch (a.rank) {
+    case 8:
+        if constexpr (8 % V == 0) { launch_seg<VT, FT, OT, 8, PKW>(a, s); return ALTO_OK; }
+        break;
+    case 16: launch_seg<VT, FT, OT, 16, PKW>(a, s); return ALTO_OK;
+    case 32: launch_seg<VT, FT, OT, 32, PKW>(a, s); return ALTO_OK;
+    case 64: launch_seg<VT, FT, OT, 64, PKW>(a, s); return ALTO_OK;
+    default: break;
+    }
+    launch_generic<VT, FT, OT, PKW>(a, s);
+    return ALTO_OK;
+}
+
+template <int PKW>
+static int dispatch_prec(const MttkrpArgs &a, hipStream_t s) {
+    if (a.prec == ALTO_PREC_F64) return dispatch_rank<double, double, double, PKW>(a, s);
+    if (a.prec == ALTO_PREC_F32)
+        return a.out64 ? dispatch_rank<float, float, double, PKW>(a, s)
+                       : dispatch_rank<float, float, float, PKW>(a, s);
+    if (a.prec == ALTO_PREC_BF16)
+        return a.out64 ? dispatch_rank<float, uint16_t, double, PKW>(a, s)
+                       : dispatch_rank<float, uint16_t, float, PKW>(a, s);
+    return ALTO_ERR_ARG;
+}
+
+int run_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const *factors,
+               void *out, int out_f64, int algo, uint64_t per_group, hipStream_t s) {
+    if (target < 0 || target >= t->nmode || rank < 1 || rank > 4096 || !factors || !out)
+        return ALTO_ERR_ARG;
+    if (prec != ALTO_PREC_F64 && prec != ALTO_PREC_F32 && prec != ALTO_PREC_BF16) return ALTO_ERR_ARG;
+    for (int m = 0; m < t->nmode; ++m)
+        if (m != target && !factors[m]) return ALTO_ERR_ARG;
+    if (t->nnz == 0) return ALTO_OK;
+    if (algo == 0) algo = 2;
+    MttkrpArgs a{};
+    a.nnz = t->nnz;
+    a.factors = factors;
+    a.out = out;
+    a.pk = t->pack;
+    a.pkw = t->pkw;
+    a.target = target;
+    a.rank = rank;
+    a.prec = prec;
+    a.out64 = prec == ALTO_PREC_F64 || out_f64;
+    a.per_group = per_group ? per_group : 256;
+    if (algo == 1) {
+        a.pkeys = t->pkeys.as<uint64_t>();
+        if (prec == ALTO_PREC_F64) {
+            a.vals = t->vals64.p;
+        } else {
+            ALTO_TRY(ensure_vals32(t, s));
+            a.vals = t->vals32.p;
+        }
+    } else if (algo == 2) {
+        ALTO_TRY(ensure_schedule(t, target, prec == ALTO_PREC_F64, s));
+        a.pkeys = t->sched[target].pkeys.as<uint64_t>();
+        a.vals = prec == ALTO_PREC_F64 ? t->sched[target].vals64.p : t->sched[target].vals32.p;
+    } else {
+        return ALTO_ERR_ARG;
+    }
+    int st = t->pkw == 1 ? dispatch_prec<1>(a, s) : dispatch_prec<2>(a, s);
+    ALTO_TRY(st);
+    ALTO_LAUNCH_CHECK();
+    return ALTO_OK;
+}
+
+// Reference COO MTTKRP (src/mttkrp.cpp:122-156) on the device, FP64.
+struct CooPtrs {
+    const uint64_t *p[kMaxModes];
+};
+
+__global__ __launch_bounds__(256) void k_mttkrp_coo(CooPtrs ci, const double *__restrict__ vals,
+                                                    uint64_t nnz, FPtrs<double> F, int nmode,
+                                                    int target, int R, double *__restrict__ out) {
+    uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    int lane = threadIdx.x & 63;
+    if (w >= nnz) return;
+    for (int r = lane; r < R; r += 64) {
+        double p = vals[w];
+        for (int m = 0; m < nmode; ++m)
+            if (m != target) p *= F.p[m][ci.p[m][w] * R + r];
+        unsafeAtomicAdd(out + ci.p[target][w] * R + r, p);
+    }
+}
+
+}  // namespace alto
+
+using namespace alto;
+
+extern "C" {
+
+int alto_hip_mttkrp_algo(alto_hip_tensor t, int target, int prec, int rank,
+                         const void *const *factors, void *out, int out_f64, int algo,
+                         void *stream) {
+    if (!t) return ALTO_ERR_ARG;
+    ALTO_TRY(ensure_init());
+    return run_mttkrp(&t->impl, target, prec, rank, factors, out, out_f64, algo, 0,
+                      to_stream(stream));
+}
+
+int alto_hip_mttkrp(alto_hip_tensor t, int target, int prec, int rank, const void *const *factors,
+                    void *out, int out_f64, void *stream) {
+    return alto_hip_mttkrp_algo(t, target, prec, rank, factors, out, out_f64, 0, stream);
+}
+
+int alto_hip_mttkrp_coo(const alto_hip_coo *coo, int target, int rank,
+                        const double *const *factors, double *out, void *stream) {
+    if (!coo || !coo->on_device || !factors || !out || rank < 1) return ALTO_ERR_ARG;
+    if (target < 0 || target >= coo->nmode || coo->nmode > kMaxModes) return ALTO_ERR_ARG;
+    ALTO_TRY(ensure_init());
+    if (coo->nnz == 0) return ALTO_OK;
+    CooPtrs ci{};
+    FPtrs<double> F{};
+    for (int m = 0; m < coo->nmode; ++m) {
+        ci.p[m] = coo->cidx[m];
+        F.p[m] = factors[m];
+    }
+    k_mttkrp_coo<<<cdiv(coo->nnz * 64, 256), 256, 0, to_stream(stream)>>>(
+        ci, coo->vals, coo->nnz, F, coo->nmode, target, rank, out);
+    ALTO_LAUNCH_CHECK();
+    return ALTO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,68 @@
+// Device-resident ALTO tensor (the GPU counterpart of AltoTensor<LIT>,
+// include/alto.hpp:35-62) and the per-mode scheduling data the MTTKRP
+// kernels use.
+#pragma once
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+#include "sort.hpp"
+
+namespace alto {
+
+// Bit positions of every mode inside the ALTO key: bit b of the mode-m index
+// sits at key bit pos[m][b] (the pdep/pext mapping of mask m).
+struct BitPos {
+    int nmode;
+    int bits[kMaxModes];
+    uint8_t pos[kMaxModes][64];
+};
+
+// Output-ordered schedule of one mode (algo 2): the nonzeros re-sorted by
+// the mode's index, stably, so ALTO order is kept inside each row run.
+struct ModeSchedule {
+    bool built = false;
+    DevBuf pkeys;   // packed keys in row order
+    DevBuf vals32;  // FP32 values in row order
+    DevBuf vals64;  // FP64 values in row order (built on demand)
+};
+
+struct TensorImpl {
+    int nmode = 0;
+    int kw = 1;          // ALTO key words (1: 64-bit mask, 2: 128-bit mask)
+    int pkw = 1;         // packed key words
+    int total_bits = 0;
+    int nprtn = 1;
+    uint64_t nnz = 0;
+    uint64_t dims[kMaxModes] = {};
+    uint64_t masks[kMaxModes * 2] = {};
+    BitPos bp{};
+    PackInfo pack{};
+
+    DevBuf keys;    // sorted ALTO keys, nnz * kw words (interleaved)
+    DevBuf pkeys;   // sorted packed keys, nnz * pkw words
+    DevBuf perm;    // u32: sorted position -> input nonzero index
+    DevBuf vals64;  // sorted FP64 values
+    DevBuf vals32;  // sorted FP32 values (built on first FP32 use)
+
+    std::vector<uint64_t> prtn_ptr;    // nprtn + 1
+    std::vector<uint64_t> intervals;   // nprtn * nmode * 2
+
+    ModeSchedule sched[kMaxModes];
+    SortScratch sort_ws;
+    DevBuf scratch;     // generic per-call scratch
+};
+
+// Host-side mask construction: LSB_FIRST + SHORT_FIRST packing
+// (include/alto.hpp:954-1057).  Returns total bits or -1 if > 64*kw.
+int build_masks(int nmode, const uint64_t *dims, int kw, uint64_t *masks, BitPos *bp);
+int mode_bits(uint64_t dim);
+
+int ensure_vals32(TensorImpl *t, hipStream_t s);
+int ensure_schedule(TensorImpl *t, int mode, bool need64, hipStream_t s);
+
+}  // namespace alto
+
+struct alto_hip_tensor_s {
+    alto::TensorImpl impl;
+};

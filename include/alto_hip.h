@@ -162,6 +162,18 @@ int alto_hip_cpstream(const alto_hip_coo *coo, int stream_mode, int rank,
                       int *out_outer_iters, double *out_admm_iters,
                       double *out_grams, double *timings, void *stream);
 
+/* ---- synthetic workloads (no reference counterpart: benchmark input) ----
+ * Fills device arrays cidx_dev[m] (nnz uint64 each) and vals_dev (nnz FP64).
+ * mode_dist[m]: 0 = uniform over the mode, 1 = Zipf(zipf_s[m]) over a seeded
+ * permutation of the mode.  value_dist: 0 = U(0,1), 1 = lognormal(p1, p2),
+ * 2 = N(0,1), 3 = Poisson(p1) + 1.  A fraction block_frac of the nonzeros is
+ * drawn uniformly inside a seeded block^nmode sub-block.  Deterministic in
+ * (seed, nonzero index). */
+int alto_hip_generate(int nmode, const uint64_t *dims, uint64_t nnz, uint64_t seed,
+                      const int *mode_dist, const double *zipf_s, int value_dist,
+                      double p1, double p2, double block_frac, uint64_t block,
+                      uint64_t *const *cidx_dev, double *vals_dev, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

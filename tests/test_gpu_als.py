@@ -1,0 +1,76 @@
+"""GPU CP-ALS (cpd_alto, include/cpd.hpp:47-191) against the oracle at the
+same initial factors.  Bars: FP64 path fit within 1e-8 and factors within
+1e-6 relative; FP32 path |delta fit| <= 1e-5; BF16 factor storage
+|delta fit| <= 2e-3."""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import golden, rand_factors, rand_tensor
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(dims, nnz, seed, rank, dup=0.0, distinct=False):
+    dims, cidx, vals = rand_tensor(dims, nnz, seed=seed, dup_frac=dup, distinct=distinct)
+    F = rand_factors(dims, rank, seed=seed + 1)
+    return dims, cidx, vals, F
+
+
+@pytest.mark.parametrize("fit_mode", [O.FIT_REFERENCE_DIAG, O.FIT_EXACT])
+def test_als_f64_matches_oracle(gpu, fit_mode):
+    dims, cidx, vals, F = _setup([120, 90, 70], 30000, 80, 16, distinct=True)
+    ref = O.AltoTensor(dims, cidx, vals, nprtn=8)
+    rf, rl, rfits, rfe = ref.cpd_als(F, 16, 6, fit_mode=fit_mode)
+    at = gpu.create_alto(dims, cidx, vals, nprtn=8)
+    gf, gl, gfits, gfe, tim = gpu.cpd_alto(at, F, 16, 6, prec=gpu.PREC_F64, fit_mode=fit_mode)
+    assert len(gfits) == 6
+    np.testing.assert_allclose(gfits, rfits, rtol=0, atol=1e-8)
+    np.testing.assert_allclose(gfe, rfe, rtol=0, atol=1e-8)
+    np.testing.assert_allclose(gl, rl, rtol=1e-6)
+    for a, b in zip(gf, rf):
+        np.testing.assert_allclose(a, b, rtol=1e-6, atol=1e-9)
+
+
+def test_als_f64_fixture_med4(gpu):
+    dims, cidx, vals = O.read_tns(golden("med4.tns"))
+    F = rand_factors(dims, 8, seed=81)
+    ref = O.AltoTensor(dims, cidx, vals, nprtn=16)
+    rf, rl, rfits, _ = ref.cpd_als(F, 8, 5)
+    at = gpu.create_alto(dims, cidx, vals, nprtn=16)
+    gf, gl, gfits, _, _ = gpu.cpd_alto(at, F, 8, 5, prec=gpu.PREC_F64)
+    np.testing.assert_allclose(gfits, rfits, rtol=0, atol=1e-8)
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 1e-5), (2, 2e-3)])
+def test_als_reduced_precision_fit(gpu, prec, tol):
+    dims, cidx, vals, F = _setup([300, 200, 150, 40], 60000, 82, 32, dup=0.1)
+    ref = O.AltoTensor(dims, cidx, vals, nprtn=8)
+    _, _, rfits, rfe = ref.cpd_als(F, 32, 5)
+    at = gpu.create_alto(dims, cidx, vals, nprtn=8)
+    _, _, gfits, gfe, _ = gpu.cpd_alto(at, F, 32, 5, prec=prec)
+    print("prec", prec, "max |dfit|", np.abs(gfits - rfits).max(), flush=True)
+    assert np.abs(gfits - rfits).max() <= tol
+    assert np.abs(gfe - rfe).max() <= tol
+
+
+def test_als_128bit_keys(gpu):
+    dims, cidx, vals, F = _setup([1 << 20, 1 << 20, 1 << 18, 1 << 16, 1 << 12], 20000, 83, 8)
+    ref = O.AltoTensor(dims, cidx, vals, nprtn=4)
+    _, _, rfits, _ = ref.cpd_als(F, 8, 3)
+    at = gpu.create_alto(dims, cidx, vals, nprtn=4)
+    assert at.kw == 2
+    _, _, gfits, _, _ = gpu.cpd_alto(at, F, 8, 3, prec=gpu.PREC_F64)
+    np.testing.assert_allclose(gfits, rfits, rtol=0, atol=1e-8)
+
+
+def test_als_epsilon_stop_and_exact_fit_is_residual(gpu):
+    dims, cidx, vals, F = _setup([8, 7, 6], 100, 84, 3, distinct=True)
+    at = gpu.create_alto(dims, cidx, vals)
+    _, _, fits, _, _ = gpu.cpd_alto(at, F, 3, 20, eps=1e30)
+    assert len(fits) == 2
+    gf, gl, fits, fe, _ = gpu.cpd_alto(at, F, 3, 4, fit_mode=gpu.FIT_EXACT)
+    X = np.zeros(dims)
+    np.add.at(X, tuple(c.astype(np.int64) for c in cidx), vals)
+    M = np.einsum("ir,jr,kr,r->ijk", gf[0], gf[1], gf[2], gl)
+    assert abs(fe[-1] - np.linalg.norm(X - M) / np.linalg.norm(X)) < 1e-9

@@ -1,0 +1,327 @@
+"""Pins the CPU oracle (oracle/alto_oracle.c) against known answers from the
+reference's fixtures, closed forms, and independent mathematics (dense
+tensors, numpy/scipy linear algebra), so that a dropped term, a wrong index or
+a transposed operand in the oracle fails here."""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import golden, rand_factors, rand_tensor
+
+
+@pytest.fixture(autouse=True, scope="module")
+def _single_thread():
+    O.set_threads(1)
+
+
+# ---------- masks (setup_packed_alto, alto.hpp:954-1057) ----------
+
+def test_masks_small_kat():
+    dims, cidx, vals = O.read_tns(golden("small.tns"))
+    assert dims == [2, 3, 2]
+    m, total = O.alto_masks(dims)
+    assert total == 4
+    assert [int(x) for x in m[:, 0]] == [0x1, 0xC, 0x2]
+
+
+@pytest.mark.parametrize("dims,expect", [
+    ([1000, 800, 600], [0x9249249, 0x12492492, 0x24924924]),
+    ([500000, 200000, 50000, 1000], [0x6a49248888888888, 0x1524924444444444, 0x92492222222222, 0x1111111111]),
+])
+def test_masks_config_kat(dims, expect):
+    m, total = O.alto_masks(dims)
+    assert [int(x) for x in m[:, 0]] == expect
+
+
+def test_masks_128_partition_the_key():
+    dims = [1 << 20, 1 << 20, 1 << 18, 1 << 16, 1 << 12]
+    m, total = O.alto_masks(dims)
+    assert total == 86 and m.shape == (5, 2)
+    whole = 0
+    for n, d in enumerate(dims):
+        v = int(m[n, 0]) | (int(m[n, 1]) << 64)
+        assert bin(v).count("1") == O.num_bits(d)
+        assert whole & v == 0
+        whole |= v
+    assert whole == (1 << 86) - 1
+
+
+def test_num_bits_size_one_mode():
+    assert O.num_bits(1) == 0 and O.num_bits(2) == 1 and O.num_bits(1024) == 10 and O.num_bits(1025) == 11
+
+
+def test_key_too_wide_is_rejected():
+    with pytest.raises(ValueError):
+        O.alto_masks([1 << 40, 1 << 40], kw=1)
+
+
+# ---------- pdep / pext (bitops.hpp) ----------
+
+def _pdep_brute(x, mask):
+    out, b = 0, 0
+    for pos in range(128):
+        if (mask >> pos) & 1:
+            if (x >> b) & 1:
+                out |= 1 << pos
+            b += 1
+    return out
+
+
+@pytest.mark.parametrize("dims", [[1000, 800, 600], [500000, 200000, 50000, 1000],
+                                  [1 << 20, 1 << 20, 1 << 18, 1 << 16, 1 << 12]])
+def test_pdep_pext_round_trip(dims):
+    m, _ = O.alto_masks(dims)
+    rng = np.random.default_rng(3)
+    for n, d in enumerate(dims):
+        mask = int(m[n, 0]) | ((int(m[n, 1]) << 64) if m.shape[1] == 2 else 0)
+        for x in list(rng.integers(0, d, size=20)) + [0, d - 1]:
+            dep = O.pdep(int(x), m[n])
+            v = int(dep[0]) | ((int(dep[1]) << 64) if len(dep) == 2 else 0)
+            assert v == _pdep_brute(int(x), mask)
+            assert O.pext(dep, m[n]) == int(x)
+
+
+# ---------- sort + partitions ----------
+
+def test_small_sorted_pairs_kat():
+    dims, cidx, vals = O.read_tns(golden("small.tns"))
+    at = O.AltoTensor(dims, cidx, vals, nprtn=8)
+    assert [int(k) for k in at.keys[:, 0]] == [1, 2, 4, 7, 8, 9]
+    assert list(at.vals) == [4.0, 3.0, 1.0, 6.0, 2.0, 5.0]
+
+
+def test_partition_closed_form_kat():
+    dims, cidx, vals = O.read_tns(golden("small.tns"))
+    at = O.AltoTensor(dims, cidx, vals, nprtn=8)
+    assert list(at.prtn_ptr) == [0, 1, 2, 3, 4, 5, 6, 6, 6]
+    assert at.intervals[6].tolist() == [[0, 0]] * 3 and at.intervals[7].tolist() == [[0, 0]] * 3
+
+
+def test_sort_is_ascending_and_stable_on_ties():
+    dims, cidx, vals = rand_tensor([7, 5, 3], 300, seed=1, dup_frac=0.5)
+    at = O.AltoTensor(dims, cidx, vals, nprtn=3)
+    k = at.keys[:, 0]
+    assert np.all(k[1:] >= k[:-1])
+    perm = at.perm
+    for i in range(1, len(k)):
+        if k[i] == k[i - 1]:
+            assert perm[i] > perm[i - 1]
+    # keys decode back to the input coordinates of the permuted nonzero
+    for i in range(0, len(k), 17):
+        for n in range(3):
+            assert O.pext(at.keys[i], at.masks[n]) == int(cidx[n][perm[i]])
+
+
+def test_partition_intervals_brute_force():
+    dims, cidx, vals = rand_tensor([50, 40, 30], 1000, seed=2)
+    at = O.AltoTensor(dims, cidx, vals, nprtn=7)
+    per = -(-1000 // 7)
+    for p in range(7):
+        s, e = min(p * per, 1000), min((p + 1) * per, 1000)
+        assert at.prtn_ptr[p + 1] == e
+        for n in range(3):
+            idx = [O.pext(at.keys[i], at.masks[n]) for i in range(s, e)]
+            assert at.intervals[p, n].tolist() == [min(idx), max(idx)]
+
+
+def test_get_num_ptrn():
+    # util.hpp:30-48
+    assert O.get_num_ptrn(5, 8) == 1
+    assert O.get_num_ptrn(1000, 8) == 8
+    assert O.get_num_ptrn(40, 8) == 4
+
+
+# ---------- MTTKRP ----------
+
+def _dense(dims, cidx, vals):
+    X = np.zeros(dims)
+    np.add.at(X, tuple(c.astype(np.int64) for c in cidx), vals)
+    return X
+
+
+def _mttkrp_dense(X, factors, mode):
+    N = X.ndim
+    letters = "abcdefgh"[:N]
+    ins = [letters] + [letters[m] + "r" for m in range(N) if m != mode]
+    ops = [X] + [factors[m] for m in range(N) if m != mode]
+    return np.einsum(",".join(ins) + "->" + letters[mode] + "r", *ops)
+
+
+@pytest.mark.parametrize("nprtn", [1, 3, 16])
+def test_mttkrp_variants_match_dense_definition(nprtn):
+    dims, cidx, vals = rand_tensor([9, 700, 6, 5], 2000, seed=4, dup_frac=0.2)
+    F = rand_factors(dims, 8, seed=5, lo=-1, hi=1)
+    X = _dense(dims, cidx, vals)
+    at = O.AltoTensor(dims, cidx, vals, nprtn=nprtn)
+    seen = set()
+    for mode in range(4):
+        ref = _mttkrp_dense(X, F, mode)
+        out, var = at.mttkrp(F, mode)
+        seen.add(var)
+        np.testing.assert_allclose(out, ref, rtol=1e-12, atol=1e-12)
+    assert O.VAR_ATOMIC in seen and O.VAR_DA_MEM_PULL in seen  # mode 1: I > 512, reuse <= 4
+
+
+def test_mttkrp_stream_variant_size_one_mode():
+    dims, cidx, vals = rand_tensor([1, 40, 30], 500, seed=6)
+    F = rand_factors(dims, 4, seed=7)
+    at = O.AltoTensor(dims, cidx, vals, nprtn=4)
+    assert at.variant(0, stream_mode=0) == O.VAR_STREAM
+    out, var = at.mttkrp(F, 0, stream_mode=0)
+    np.testing.assert_allclose(out, _mttkrp_dense(_dense(dims, cidx, vals), F, 0), rtol=1e-12)
+
+
+def test_mttkrp_accumulates():
+    dims, cidx, vals = rand_tensor([10, 12, 14], 400, seed=8)
+    F = rand_factors(dims, 4, seed=9)
+    at = O.AltoTensor(dims, cidx, vals, nprtn=2)
+    base = np.full((12, 4), 2.5)
+    out, _ = at.mttkrp(F, 1, out=base.copy())
+    np.testing.assert_allclose(out - 2.5, _mttkrp_dense(_dense(dims, cidx, vals), F, 1), rtol=1e-12)
+
+
+def test_mttkrp_coo_med_fixture():
+    dims, cidx, vals = O.read_tns(golden("med.tns"))
+    F = rand_factors(dims, 8, seed=10)
+    at = O.AltoTensor(dims, cidx, vals, nprtn=8)
+    for mode in range(3):
+        ref = O.mttkrp_coo(cidx, vals, F, mode)
+        out, _ = at.mttkrp(F, mode)
+        np.testing.assert_allclose(out, ref, rtol=1e-12, atol=1e-9)
+
+
+def test_zero_based_fixture_reads_like_reference():
+    d1, c1, v1 = O.read_tns(golden("small4.tns"))
+    d0, c0, v0 = O.read_tns(golden("small4_zeroidx.tns"), zero_based=True)
+    assert d1 == [2, 2, 3, 3] and d0 == [3, 3, 4, 3]
+    np.testing.assert_array_equal(v0, v1)
+
+
+# ---------- dense step ----------
+
+def test_gram_upper_triangle_only():
+    U = np.random.default_rng(11).random((50, 6))
+    G = O.gram_upper(U)
+    full = U.T @ U
+    iu = np.triu_indices(6)
+    np.testing.assert_allclose(G[iu], full[iu], rtol=1e-13)
+    assert np.all(G[np.tril_indices(6, -1)] == 0)
+
+
+def test_kruskal_norms():
+    U = np.random.default_rng(12).random((40, 5)) - 0.3
+    U2, lam2 = O.kruskal_norm(U, 0)
+    np.testing.assert_allclose(lam2, np.linalg.norm(U, axis=0), rtol=1e-13)
+    np.testing.assert_allclose(np.linalg.norm(U2, axis=0), 1.0, rtol=1e-13)
+    Um, lamm = O.kruskal_norm(U * 4, 1)
+    np.testing.assert_allclose(lamm, np.maximum(1.0, (U * 4).max(axis=0)), rtol=0)
+    Uneg = -np.abs(U)
+    _, lneg = O.kruskal_norm(Uneg, 1)
+    assert np.all(lneg == 1.0)  # signed maximum, floored at 1
+
+
+def test_pseudo_inverse_solves_regularised_hadamard():
+    rng = np.random.default_rng(13)
+    R = 6
+    facs = [rng.random((30, R)) for _ in range(3)]
+    grams = [np.triu(f.T @ f) for f in facs]
+    B = rng.random((20, R))
+    X, g2, fb = O.pseudo_inverse_als(grams, 1, B)
+    assert fb == 0
+    V = (facs[0].T @ facs[0]) * (facs[2].T @ facs[2]) + 1e-3 * np.eye(R)
+    np.testing.assert_allclose(X, np.linalg.solve(V, B.T).T, rtol=1e-9)
+
+
+def test_pseudo_inverse_fallback_uses_diagonal_min_norm():
+    R = 4
+    # Grams as the reference holds them: upper triangle written, lower zero.
+    g1 = np.eye(R) * 2.0 + np.triu(np.full((R, R), 0.5), 1)
+    g2 = np.diag([3.0, -1.0, 1.0, 2.0]) + np.triu(np.full((R, R), 0.25), 1)
+    grams = [np.full((R, R), np.nan), g1, g2]
+    B = np.ones((3, R))
+    X, _, fb = O.pseudo_inverse_als(grams, 0, B)
+    assert fb == 1  # V + 1e-3 I is indefinite (pivot 1)
+    # fallback symmetrises V from its (zero) lower triangle: a diagonal solve
+    np.testing.assert_allclose(X, np.tile([1 / 6.0, -0.5, 0.5, 0.25], (3, 1)))
+
+
+def test_cpd_fit_exact_is_relative_residual():
+    dims, cidx, vals = rand_tensor([6, 5, 4], 60, seed=14, distinct=True)
+    R = 3
+    at = O.AltoTensor(dims, cidx, vals, nprtn=2)
+    F = rand_factors(dims, R, seed=15)
+    facs, lam, fits, fits_exact = at.cpd_als(F, R, 4, fit_mode=O.FIT_EXACT)
+    X = _dense(dims, cidx, vals)
+    M = np.einsum("ir,jr,kr,r->ijk", facs[0], facs[1], facs[2], lam)
+    assert abs(fits_exact[-1] - np.linalg.norm(X - M) / np.linalg.norm(X)) < 1e-10
+    # rank 1: the reference's diagonal-only norm equals the exact one
+    F1 = rand_factors(dims, 1, seed=16)
+    f1, l1, fd, fe = at.cpd_als(F1, 1, 3, fit_mode=O.FIT_REFERENCE_DIAG)
+    np.testing.assert_allclose(fd, fe, rtol=1e-12)
+    M1 = np.einsum("ir,jr,kr,r->ijk", f1[0], f1[1], f1[2], l1)
+    assert abs(fd[-1] - np.linalg.norm(X - M1) / np.linalg.norm(X)) < 1e-10
+
+
+def test_cpd_als_decreases_residual_and_stops_on_epsilon():
+    dims, cidx, vals = rand_tensor([20, 15, 10], 800, seed=17)
+    at = O.AltoTensor(dims, cidx, vals, nprtn=4)
+    F = rand_factors(dims, 4, seed=18)
+    _, _, fits, fe = at.cpd_als(F, 4, 30, eps=1e-30, fit_mode=O.FIT_EXACT)
+    assert len(fits) == 30 and fe[-1] < fe[0]
+    _, _, fits2, _ = at.cpd_als(F, 4, 30, eps=1e30)
+    assert len(fits2) == 2  # stops at it = 1 (cpd.hpp:163)
+
+
+# ---------- ADMM (admm.cpp:216-429) ----------
+
+def test_admm_unconstrained_is_least_squares():
+    rng = np.random.default_rng(19)
+    R = 5
+    facs = [rng.random((25, R)) for _ in range(3)]
+    grams = [f.T @ f for f in facs]
+    mt = rng.random((12, R))
+    x, d, it = O.admm_cf(grams, 2, np.zeros((12, R)), np.zeros((12, R)), mt, nonneg=False)
+    Phi = grams[0] * grams[1] + 1e-12 * np.eye(R)
+    np.testing.assert_allclose(x, np.linalg.solve(Phi, mt.T).T, rtol=1e-9)
+
+
+def test_admm_nonneg_approaches_nnls_solution():
+    from scipy.optimize import nnls
+    rng = np.random.default_rng(20)
+    R = 4
+    facs = [rng.random((30, R)) for _ in range(3)]
+    grams = [f.T @ f for f in facs]
+    Phi = grams[1] * grams[2]
+    mt = rng.normal(size=(10, R)) * 5
+    x = np.zeros((10, R))
+    d = np.zeros((10, R))
+    for _ in range(40):  # repeated calls keep the dual, as CP-stream's outer loop does
+        x, d, it = O.admm_cf(grams, 0, x, d, mt, nonneg=True)
+    assert np.all(x >= 0)
+    Lc = np.linalg.cholesky(Phi)
+    for i in range(10):
+        # min 1/2 x^T Phi x - b^T x, x >= 0  <=>  NNLS on ||L^T x - L^-1 b||
+        ref, _ = nnls(Lc.T, np.linalg.solve(Lc, mt[i]))
+        np.testing.assert_allclose(x[i], ref, atol=2e-3 * (1 + np.abs(ref).max()))
+
+
+# ---------- CP-stream (streaming_cpd.cpp) ----------
+
+def test_cpstream_batching_and_invariants():
+    rng = np.random.default_rng(21)
+    T, I, J, nnz = 6, 30, 25, 1200
+    t = np.sort(rng.integers(0, T, size=nnz)).astype(np.uint64)
+    t[:T] = np.arange(T)  # every slice non-empty
+    i = rng.integers(0, I, size=nnz).astype(np.uint64)
+    j = rng.integers(0, J, size=nnz).astype(np.uint64)
+    v = rng.poisson(3, size=nnz) + 1.0
+    R = 4
+    init = [rng.random((T, R)), rng.random((I, R)), rng.random((J, R))]
+    res = O.cpstream([T, I, J], [t, i, j], v, 0, R, 5, 1e-5, True, init)
+    assert res["slices"] == T
+    # running max of first-appearance labels: final dims = number of distinct indices
+    assert int(res["dims"][1]) == len(np.unique(i)) and int(res["dims"][2]) == len(np.unique(j))
+    assert all(np.all(f >= 0) for f in res["factors"][1:])
+    assert np.all((res["outer_iters"] >= 1) & (res["outer_iters"] <= 5))
+    G = res["grams"]
+    np.testing.assert_allclose(G[0], G[0].T, rtol=1e-12)
