@@ -20,7 +20,7 @@
 namespace alto {
 
 int run_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const *factors,
-               void *out, int out_f64, int algo, uint64_t per_group, hipStream_t s);
+               void *out, int out_f64, int algo, uint64_t per_group, int flags, hipStream_t s);
 
 // inner = sum over blocks/columns of u*M; ||M||^2 exact and diagonal-only.
 __global__ __launch_bounds__(256) void k_fit(const double *__restrict__ grams, int nmode, int R,
@@ -154,7 +154,7 @@ extern "C" int alto_hip_cpd_als(alto_hip_tensor th, const alto_hip_als_opts *o,
             ALTO_HIP_TRY(hipEventRecord(E[0], s));
             ALTO_TRY(launch_fill_zero(mt.as<double>(), I * R, s));
             ALTO_HIP_TRY(hipEventRecord(E[1], s));
-            ALTO_TRY(run_mttkrp(t, n, prec, R, fptr, mt.p, 1, 2, 0, s));
+            ALTO_TRY(run_mttkrp(t, n, prec, R, fptr, mt.p, 1, 2, 0, -1, s));
             ALTO_HIP_TRY(hipEventRecord(E[2], s));
             ALTO_TRY(launch_chol_inv(grams.as<double>(), N, n, R, 1e-3, true, vinv.as<double>(),
                                      flag.as<int>(), nullptr, s));

@@ -70,44 +70,78 @@ __device__ __forceinline__ uint64_t field(uint64_t lo, uint64_t hi, int sh, int 
     return extract_field(lo, hi, sh, w);
 }
 
+template <typename T> __device__ __forceinline__ T stream_load(const T *p, bool nt) {
+    return nt ? __builtin_nontemporal_load(p) : *p;
+}
+
+// flags: bit 0 = non-temporal loads of the key/value stream (keeps factor
+// rows resident in L2), bit 1 = skip the gathers of a nonzero whose key
+// equals the previous one (duplicate coordinates reuse the gathered product),
+// bit 2 = exclusive runs: the nonzeros are in output-row order, so a run that
+// starts and ends inside one lane group's range owns its row (plain
+// read-add-store), and only the runs at range boundaries are combined -- in
+// LDS across the workgroup, then with atomics for the workgroup's first and
+// last rows.
 template <typename VT, typename FT, typename OT, int R, int PKW, int B>
 __global__ __launch_bounds__(256) void k_mttkrp_seg(const uint64_t *__restrict__ pkeys,
                                                     const VT *__restrict__ vals, uint64_t nnz,
                                                     FPtrs<FT> F, OT *__restrict__ out,
-                                                    PackInfo pk, int target, uint64_t per_group) {
+                                                    PackInfo pk, int target, uint64_t per_group,
+                                                    int flags) {
     using Vec = FVec<FT>;
     using CT = typename Vec::CT;
     constexpr int V = Vec::V;
     constexpr int L = R / V;
+    constexpr int G = 256 / L;  // lane groups per workgroup
     static_assert(R % V == 0 && L >= 1 && L <= 64 && (64 % L) == 0, "bad rank tiling");
+    __shared__ CT s_part[2][G][R];       // first / last run partial of each group
+    __shared__ uint64_t s_row[2][G];     // their rows (~0 = none)
 
     const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t group = gtid / L;
+    const int gib = threadIdx.x / L;     // group in block
     const int lig = (int)(gtid % L);
+    const bool excl = flags & 4;
     uint64_t start = group * per_group;
-    if (start >= nnz) return;
     uint64_t end = start + per_group < nnz ? start + per_group : nnz;
+    if (start > nnz) start = nnz;
 
     const int nmode = pk.nmode;
     const int tsh = pk.shift[target], tw = pk.width[target];
     OT *outp = out + lig * V;
 
-    uint64_t cur = ~0ULL;
-    CT acc[V];
+    const bool nt = flags & 1, dedup = flags & 2;
+    uint64_t cur = ~0ULL, first_row = ~0ULL;
+    bool in_first = true;
+    CT acc[V], facc[V];
 #pragma unroll
-    for (int v = 0; v < V; ++v) acc[v] = 0;
+    for (int v = 0; v < V; ++v) {
+        acc[v] = 0;
+        facc[v] = 0;
+    }
+    uint64_t plo = ~0ULL, phi = ~0ULL;
+    CT pg[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) pg[v] = 0;
 
     for (uint64_t base = start; base < end; base += B) {
         uint64_t klo[B], khi[B];
-        CT prod[B][V];
+        CT val[B];
+        bool dup[B];
+        CT g[B][V];
 #pragma unroll
         for (int k = 0; k < B; ++k) {
             uint64_t i = base + k < end ? base + k : end - 1;
-            klo[k] = pkeys[i * PKW];
-            khi[k] = PKW == 2 ? pkeys[i * PKW + 1] : 0;
-            CT val = (CT)vals[i];
+            klo[k] = stream_load(pkeys + i * PKW, nt);
+            khi[k] = PKW == 2 ? stream_load(pkeys + i * PKW + 1, nt) : 0;
+            val[k] = (CT)stream_load(vals + i, nt);
+        }
 #pragma unroll
-            for (int v = 0; v < V; ++v) prod[k][v] = val;
+        for (int k = 0; k < B; ++k) {
+            uint64_t pl = k == 0 ? plo : klo[k - 1], ph = k == 0 ? phi : khi[k - 1];
+            dup[k] = dedup && (k > 0 || base > start) && klo[k] == pl && khi[k] == ph;
+#pragma unroll
+            for (int v = 0; v < V; ++v) g[k][v] = 1;
         }
         for (int m = 0; m < nmode; ++m) {
             if (m == target) continue;
@@ -116,36 +150,310 @@ __global__ __launch_bounds__(256) void k_mttkrp_seg(const uint64_t *__restrict__
             CT f[B][V];
 #pragma unroll
             for (int k = 0; k < B; ++k) {
-                uint64_t row = field<PKW>(klo[k], khi[k], sh, w);
-                Vec::load(fm + row * R, f[k]);
+                if (!dup[k]) {
+                    uint64_t row = field<PKW>(klo[k], khi[k], sh, w);
+                    Vec::load(fm + row * R, f[k]);
+                } else {
+#pragma unroll
+                    for (int v = 0; v < V; ++v) f[k][v] = 1;
+                }
             }
 #pragma unroll
             for (int k = 0; k < B; ++k)
 #pragma unroll
-                for (int v = 0; v < V; ++v) prod[k][v] *= f[k][v];
+                for (int v = 0; v < V; ++v) g[k][v] *= f[k][v];
         }
+#pragma unroll
+        for (int k = 0; k < B; ++k) {
+            if (dup[k]) {
+#pragma unroll
+                for (int v = 0; v < V; ++v) g[k][v] = k == 0 ? pg[v] : g[k - 1][v];
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < V; ++v) pg[v] = g[B - 1][v];
+        plo = klo[B - 1];
+        phi = khi[B - 1];
 #pragma unroll
         for (int k = 0; k < B; ++k) {
             if (base + k < end) {
                 uint64_t row = field<PKW>(klo[k], khi[k], tsh, tw);
                 if (row != cur) {
                     if (cur != ~0ULL) {
+                        if (!excl) {
 #pragma unroll
-                        for (int v = 0; v < V; ++v) atomic_add_out(outp + cur * R + v, (OT)acc[v]);
+                            for (int v = 0; v < V; ++v) atomic_add_out(outp + cur * R + v, (OT)acc[v]);
+                        } else if (in_first) {
+                            first_row = cur;
+#pragma unroll
+                            for (int v = 0; v < V; ++v) facc[v] = acc[v];
+                            in_first = false;
+                        } else {
+#pragma unroll
+                            for (int v = 0; v < V; ++v) outp[cur * R + v] += (OT)acc[v];
+                        }
                     }
                     cur = row;
 #pragma unroll
-                    for (int v = 0; v < V; ++v) acc[v] = prod[k][v];
+                    for (int v = 0; v < V; ++v) acc[v] = val[k] * g[k][v];
                 } else {
 #pragma unroll
-                    for (int v = 0; v < V; ++v) acc[v] += prod[k][v];
+                    for (int v = 0; v < V; ++v) acc[v] += val[k] * g[k][v];
                 }
             }
         }
     }
-    if (cur != ~0ULL) {
+    if (!excl) {
+        if (cur != ~0ULL) {
 #pragma unroll
-        for (int v = 0; v < V; ++v) atomic_add_out(outp + cur * R + v, (OT)acc[v]);
+            for (int v = 0; v < V; ++v) atomic_add_out(outp + cur * R + v, (OT)acc[v]);
+        }
+        return;
+    }
+    // Boundary runs: [first run] ... [last run] of each group, in order.
+    if (in_first) {  // zero or one run in this group: it is the first run
+        first_row = cur;
+#pragma unroll
+        for (int v = 0; v < V; ++v) facc[v] = acc[v];
+        cur = ~0ULL;
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        s_part[0][gib][lig * V + v] = facc[v];
+        s_part[1][gib][lig * V + v] = acc[v];
+    }
+    if (lig == 0) {
+        s_row[0][gib] = first_row;
+        s_row[1][gib] = cur;
+    }
+    __syncthreads();
+    if (threadIdx.x < R) {
+        const int r = threadIdx.x;
+        OT *o = out + r;
+        uint64_t crow = ~0ULL;
+        CT a = 0;
+        bool first_merged = true;
+        for (int e = 0; e < 2 * G; ++e) {
+            uint64_t row = s_row[e & 1][e >> 1];
+            if (row == ~0ULL) continue;
+            CT x = s_part[e & 1][e >> 1][r];
+            if (row == crow) {
+                a += x;
+            } else {
+                if (crow != ~0ULL) {
+                    if (first_merged) atomic_add_out(o + crow * R, (OT)a);
+                    else o[crow * R] += (OT)a;
+                    first_merged = false;
+                }
+                crow = row;
+                a = x;
+            }
+        }
+        if (crow != ~0ULL) atomic_add_out(o + crow * R, (OT)a);
+    }
+}
+
+// Decode-once variant of k_mttkrp_seg (same ranges, same run merging and
+// boundary combining).  Each lane of a group decodes one nonzero of the
+// group's next L (coalesced key/value loads) into an LDS record holding the
+// byte offsets of the gathered rows; the group reads the records back with
+// broadcast LDS reads, so per nonzero a lane only issues its 16-byte gathers
+// (uniform base address + 32-bit offset) and its multiplies.  The keys and
+// values of the next L nonzeros are loaded while the current gathers are in
+// flight.  Needs nmode <= 5 and I_m * R * sizeof(factor) < 2^32 bytes.
+template <typename VT> struct __align__(16) DecRec {
+    uint32_t off[4];  // byte offsets of the gathered rows (non-target modes)
+    uint32_t trow;    // output row
+    uint32_t pad;
+    VT val;
+};
+
+template <typename VT, typename FT, typename OT, int R, int PKW>
+__global__ __launch_bounds__(256) void k_mttkrp_dec(const uint64_t *__restrict__ pkeys,
+                                                    const VT *__restrict__ vals, uint64_t nnz,
+                                                    FPtrs<FT> F, OT *__restrict__ out,
+                                                    PackInfo pk, int target, uint64_t per_group,
+                                                    int flags) {
+    using Vec = FVec<FT>;
+    using CT = typename Vec::CT;
+    constexpr int V = Vec::V;
+    constexpr int L = R / V;
+    constexpr int G = 256 / L;
+    constexpr int CB = L < 4 ? L : 4;  // nonzeros whose gathers are issued together
+    static_assert(R % V == 0 && L >= 1 && L <= 64 && (64 % L) == 0, "bad rank tiling");
+    using Rec = DecRec<VT>;
+    __shared__ Rec s_rec[256];
+    __shared__ CT s_part[2][G][R];
+    __shared__ uint64_t s_row[2][G];
+
+    const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t group = gtid / L;
+    const int gib = threadIdx.x / L;
+    const int lig = (int)(gtid % L);
+    const bool nt = flags & 1, excl = flags & 4;
+    uint64_t start = group * per_group;
+    uint64_t end = start + per_group < nnz ? start + per_group : nnz;
+    if (start > nnz) start = nnz;
+
+    const int nm1 = pk.nmode - 1;
+    int osh[4], ow[4];
+    const char *fb[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        int m = q < target ? q : q + 1;  // q-th mode other than the target
+        bool ok = q < nm1;
+        osh[q] = ok ? pk.shift[m] : 0;
+        ow[q] = ok ? pk.width[m] : 0;
+        fb[q] = ok ? (const char *)F.p[m] : nullptr;
+    }
+    const int tsh = pk.shift[target], tw = pk.width[target];
+    const uint32_t lane_b = (uint32_t)(lig * V * sizeof(FT));
+    OT *outp = out + lig * V;
+    Rec *myrec = s_rec + gib * L;
+
+    uint64_t cur = ~0ULL, first_row = ~0ULL;
+    bool in_first = true;
+    CT acc[V], facc[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        acc[v] = 0;
+        facc[v] = 0;
+    }
+    // keys / values of the first step
+    uint64_t klo = 0, khi = 0;
+    VT kval = 0;
+    if (start < end) {
+        uint64_t i = start + lig < end ? start + lig : end - 1;
+        klo = stream_load(pkeys + i * PKW, nt);
+        if (PKW == 2) khi = stream_load(pkeys + i * PKW + 1, nt);
+        kval = stream_load(vals + i, nt);
+    }
+    for (uint64_t base = start; base < end; base += L) {
+        {
+            Rec r;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                r.off[q] = (uint32_t)field<PKW>(klo, khi, osh[q], ow[q]) * (uint32_t)(R * sizeof(FT));
+            r.trow = (uint32_t)field<PKW>(klo, khi, tsh, tw);
+            r.pad = 0;
+            r.val = kval;
+            myrec[lig] = r;
+        }
+        const uint64_t nb = base + L;
+        if (nb < end) {  // prefetch the next step
+            uint64_t i = nb + lig < end ? nb + lig : end - 1;
+            klo = stream_load(pkeys + i * PKW, nt);
+            if (PKW == 2) khi = stream_load(pkeys + i * PKW + 1, nt);
+            kval = stream_load(vals + i, nt);
+        }
+        __builtin_amdgcn_wave_barrier();
+        const int nh = (end - base) < (uint64_t)L ? (int)(end - base) : L;
+#pragma unroll
+        for (int c0 = 0; c0 < L; c0 += CB) {
+            uint32_t ob[CB][4], tr[CB];
+            CT vv[CB];
+#pragma unroll
+            for (int k = 0; k < CB; ++k) {
+                const Rec &r = myrec[c0 + k];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) ob[k][q] = r.off[q] + lane_b;
+                tr[k] = r.trow;
+                vv[k] = (CT)r.val;
+            }
+            CT f[4][CB][V];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (q < nm1) {
+#pragma unroll
+                    for (int k = 0; k < CB; ++k) Vec::load((const FT *)(fb[q] + ob[k][q]), f[q][k]);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < CB; ++k) {
+                CT g[V];
+#pragma unroll
+                for (int v = 0; v < V; ++v) g[v] = vv[k] * f[0][k][v];
+#pragma unroll
+                for (int q = 1; q < 4; ++q)
+                    if (q < nm1) {
+#pragma unroll
+                        for (int v = 0; v < V; ++v) g[v] *= f[q][k][v];
+                    }
+                if (c0 + k < nh) {
+                    const uint64_t row = tr[k];
+                    if (row != cur) {
+                        if (cur != ~0ULL) {
+                            if (!excl) {
+#pragma unroll
+                                for (int v = 0; v < V; ++v) atomic_add_out(outp + cur * R + v, (OT)acc[v]);
+                            } else if (in_first) {
+                                first_row = cur;
+#pragma unroll
+                                for (int v = 0; v < V; ++v) facc[v] = acc[v];
+                                in_first = false;
+                            } else {
+#pragma unroll
+                                for (int v = 0; v < V; ++v) outp[cur * R + v] += (OT)acc[v];
+                            }
+                        }
+                        cur = row;
+#pragma unroll
+                        for (int v = 0; v < V; ++v) acc[v] = g[v];
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < V; ++v) acc[v] += g[v];
+                    }
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (!excl) {
+        if (cur != ~0ULL) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) atomic_add_out(outp + cur * R + v, (OT)acc[v]);
+        }
+        return;
+    }
+    if (in_first) {
+        first_row = cur;
+#pragma unroll
+        for (int v = 0; v < V; ++v) facc[v] = acc[v];
+        cur = ~0ULL;
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        s_part[0][gib][lig * V + v] = facc[v];
+        s_part[1][gib][lig * V + v] = acc[v];
+    }
+    if (lig == 0) {
+        s_row[0][gib] = first_row;
+        s_row[1][gib] = cur;
+    }
+    __syncthreads();
+    if (threadIdx.x < R) {
+        const int r = threadIdx.x;
+        OT *o = out + r;
+        uint64_t crow = ~0ULL;
+        CT a = 0;
+        bool first_merged = true;
+        for (int e = 0; e < 2 * G; ++e) {
+            uint64_t row = s_row[e & 1][e >> 1];
+            if (row == ~0ULL) continue;
+            CT x = s_part[e & 1][e >> 1][r];
+            if (row == crow) {
+                a += x;
+            } else {
+                if (crow != ~0ULL) {
+                    if (first_merged) atomic_add_out(o + crow * R, (OT)a);
+                    else o[crow * R] += (OT)a;
+                    first_merged = false;
+                }
+                crow = row;
+                a = x;
+            }
+        }
+        if (crow != ~0ULL) atomic_add_out(o + crow * R, (OT)a);
     }
 }
 
@@ -208,6 +516,8 @@ struct MttkrpArgs {
     int prec;
     bool out64;
     uint64_t per_group;
+    int flags;
+    bool decode_once;
 };
 
 template <typename VT, typename FT, typename OT, int R, int PKW>
@@ -220,8 +530,12 @@ static void launch_seg(const MttkrpArgs &a, hipStream_t s) {
     uint64_t threads = groups * L;
     unsigned blocks = cdiv(threads, 256);
     constexpr int B = 8;
-    k_mttkrp_seg<VT, FT, OT, R, PKW, B><<<blocks, 256, 0, s>>>(
-        a.pkeys, (const VT *)a.vals, a.nnz, F, (OT *)a.out, a.pk, a.target, a.per_group);
+    if (a.decode_once)
+        k_mttkrp_dec<VT, FT, OT, R, PKW><<<blocks, 256, 0, s>>>(
+            a.pkeys, (const VT *)a.vals, a.nnz, F, (OT *)a.out, a.pk, a.target, a.per_group, a.flags);
+    else
+        k_mttkrp_seg<VT, FT, OT, R, PKW, B><<<blocks, 256, 0, s>>>(
+            a.pkeys, (const VT *)a.vals, a.nnz, F, (OT *)a.out, a.pk, a.target, a.per_group, a.flags);
 }
 
 template <typename VT, typename FT, typename OT, int PKW>
@@ -263,7 +577,7 @@ static int dispatch_prec(const MttkrpArgs &a, hipStream_t s) {
 }
 
 int run_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const *factors,
-               void *out, int out_f64, int algo, uint64_t per_group, hipStream_t s) {
+               void *out, int out_f64, int algo, uint64_t per_group, int flags, hipStream_t s) {
     if (target < 0 || target >= t->nmode || rank < 1 || rank > 4096 || !factors || !out)
         return ALTO_ERR_ARG;
     if (prec != ALTO_PREC_F64 && prec != ALTO_PREC_F32 && prec != ALTO_PREC_BF16) return ALTO_ERR_ARG;
@@ -282,6 +596,14 @@ int run_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const 
     a.prec = prec;
     a.out64 = prec == ALTO_PREC_F64 || out_f64;
     a.per_group = per_group ? per_group : 256;
+    a.flags = flags < 0 ? 3 : flags;
+    if (algo == 2) a.flags |= 4;  // output-ordered: runs inside a range own their row
+    // decode-once kernel: nmode <= 5 and 32-bit element offsets into every factor
+    a.decode_once = t->nmode <= 5 && !(a.flags & 8);
+    for (int m = 0; m < t->nmode; ++m)
+        if (t->dims[m] * (uint64_t)rank * (prec == ALTO_PREC_F64 ? 8 : prec == ALTO_PREC_F32 ? 4 : 2) >=
+            (1ULL << 32))
+            a.decode_once = false;
     if (algo == 1) {
         a.pkeys = t->pkeys.as<uint64_t>();
         if (prec == ALTO_PREC_F64) {
@@ -333,7 +655,16 @@ int alto_hip_mttkrp_algo(alto_hip_tensor t, int target, int prec, int rank,
                          void *stream) {
     if (!t) return ALTO_ERR_ARG;
     ALTO_TRY(ensure_init());
-    return run_mttkrp(&t->impl, target, prec, rank, factors, out, out_f64, algo, 0,
+    return run_mttkrp(&t->impl, target, prec, rank, factors, out, out_f64, algo, 0, -1,
+                      to_stream(stream));
+}
+
+int alto_hip_mttkrp_config(alto_hip_tensor t, int target, int prec, int rank,
+                           const void *const *factors, void *out, int out_f64, int algo,
+                           uint64_t segment, int flags, void *stream) {
+    if (!t) return ALTO_ERR_ARG;
+    ALTO_TRY(ensure_init());
+    return run_mttkrp(&t->impl, target, prec, rank, factors, out, out_f64, algo, segment, flags,
                       to_stream(stream));
 }
 

@@ -39,7 +39,9 @@ extern "C" {
 #define ALTO_PREC_F32 1   /* FP32 values and factors, FP32 products */
 #define ALTO_PREC_BF16 2  /* FP32 values, BF16-stored factors widened to FP32 */
 
-/* Fit convention (cpd.hpp:193-275, see DESIGN notes in the library). */
+/* Fit convention of cpd_fit_alto (cpd.hpp:193-275).  The reference reads the
+ * Gram lower triangles, which it never writes (gram.cpp:29-42); with that
+ * memory zero its ||M||^2 keeps only the diagonal terms. */
 #define ALTO_FIT_EXACT 0           /* ||M||^2 from the symmetric Grams */
 #define ALTO_FIT_REFERENCE_DIAG 1  /* ||M||^2 from the Gram lower triangles as
                                       the reference reads them (zero off-diagonal) */
@@ -115,10 +117,17 @@ int alto_hip_mttkrp(alto_hip_tensor t, int target, int prec, int rank,
                     void *stream);
 /* Same, selecting the kernel explicitly (for tests / the autotuner):
  * algo 0 = automatic, 1 = ALTO-order segments (atomic flush),
- * 2 = output-ordered segments (per-mode row order), 3 = LDS-privatised rows. */
+ * 2 = output-ordered segments (per-mode row order). */
 int alto_hip_mttkrp_algo(alto_hip_tensor t, int target, int prec, int rank,
                          const void *const *factors, void *out, int out_f64,
                          int algo, void *stream);
+/* Same, with the kernel's tuning knobs (the autotuner's entry point):
+ * segment = nonzeros per lane group (0 = default 256); flags bit 0 =
+ * non-temporal key/value stream loads, bit 1 = reuse the gathered product
+ * across equal consecutive keys (-1 = default: both). */
+int alto_hip_mttkrp_config(alto_hip_tensor t, int target, int prec, int rank,
+                           const void *const *factors, void *out, int out_f64,
+                           int algo, uint64_t segment, int flags, void *stream);
 /* Reference COO MTTKRP, mttkrp (src/mttkrp.cpp:122-156), on the GPU:
  * out += MTTKRP over a device COO tensor, FP64. */
 int alto_hip_mttkrp_coo(const alto_hip_coo *coo, int target, int rank,
