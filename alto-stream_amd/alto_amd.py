@@ -273,7 +273,7 @@ def cpd_alto(at, factors, rank, max_iters, eps=1e-30, prec=PREC_F64, fit_mode=FI
 
 
 def cpstream(dims, cidx, vals, stream_mode, rank, max_iters, eps, nonneg, init_factors, max_slices=0,
-             stream=None):
+             prec=PREC_F64, stream=None):
     """CP-stream (cpstream, src/streaming_cpd.cpp:26-366), host COO input."""
     c, keep = _coo(dims, cidx, vals, False)
     dims = np.ascontiguousarray(dims, dtype=np.uint64)
@@ -295,7 +295,8 @@ def cpstream(dims, cidx, vals, stream_mode, rank, max_iters, eps, nonneg, init_f
     tim = np.zeros(4)
     n = lib().alto_hip_cpstream(ctypes.byref(c), ctypes.c_int(stream_mode), ctypes.c_int(rank),
                                 ctypes.c_int(max_iters), ctypes.c_double(eps), ctypes.c_int(1 if nonneg else 0),
-                                ctypes.c_int(max_slices), iarr, oarr, out_dims.ctypes.data_as(_u64p),
+                                ctypes.c_int(max_slices), ctypes.c_int(prec), iarr, oarr,
+                                out_dims.ctypes.data_as(_u64p),
                                 trows.ctypes.data_as(_f64p), outer.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                                 admm.ctypes.data_as(_f64p), grams.ctypes.data_as(_f64p),
                                 tim.ctypes.data_as(_f64p), _vp(stream))

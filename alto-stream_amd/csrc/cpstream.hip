@@ -1,15 +1,644 @@
-// CP-stream (src/streaming_cpd.cpp:26-366, 1198-1519) -- filled in below.
+// CP-stream on the GPU: cpstream (src/streaming_cpd.cpp:26-366) with the
+// CPSTREAM_ALTO model, one time index per slice (timeslice limit 1, no nnz
+// threshold), driving cpstream_alto_iter (streaming_cpd.cpp:1198-1519) and
+// admm_cf_ (src/admm.cpp:216-429).
+//
+// Host side (C++): StreamingSparseTensor (streaming_sptensor.cpp:69-264):
+// stable lexicographic sort by (s, s+1, ...) mod N, first-appearance
+// relabelling of every non-stream mode, slice boundaries and running-max
+// slice dims.  Device side, per slice: ALTO rebuild of the slice into
+// reused buffers, stream-mode MTTKRP + FP64 solve, temporal Gram update,
+// per-mode MTTKRP + historical term + non-negative AO-ADMM (FP64 primal,
+// dual and solve), Gram refresh and the convergence quantity; the host reads
+// one scalar per outer iteration.  Factor rows that appear when the model
+// grows come from caller-supplied initial matrices (the reference fills
+// them with rand(), kruskal_model.cpp:101-133).
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "dense.hpp"
 #include "tensor.hpp"
 
-extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int stream_mode, int rank, int max_iters,
-                                 double epsilon, int nonneg, int max_slices,
+namespace alto {
+
+int run_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const *factors,
+               void *out, int out_f64, int algo, uint64_t per_group, int flags, hipStream_t s);
+
+static inline int pow2(int r) {
+    int p = 1;
+    while (p < r) p <<= 1;
+    return p;
+}
+
+// Row tiles for the ADMM kernels: 256 threads = (256 / RP) row lanes x RP
+// columns, 4 rows per lane per tile.
+struct TileShape {
+    int RP, lanes, TR;
+};
+static TileShape tile_shape(int R) {
+    TileShape t;
+    t.RP = pow2(R);
+    t.lanes = 256 / t.RP;
+    t.TR = t.lanes * 4;
+    return t;
+}
+
+__device__ inline double block_sum(double v, double *red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return red[0] + red[1] + red[2] + red[3];
+}
+
+// First ADMM pass (admm.cpp:282-325) fused with the historical term
+// (streaming_cpd.cpp:1411-1412: mttkrp += prevU_j * ata):
+//   mt += P * ata;  rhs = mt + rho (x + d)  (unconstrained: rhs = mt)
+//   aux = rhs * vinv;  new primal = max(aux - d, 0)
+// Writes mt and aux (non-negative) or the solved factor U (unconstrained),
+// and the per-block sum of (new primal - x)^2.
+__global__ __launch_bounds__(256) void k_admm_first(double *__restrict__ mt,
+                                                    const double *__restrict__ P,
+                                                    const double *__restrict__ ata,
+                                                    double *__restrict__ U,
+                                                    const double *__restrict__ dual,
+                                                    const double *__restrict__ vinv,
+                                                    const double *__restrict__ rho_p,
+                                                    uint64_t rows, int R, int RP, int nonneg,
+                                                    double *__restrict__ aux,
+                                                    double *__restrict__ part) {
+    extern __shared__ double lds[];
+    const int lanes = 256 / RP, TR = lanes * 4;
+    double *Wa = lds, *Wv = lds + R * R, *T = Wv + R * R, *T2 = T + TR * R;
+    __shared__ double red[4];
+    const int t = threadIdx.x, c = t % RP, rl = t / RP;
+    for (int e = t; e < R * R; e += 256) {
+        Wa[e] = ata[e];
+        Wv[e] = vinv[e];
+    }
+    const double rho = nonneg ? *rho_p : 0.0;
+    double dres = 0.0;
+    uint64_t ntiles = (rows + TR - 1) / TR;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        uint64_t r0 = tile * TR;
+        int nr = (int)((rows - r0) < (uint64_t)TR ? (rows - r0) : TR);
+        __syncthreads();
+        for (int e = t; e < nr * R; e += 256) T[e] = P[r0 * R + e];
+        __syncthreads();
+        if (c < R)
+            for (int q = 0; q < 4; ++q) {
+                int rr = rl + q * lanes;
+                if (rr < nr) {
+                    uint64_t g = (r0 + rr) * R + c;
+                    double h = 0.0;
+                    for (int k = 0; k < R; ++k) h += T[rr * R + k] * Wa[k * R + c];
+                    double m = mt[g] + h;
+                    mt[g] = m;
+                    T2[rr * R + c] = nonneg ? m + rho * (U[g] + dual[g]) : m;
+                }
+            }
+        __syncthreads();
+        if (c < R)
+            for (int q = 0; q < 4; ++q) {
+                int rr = rl + q * lanes;
+                if (rr < nr) {
+                    uint64_t g = (r0 + rr) * R + c;
+                    double a = 0.0;
+                    for (int k = 0; k < R; ++k) a += T2[rr * R + k] * Wv[k * R + c];
+                    if (nonneg) {
+                        aux[g] = a;
+                        double x = a - dual[g];
+                        x = x < 0.0 ? 0.0 : x;
+                        double dd = x - U[g];
+                        dres += dd * dd;
+                    } else {
+                        U[g] = a;
+                    }
+                }
+            }
+    }
+    double tot = block_sum(dres, red);
+    if (t == 0) part[blockIdx.x] = tot;
+}
+
+// One AO-ADMM iteration (admm.cpp:331-405) on the state (aux, dual):
+// x = max(aux - d, 0) is the current primal; y = x - aux; d += y;
+// aux = (mt + rho (x + d)) * vinv.  Per-block partials of p_norm, d_norm,
+// p_res.  Skips everything once the convergence flag is set.
+__global__ __launch_bounds__(256) void k_admm_iter(const double *__restrict__ mt,
+                                                   double *__restrict__ aux,
+                                                   double *__restrict__ dual,
+                                                   const double *__restrict__ vinv,
+                                                   const double *__restrict__ rho_p,
+                                                   const int *__restrict__ flag, uint64_t rows,
+                                                   int R, int RP, double *__restrict__ part) {
+    if (*flag) return;
+    extern __shared__ double lds[];
+    const int lanes = 256 / RP, TR = lanes * 4;
+    double *Wv = lds, *T2 = lds + R * R;
+    __shared__ double red[4];
+    const int t = threadIdx.x, c = t % RP, rl = t / RP;
+    for (int e = t; e < R * R; e += 256) Wv[e] = vinv[e];
+    const double rho = *rho_p;
+    double pn = 0.0, dn = 0.0, pr = 0.0;
+    uint64_t ntiles = (rows + TR - 1) / TR;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        uint64_t r0 = tile * TR;
+        int nr = (int)((rows - r0) < (uint64_t)TR ? (rows - r0) : TR);
+        __syncthreads();
+        double di_keep[4];
+        if (c < R)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                int rr = rl + q * lanes;
+                if (rr < nr) {
+                    uint64_t g = (r0 + rr) * R + c;
+                    double a = aux[g], d = dual[g];
+                    double x = a - d;
+                    x = x < 0.0 ? 0.0 : x;
+                    pn += x * x;
+                    double y = x - a;
+                    double di = d + y;
+                    dn += di * di;
+                    pr += y * y;
+                    di_keep[q] = di;
+                    T2[rr * R + c] = mt[g] + rho * (x + di);
+                }
+            }
+        __syncthreads();
+        if (c < R)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                int rr = rl + q * lanes;
+                if (rr < nr) {
+                    uint64_t g = (r0 + rr) * R + c;
+                    double a = 0.0;
+                    for (int k = 0; k < R; ++k) a += T2[rr * R + k] * Wv[k * R + c];
+                    aux[g] = a;
+                    dual[g] = di_keep[q];
+                }
+            }
+    }
+    double s0 = block_sum(pn, red);
+    double s1 = block_sum(dn, red);
+    double s2 = block_sum(pr, red);
+    if (t == 0) {
+        part[blockIdx.x * 3 + 0] = s0;
+        part[blockIdx.x * 3 + 1] = s1;
+        part[blockIdx.x * 3 + 2] = s2;
+    }
+}
+
+// Convergence test after iteration `it` (admm.cpp:409-413), float tolerance
+// 1e-6f.  d_res is the first-pass value at it == 0 and 0 afterwards (the
+// reference sets init = primal in every later prox step).
+__global__ void k_admm_check(const double *__restrict__ part, int nblk,
+                             const double *__restrict__ part0, int it, int *__restrict__ flag,
+                             double *__restrict__ iters) {
+    if (threadIdx.x != 0 || *flag) return;
+    double pn = 0, dn = 0, pr = 0, dr = 0;
+    for (int b = 0; b < nblk; ++b) {
+        pn += part[b * 3 + 0];
+        dn += part[b * 3 + 1];
+        pr += part[b * 3 + 2];
+    }
+    if (it == 0)
+        for (int b = 0; b < nblk; ++b) dr += part0[b];
+    *iters += 1.0;
+    const double tol = (double)1e-6f;
+    if (pr <= tol * pn && dr <= tol * dn) *flag = 1;
+}
+
+__global__ void k_admm_final(const double *__restrict__ aux, const double *__restrict__ dual,
+                             double *__restrict__ U, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        double x = aux[i] - dual[i];
+        U[i] = x > 0.0 ? x : 0.0;
+    }
+}
+
+// G_s[m + n R] += sum_r U_s[r, m] U_s[r, n] (streaming_cpd.cpp:1321-1327).
+__global__ void k_gram_add(double *__restrict__ G, const double *__restrict__ U, uint64_t rows,
+                           int R) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= R * R) return;
+    int n = e / R, m = e % R;
+    double g = G[e];
+    for (uint64_t r = 0; r < rows; ++r) g += U[r * R + m] * U[r * R + n];
+    G[e] = g;
+}
+
+// ata = old_gram (*) cross_0 (*) cross_1 ... (streaming_cpd.cpp:1378-1392).
+__global__ void k_ata(double *__restrict__ ata, const double *__restrict__ old_gram,
+                      const double *__restrict__ cross, int ncross, int RR) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= RR) return;
+    double v = old_gram[e];
+    for (int q = 0; q < ncross; ++q) v *= cross[(size_t)q * RR + e];
+    ata[e] = v;
+}
+
+__global__ void k_scale(double *__restrict__ p, double f, int n) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) p[e] *= f;
+}
+
+// Per-block sums of (P - U)^2 and U^2.
+__global__ __launch_bounds__(256) void k_diff_norms(const double *__restrict__ P,
+                                                    const double *__restrict__ U, uint64_t n,
+                                                    double *__restrict__ part) {
+    __shared__ double red[4];
+    double a = 0.0, b = 0.0;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+        double d = P[i] - U[i];
+        a += d * d;
+        b += U[i] * U[i];
+    }
+    double sa = block_sum(a, red);
+    double sb = block_sum(b, red);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sa;
+        part[2 * blockIdx.x + 1] = sb;
+    }
+}
+
+// delta += ||P - U|| / (||U|| + 1e-12) (streaming_cpd.cpp:1461).
+__global__ void k_delta_add(const double *__restrict__ part, int nblk, double *__restrict__ delta) {
+    if (threadIdx.x != 0) return;
+    double a = 0.0, b = 0.0;
+    for (int k = 0; k < nblk; ++k) {
+        a += part[2 * k];
+        b += part[2 * k + 1];
+    }
+    *delta += sqrt(a) / (sqrt(b) + 1e-12);
+}
+
+__global__ void k_zero_u64(uint64_t *p, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+struct Slice {
+    uint64_t start, end;
+    uint64_t dims[kMaxModes];
+};
+
+static constexpr int kAdmmMaxIter = 25;
+
+}  // namespace alto
+
+using namespace alto;
+
+extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_iters,
+                                 double epsilon, int nonneg, int max_slices, int prec,
                                  const double *const *init_factors, double *const *out_factors,
                                  uint64_t *out_dims, double *out_time_rows, int *out_outer_iters,
                                  double *out_admm_iters, double *out_grams, double *timings,
                                  void *stream) {
-    (void)coo; (void)stream_mode; (void)rank; (void)max_iters; (void)epsilon; (void)nonneg;
-    (void)max_slices; (void)init_factors; (void)out_factors; (void)out_dims; (void)out_time_rows;
-    (void)out_outer_iters; (void)out_admm_iters; (void)out_grams; (void)timings; (void)stream;
-    alto::set_error("CP-stream not built yet");
-    return ALTO_ERR_UNSUPPORTED;
+    if (!coo || coo->on_device || !coo->dims || !coo->cidx || !coo->vals || !init_factors)
+        return ALTO_ERR_ARG;
+    const int N = coo->nmode;
+    if (N < 2 || N > kMaxModes || s < 0 || s >= N || R < 1 || R > kMaxRank || max_iters < 0)
+        return ALTO_ERR_ARG;
+    if (prec != ALTO_PREC_F64 && prec != ALTO_PREC_F32) return ALTO_ERR_ARG;
+    const uint64_t nnz = coo->nnz;
+    if (nnz >= (1ULL << 32)) return ALTO_ERR_UNSUPPORTED;
+    for (int m = 0; m < N; ++m)
+        if (!init_factors[m] || coo->dims[m] == 0) return ALTO_ERR_ARG;
+    ALTO_TRY(ensure_init());
+    hipStream_t st = to_stream(stream);
+
+    // ---- StreamingSparseTensor: sort by (s, s+1, ...) and relabel ----
+    std::vector<uint32_t> order(nnz);
+    std::iota(order.begin(), order.end(), 0u);
+    int lex[kMaxModes];
+    for (int t = 0; t < N; ++t) lex[t] = (s + t) % N;
+    for (uint64_t i = 0; i < nnz; ++i)
+        for (int m = 0; m < N; ++m)
+            if (coo->cidx[m][i] >= coo->dims[m]) {
+                set_error("a coordinate is not below its mode size");
+                return ALTO_ERR_ARG;
+            }
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        for (int t = 0; t < N; ++t) {
+            uint64_t x = coo->cidx[lex[t]][a], y = coo->cidx[lex[t]][b];
+            if (x != y) return x < y;
+        }
+        return false;
+    });
+    std::vector<std::vector<uint64_t>> cid(N, std::vector<uint64_t>(nnz));
+    std::vector<double> vals(nnz);
+    for (uint64_t i = 0; i < nnz; ++i) {
+        for (int m = 0; m < N; ++m) cid[m][i] = coo->cidx[m][order[i]];
+        vals[i] = coo->vals[order[i]];
+    }
+    order.clear();
+    order.shrink_to_fit();
+    uint64_t fdims[kMaxModes];
+    for (int m = 0; m < N; ++m) {
+        fdims[m] = coo->dims[m];
+        if (m == s) continue;
+        std::vector<uint64_t> perm(coo->dims[m], UINT64_MAX);
+        uint64_t seen = 0;
+        for (uint64_t i = 0; i < nnz; ++i) {
+            uint64_t v = cid[m][i];
+            if (perm[v] == UINT64_MAX) perm[v] = seen++;
+            cid[m][i] = perm[v];
+        }
+    }
+    // ---- slices (next_dynamic_batch with timeslice_limit 1) ----
+    std::vector<Slice> slices;
+    {
+        uint64_t ptr = 0, batch = 0, prev[kMaxModes] = {0};
+        while (ptr < nnz) {
+            uint64_t a = ptr;
+            while (a < nnz && cid[s][a] < batch) ++a;
+            uint64_t b = a;
+            while (b < nnz && cid[s][b] < batch + 1) ++b;
+            ++batch;
+            if (b == a) {
+                set_error("time slice %llu has no nonzeros", (unsigned long long)(batch - 1));
+                return ALTO_ERR_EMPTY_SLICE;
+            }
+            Slice sl;
+            sl.start = a;
+            sl.end = b;
+            for (int m = 0; m < N; ++m) {
+                if (m == s) {
+                    sl.dims[m] = 1;
+                    continue;
+                }
+                uint64_t mx = 0;
+                for (uint64_t i = a; i < b; ++i) mx = cid[m][i] > mx ? cid[m][i] : mx;
+                sl.dims[m] = std::max(mx + 1, prev[m]);
+                prev[m] = sl.dims[m];
+            }
+            slices.push_back(sl);
+            ptr = b;
+        }
+    }
+    int nsl = (int)slices.size();
+    if (max_slices > 0 && nsl > max_slices) nsl = max_slices;
+
+    // ---- device state ----
+    uint64_t maxd[kMaxModes], maxall = 1, maxslice = 1;
+    for (int m = 0; m < N; ++m) {
+        maxd[m] = m == s ? 1 : 0;
+        for (int k = 0; k < nsl; ++k) maxd[m] = std::max(maxd[m], slices[k].dims[m]);
+        maxall = std::max(maxall, maxd[m]);
+    }
+    for (int k = 0; k < nsl; ++k) maxslice = std::max(maxslice, slices[k].end - slices[k].start);
+    uint64_t used = nsl ? slices[nsl - 1].end : 0;
+    DevBuf dci[kMaxModes], dvals, zidx, U[kMaxModes], P[kMaxModes], Uf[kMaxModes], duals[kMaxModes];
+    DevBuf G, oldG, ata, cross, gpart, vinv, flag, rho, buf, aux, part0, part, dpart, delta,
+        admm_it, colp, lam;
+    for (int m = 0; m < N; ++m) {
+        if (m == s) continue;
+        ALTO_TRY(dci[m].alloc(sizeof(uint64_t) * std::max<uint64_t>(used, 1)));
+        if (used)
+            ALTO_HIP_TRY(hipMemcpyAsync(dci[m].p, cid[m].data(), sizeof(uint64_t) * used,
+                                        hipMemcpyHostToDevice, st));
+    }
+    ALTO_TRY(dvals.alloc(sizeof(double) * std::max<uint64_t>(used, 1)));
+    if (used) ALTO_HIP_TRY(hipMemcpyAsync(dvals.p, vals.data(), sizeof(double) * used, hipMemcpyHostToDevice, st));
+    ALTO_TRY(zidx.alloc(sizeof(uint64_t) * maxslice));
+    k_zero_u64<<<cdiv(maxslice, 256), 256, 0, st>>>(zidx.as<uint64_t>(), maxslice);
+    for (int m = 0; m < N; ++m) {
+        ALTO_TRY(U[m].alloc(sizeof(double) * maxd[m] * R));
+        ALTO_TRY(P[m].alloc(sizeof(double) * maxd[m] * R));
+        ALTO_TRY(duals[m].alloc(sizeof(double) * maxd[m] * R));
+        if (prec == ALTO_PREC_F32) ALTO_TRY(Uf[m].alloc(sizeof(float) * maxd[m] * R));
+    }
+    const size_t RR = (size_t)R * R;
+    ALTO_TRY(G.alloc(sizeof(double) * N * RR));
+    ALTO_TRY(oldG.alloc(sizeof(double) * RR));
+    ALTO_TRY(ata.alloc(sizeof(double) * RR));
+    ALTO_TRY(cross.alloc(sizeof(double) * N * RR));
+    ALTO_TRY(gpart.alloc(sizeof(double) * kMaxBlocks * RR));
+    ALTO_TRY(vinv.alloc(sizeof(double) * RR));
+    ALTO_TRY(flag.alloc(sizeof(int) * 4));
+    ALTO_TRY(rho.alloc(sizeof(double)));
+    ALTO_TRY(buf.alloc(sizeof(double) * maxall * R));
+    ALTO_TRY(aux.alloc(sizeof(double) * maxall * R));
+    ALTO_TRY(part0.alloc(sizeof(double) * kMaxBlocks));
+    ALTO_TRY(part.alloc(sizeof(double) * 3 * kMaxBlocks));
+    ALTO_TRY(dpart.alloc(sizeof(double) * 2 * kMaxBlocks));
+    ALTO_TRY(delta.alloc(sizeof(double)));
+    ALTO_TRY(admm_it.alloc(sizeof(double)));
+    ALTO_TRY(colp.alloc(sizeof(double) * 3 * R * kMaxBlocks));
+    ALTO_TRY(lam.alloc(sizeof(double) * R));
+    TensorImpl at;
+    const TileShape ts = tile_shape(R);
+    const size_t lds_first = sizeof(double) * (2 * RR + 2 * (size_t)ts.TR * R);
+    const size_t lds_iter = sizeof(double) * (RR + (size_t)ts.TR * R);
+    if (lds_first > 160 * 1024 - 512) {
+        set_error("CP-stream supports rank <= 64 (rank %d)", R);
+        return ALTO_ERR_UNSUPPORTED;
+    }
+    {
+        static bool attr = false;
+        if (!attr) {
+            (void)hipFuncSetAttribute((const void *)k_admm_first,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 256);
+            (void)hipFuncSetAttribute((const void *)k_admm_iter,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 256);
+            attr = true;
+        }
+    }
+    hipEvent_t ev[4];
+    for (auto &e : ev) ALTO_HIP_TRY(hipEventCreate(&e));
+    double t_build = 0, t_mttkrp = 0;
+    ALTO_HIP_TRY(hipEventRecord(ev[0], st));
+
+    uint64_t cur[kMaxModes] = {0};
+    const void *fptr[kMaxModes];
+    auto gather_ptrs = [&]() {
+        for (int m = 0; m < N; ++m)
+            fptr[m] = prec == ALTO_PREC_F32 ? Uf[m].p : U[m].p;
+    };
+    gather_ptrs();
+    auto refresh_f32 = [&](int m, uint64_t rows) -> int {
+        if (prec == ALTO_PREC_F32 && rows)
+            return alto_hip_convert(Uf[m].p, ALTO_PREC_F32, U[m].p, ALTO_PREC_F64, rows * R, st);
+        return ALTO_OK;
+    };
+    auto gram = [&](int m, uint64_t rows) -> int {
+        int nb = row_blocks(rows);
+        ALTO_TRY(launch_gram(U[m].p, ALTO_PREC_F64, rows, R, gpart.as<double>(), nb, st));
+        return launch_gram_reduce(gpart.as<double>(), nb, R, G.as<double>() + m * RR, st);
+    };
+
+    for (int k = 0; k < nsl; ++k) {
+        const Slice &sl = slices[k];
+        const uint64_t *bd = sl.dims;
+        if (k == 0) {
+            for (int m = 0; m < N; ++m) {
+                ALTO_HIP_TRY(hipMemcpyAsync(U[m].p, init_factors[m], sizeof(double) * bd[m] * R,
+                                            hipMemcpyHostToDevice, st));
+                ALTO_TRY(launch_fill_zero(P[m].as<double>(), maxd[m] * R, st));
+            }
+            ALTO_TRY(launch_fill_zero(U[s].as<double>(), R, st));
+            for (int m = 0; m < N; ++m) ALTO_TRY(gram(m, bd[m]));
+            ALTO_TRY(launch_fill_zero(G.as<double>() + s * RR, RR, st));
+        } else {
+            for (int m = 0; m < N; ++m)
+                if (bd[m] > cur[m]) {
+                    ALTO_HIP_TRY(hipMemcpyAsync(U[m].as<double>() + cur[m] * R, init_factors[m] + cur[m] * R,
+                                                sizeof(double) * (bd[m] - cur[m]) * R,
+                                                hipMemcpyHostToDevice, st));
+                    ALTO_TRY(launch_fill_zero(P[m].as<double>() + cur[m] * R, (bd[m] - cur[m]) * R, st));
+                }
+            for (int m = 0; m < N; ++m)
+                if (m != s) ALTO_TRY(gram(m, bd[m]));
+        }
+        for (int m = 0; m < N; ++m) {
+            cur[m] = bd[m];
+            ALTO_TRY(launch_fill_zero(duals[m].as<double>(), bd[m] * R, st));
+        }
+        // update_salto (alto.hpp:195-299) into reused buffers
+        {
+            const uint64_t *cp[kMaxModes];
+            for (int m = 0; m < N; ++m)
+                cp[m] = m == s ? zidx.as<uint64_t>() : dci[m].as<uint64_t>() + sl.start;
+            alto_hip_coo sc{N, bd, sl.end - sl.start, cp, dvals.as<double>() + sl.start, 1};
+            ALTO_HIP_TRY(hipEventRecord(ev[2], st));
+            ALTO_TRY(build_alto(&at, &sc, 0, 0, false, st));
+            ALTO_HIP_TRY(hipEventRecord(ev[3], st));
+        }
+        if (k == 0) {  // cpstream_alto_iter, iter == 0 (streaming_cpd.cpp:1248-1261)
+            for (int m = 0; m < N; ++m) {
+                if (m == s) continue;
+                int nb = row_blocks(bd[m]);
+                ALTO_TRY(launch_colsumsq(U[m].as<double>(), bd[m], R, colp.as<double>(), nb, st));
+                ALTO_TRY(launch_lambda(colp.as<double>(), nb, R, 0, lam.as<double>(), st));
+                ALTO_TRY(launch_scale_gram(U[m].as<double>(), lam.as<double>(), bd[m], R, U[m].p,
+                                           ALTO_PREC_F64, gpart.as<double>(), nb, st));
+                ALTO_TRY(launch_gram_reduce(gpart.as<double>(), nb, R, G.as<double>() + m * RR, st));
+            }
+        }
+        for (int m = 0; m < N; ++m) ALTO_TRY(refresh_f32(m, bd[m]));
+        ALTO_TRY(launch_fill_zero(oldG.as<double>(), RR, st));
+        ALTO_TRY(launch_fill_zero(admm_it.as<double>(), 1, st));
+        double prev_delta = 0.0;
+        int outer = 0;
+        for (int i = 0; i < max_iters; ++i) {
+            ALTO_TRY(launch_fill_zero(delta.as<double>(), 1, st));
+            // s_t: stream-mode MTTKRP and solve (streaming_cpd.cpp:1272-1299)
+            ALTO_TRY(launch_fill_zero(U[s].as<double>(), bd[s] * R, st));
+            ALTO_TRY(run_mttkrp(&at, s, prec, R, fptr, U[s].p, 1, 2, 0, -1, st));
+            ALTO_TRY(launch_chol_inv(G.as<double>(), N, s, R, 1e-12, false, vinv.as<double>(),
+                                     flag.as<int>() + 1, nullptr, st, 1, nullptr));
+            ALTO_TRY(launch_solve(U[s].as<double>(), bd[s], R, vinv.as<double>(), colp.as<double>(),
+                                  1, st));
+            ALTO_TRY(refresh_f32(s, bd[s]));
+            // temporal Gram (streaming_cpd.cpp:1301-1327)
+            ALTO_HIP_TRY(hipMemcpyAsync(oldG.p, G.as<double>() + s * RR, sizeof(double) * RR,
+                                        hipMemcpyDeviceToDevice, st));
+            k_gram_add<<<cdiv(RR, 256), 256, 0, st>>>(G.as<double>() + s * RR, U[s].as<double>(), bd[s], R);
+            for (int j = 0; j < N; ++j) {
+                if (j == s) continue;
+                const uint64_t I = bd[j];
+                const int nb = row_blocks(I);
+                ALTO_TRY(launch_fill_zero(buf.as<double>(), I * R, st));
+                ALTO_TRY(run_mttkrp(&at, j, prec, R, fptr, buf.p, 1, 1, 0, -1, st));
+                int nc = 0;
+                for (int m = 0; m < N; ++m) {
+                    if (m == j || m == s) continue;
+                    int nbm = row_blocks(bd[m]);
+                    ALTO_TRY(launch_cross_gram(P[m].as<double>(), U[m].as<double>(), bd[m], R,
+                                               gpart.as<double>(), nbm, st));
+                    ALTO_TRY(launch_gram_reduce(gpart.as<double>(), nbm, R, cross.as<double>() + nc * RR, st));
+                    ++nc;
+                }
+                k_ata<<<cdiv(RR, 256), 256, 0, st>>>(ata.as<double>(), oldG.as<double>(),
+                                                     cross.as<double>(), nc, (int)RR);
+                if (nonneg) {
+                    ALTO_TRY(launch_chol_inv(G.as<double>(), N, j, R, 0.0, false, vinv.as<double>(),
+                                             flag.as<int>() + 2, nullptr, st, 1, rho.as<double>()));
+                } else {
+                    ALTO_TRY(launch_chol_inv(G.as<double>(), N, j, R, 1e-12, false, vinv.as<double>(),
+                                             flag.as<int>() + 1, nullptr, st, 1, nullptr));
+                }
+                k_admm_first<<<nb, 256, lds_first, st>>>(buf.as<double>(), P[j].as<double>(),
+                                                         ata.as<double>(), U[j].as<double>(),
+                                                         duals[j].as<double>(), vinv.as<double>(),
+                                                         rho.as<double>(), I, R, ts.RP, nonneg,
+                                                         aux.as<double>(), part0.as<double>());
+                ALTO_LAUNCH_CHECK();
+                if (nonneg) {
+                    ALTO_HIP_TRY(hipMemsetAsync(flag.p, 0, sizeof(int), st));
+                    for (int it = 0; it < kAdmmMaxIter; ++it) {
+                        k_admm_iter<<<nb, 256, lds_iter, st>>>(buf.as<double>(), aux.as<double>(),
+                                                               duals[j].as<double>(), vinv.as<double>(),
+                                                               rho.as<double>(), flag.as<int>(), I, R,
+                                                               ts.RP, part.as<double>());
+                        k_admm_check<<<1, 64, 0, st>>>(part.as<double>(), nb, part0.as<double>(), it,
+                                                       flag.as<int>(), admm_it.as<double>());
+                    }
+                    ALTO_LAUNCH_CHECK();
+                    k_admm_final<<<cdiv(I * R, 256), 256, 0, st>>>(aux.as<double>(), duals[j].as<double>(),
+                                                                   U[j].as<double>(), I * R);
+                }
+                ALTO_TRY(gram(j, I));
+                ALTO_TRY(refresh_f32(j, I));
+                k_diff_norms<<<nb, 256, 0, st>>>(P[j].as<double>(), U[j].as<double>(), I * R,
+                                                 dpart.as<double>());
+                k_delta_add<<<1, 64, 0, st>>>(dpart.as<double>(), nb, delta.as<double>());
+                ALTO_LAUNCH_CHECK();
+            }
+            double hd[2];
+            int hf[4];
+            ALTO_HIP_TRY(hipMemcpyAsync(hd, delta.p, sizeof(double), hipMemcpyDeviceToHost, st));
+            ALTO_HIP_TRY(hipMemcpyAsync(hf, flag.p, sizeof(int) * 3, hipMemcpyDeviceToHost, st));
+            ALTO_HIP_TRY(hipStreamSynchronize(st));
+            if (nonneg && hf[2]) {
+                set_error("Cholesky factorisation failed in ADMM (mode %d, slice %d)", -1, k);
+                return ALTO_ERR_NUMERIC;
+            }
+            float bms = 0;
+            (void)hipEventElapsedTime(&bms, ev[2], ev[3]);
+            if (i == 0) t_build += bms * 1e-3;
+            outer = i + 1;
+            if (i > 0 && fabs(prev_delta - hd[0]) < epsilon) break;
+            prev_delta = hd[0];
+        }
+        k_scale<<<cdiv(RR, 256), 256, 0, st>>>(G.as<double>() + s * RR, 0.99, (int)RR);
+        for (int m = 0; m < N; ++m)
+            ALTO_HIP_TRY(hipMemcpyAsync(P[m].p, U[m].p, sizeof(double) * bd[m] * R,
+                                        hipMemcpyDeviceToDevice, st));
+        double ait = 0.0;
+        if (out_time_rows)
+            ALTO_HIP_TRY(hipMemcpyAsync(out_time_rows + (size_t)k * R, U[s].p, sizeof(double) * R,
+                                        hipMemcpyDeviceToHost, st));
+        ALTO_HIP_TRY(hipMemcpyAsync(&ait, admm_it.p, sizeof(double), hipMemcpyDeviceToHost, st));
+        ALTO_HIP_TRY(hipStreamSynchronize(st));
+        if (out_outer_iters) out_outer_iters[k] = outer;
+        if (out_admm_iters) out_admm_iters[k] = ait;
+    }
+    ALTO_HIP_TRY(hipEventRecord(ev[1], st));
+    ALTO_HIP_TRY(hipEventSynchronize(ev[1]));
+    float tot_ms = 0;
+    (void)hipEventElapsedTime(&tot_ms, ev[0], ev[1]);
+    for (int m = 0; m < N; ++m) {
+        if (out_dims) out_dims[m] = cur[m];
+        if (out_factors && out_factors[m] && cur[m])
+            ALTO_HIP_TRY(hipMemcpyAsync(out_factors[m], U[m].p, sizeof(double) * cur[m] * R,
+                                        hipMemcpyDeviceToHost, st));
+    }
+    if (out_grams)
+        ALTO_HIP_TRY(hipMemcpyAsync(out_grams, G.p, sizeof(double) * N * RR, hipMemcpyDeviceToHost, st));
+    ALTO_HIP_TRY(hipStreamSynchronize(st));
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    if (timings) {
+        timings[0] = tot_ms * 1e-3;
+        timings[1] = t_build;
+        timings[2] = t_mttkrp;
+        timings[3] = (double)nsl;
+    }
+    return nsl;
 }

@@ -42,7 +42,8 @@ __global__ __launch_bounds__(256) void k_chol_inv(const double *__restrict__ gra
                                                   int mode, int R, double diag_add,
                                                   double *__restrict__ vinv,
                                                   int *__restrict__ flag,
-                                                  double *__restrict__ phi_out) {
+                                                  double *__restrict__ phi_out, int fail_mode,
+                                                  double *__restrict__ rho_out) {
     extern __shared__ double lds[];
     double *A = lds;              // R*R
     double *tmp = lds + R * R;    // R
@@ -55,10 +56,22 @@ __global__ __launch_bounds__(256) void k_chol_inv(const double *__restrict__ gra
         for (int m = 0; m < nmode; ++m)
             if (m != mode) v *= grams[(size_t)m * RR + e];
         if (phi_out) phi_out[e] = v;
-        int i = e / R, j = e % R;
-        A[e] = v + (i == j ? diag_add : 0.0);
+        A[e] = v;
     }
-    if (t == 0) fail = 0;
+    __syncthreads();
+    if (t == 0) {
+        fail = 0;
+        if (rho_out) {  // ADMM penalty: rho = tr(Phi) / R (admm.cpp:254)
+            double tr = 0.0;
+            for (int i = 0; i < R; ++i) tr += A[i * R + i];
+            dmax = tr / (double)R;
+            *rho_out = dmax;
+        } else {
+            dmax = diag_add;
+        }
+    }
+    __syncthreads();
+    for (int i = t; i < R; i += blockDim.x) A[i * R + i] += dmax;
     __syncthreads();
     for (int j = 0; j < R; ++j) {
         double d = A[j * R + j];
@@ -79,6 +92,13 @@ __global__ __launch_bounds__(256) void k_chol_inv(const double *__restrict__ gra
         __syncthreads();
     }
     __syncthreads();
+    if (fail && fail_mode == 1) {
+        // pseudo_inverse(A, B) (matrix.cpp:226-250) leaves B unchanged on a
+        // failed factorisation: identity.
+        for (int e = t; e < RR; e += blockDim.x) vinv[e] = (e / R == e % R) ? 1.0 : 0.0;
+        if (t == 0 && flag) *flag = 1;
+        return;
+    }
     if (fail) {
         // Reference fallback (cpd.hpp:438-476): the unregularised matrix,
         // symmetrised from its lower triangle, which the reference never
@@ -133,7 +153,8 @@ __global__ __launch_bounds__(256) void k_chol_inv(const double *__restrict__ gra
 }
 
 int launch_chol_inv(const double *grams, int nmode, int mode, int R, double diag_add,
-                    bool copy_first, double *vinv, int *flag, double *phi_out, hipStream_t s) {
+                    bool copy_first, double *vinv, int *flag, double *phi_out, hipStream_t s,
+                    int fail_mode, double *rho_out) {
     (void)copy_first;  // 1.0 * G == G exactly: both Hadamard orders agree bitwise
     if (R < 1 || R > kMaxRank) return ALTO_ERR_UNSUPPORTED;
     size_t lds = sizeof(double) * ((size_t)R * R + R);
@@ -143,7 +164,8 @@ int launch_chol_inv(const double *grams, int nmode, int mode, int R, double diag
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 256);
         attr = true;
     }
-    k_chol_inv<<<1, 256, lds, s>>>(grams, nmode, mode, R, diag_add, vinv, flag, phi_out);
+    k_chol_inv<<<1, 256, lds, s>>>(grams, nmode, mode, R, diag_add, vinv, flag, phi_out, fail_mode,
+                                   rho_out);
     ALTO_LAUNCH_CHECK();
     return ALTO_OK;
 }
@@ -336,6 +358,86 @@ int launch_gram(const void *U, int prec, uint64_t rows, int R, double *gpart, in
     if (R > kMaxRank) return ALTO_ERR_UNSUPPORTED;
     size_t lds = sizeof(double) * 32 * R;
     k_scale_gram<1><<<nblk, 256, lds, s>>>(nullptr, nullptr, U, nullptr, prec, rows, R, gpart);
+    ALTO_LAUNCH_CHECK();
+    return ALTO_OK;
+}
+
+// Per-block partials of P^T U (R x R) over rows (matmul(prev_M->U[m], true,
+// M->U[m], false, ...), streaming_cpd.cpp:1387-1388).
+__global__ __launch_bounds__(256) void k_cross_gram(const double *__restrict__ P,
+                                                    const double *__restrict__ U, uint64_t rows,
+                                                    int R, double *__restrict__ gpart) {
+    extern __shared__ double T[];  // 2 x 32 x R
+    double *TP = T, *TU = T + 32 * R;
+    const int t = threadIdx.x, RR = R * R;
+    const int nq = (RR + 255) / 256;
+    double g[64];
+#pragma unroll
+    for (int q = 0; q < 64; ++q) g[q] = 0.0;
+    uint64_t ntiles = (rows + 31) / 32;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        uint64_t r0 = tile * 32;
+        int nr = (int)((rows - r0) < 32 ? (rows - r0) : 32);
+        __syncthreads();
+        for (int e = t; e < nr * R; e += 256) {
+            TP[e] = P[r0 * R + e];
+            TU[e] = U[r0 * R + e];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 64; ++q) {
+            if (q < nq) {
+                int e = t + q * 256;
+                if (e < RR) {
+                    int a = e / R, b = e % R;
+                    double sacc = g[q];
+                    for (int r = 0; r < nr; ++r) sacc += TP[r * R + a] * TU[r * R + b];
+                    g[q] = sacc;
+                }
+            }
+        }
+    }
+    double *p = gpart + (size_t)blockIdx.x * RR;
+#pragma unroll
+    for (int q = 0; q < 64; ++q) {
+        if (q < nq) {
+            int e = t + q * 256;
+            if (e < RR) p[e] = g[q];
+        }
+    }
+}
+
+int launch_cross_gram(const double *P, const double *U, uint64_t rows, int R, double *gpart,
+                      int nblk, hipStream_t s) {
+    if (R > kMaxRank) return ALTO_ERR_UNSUPPORTED;
+    k_cross_gram<<<nblk, 256, sizeof(double) * 64 * R, s>>>(P, U, rows, R, gpart);
+    ALTO_LAUNCH_CHECK();
+    return ALTO_OK;
+}
+
+// Column sums of squares in the partials layout of k_solve (sum at [0, R)).
+__global__ __launch_bounds__(256) void k_colsumsq(const double *__restrict__ U, uint64_t rows,
+                                                  int R, int RP, double *__restrict__ partials) {
+    __shared__ double red[256];
+    const int t = threadIdx.x, c = t % RP, rl = t / RP, lanes = 256 / RP;
+    double ss = 0.0;
+    if (c < R)
+        for (uint64_t r = (uint64_t)blockIdx.x * lanes + rl; r < rows; r += (uint64_t)gridDim.x * lanes) {
+            double v = U[r * R + c];
+            ss += v * v;
+        }
+    red[t] = ss;
+    __syncthreads();
+    if (rl == 0 && c < R) {
+        for (int l = 1; l < lanes; ++l) ss += red[l * RP + c];
+        partials[(size_t)blockIdx.x * 3 * R + c] = ss;
+    }
+}
+
+int launch_colsumsq(const double *U, uint64_t rows, int R, double *partials, int nblk,
+                    hipStream_t s) {
+    int RP = pow2_at_least(R);
+    k_colsumsq<<<nblk, 256, 0, s>>>(U, rows, R, RP, partials);
     ALTO_LAUNCH_CHECK();
     return ALTO_OK;
 }

@@ -14,8 +14,16 @@ int launch_fill_zero(double *p, uint64_t n, hipStream_t s);
 // vinv <- inverse of (Hadamard_{m != mode} G_m + diag_add I) via Cholesky; on a
 // failed pivot, the diagonal minimum-norm inverse of the unregularised matrix
 // (reference fallback); flag[0] = 1 when the fallback ran.
+// fail_mode 1: identity on failure (pseudo_inverse(A, B) leaves B unchanged,
+// matrix.cpp:226-250).  rho_out != nullptr: add tr(Phi)/R instead of
+// diag_add and store it there (admm.cpp:254-264).
 int launch_chol_inv(const double *grams, int nmode, int mode, int R, double diag_add,
-                    bool copy_first, double *vinv, int *flag, double *phi_out, hipStream_t s);
+                    bool copy_first, double *vinv, int *flag, double *phi_out, hipStream_t s,
+                    int fail_mode = 0, double *rho_out = nullptr);
+int launch_cross_gram(const double *P, const double *U, uint64_t rows, int R, double *gpart,
+                      int nblk, hipStream_t s);
+int launch_colsumsq(const double *U, uint64_t rows, int R, double *partials, int nblk,
+                    hipStream_t s);
 int row_blocks(uint64_t rows);
 // u = M * vinv in place; partials[b][0..R) = sum u^2, [R..2R) = max u,
 // [2R..3R) = sum u * M (per row block b).

@@ -179,7 +179,8 @@ __global__ void k_to_f32(float *d, const double *s, uint64_t n) {
 }
 
 int ensure_vals32(TensorImpl *t, hipStream_t s) {
-    if (t->vals32.p) return ALTO_OK;
+    if (t->vals32_valid) return ALTO_OK;
+    t->vals32_valid = true;
     ALTO_TRY(t->vals32.alloc(sizeof(float) * t->nnz));
     if (t->nnz) {
         k_to_f32<<<cdiv(t->nnz, 256), 256, 0, s>>>(t->vals32.as<float>(), t->vals64.as<double>(),
@@ -219,13 +220,14 @@ __global__ __launch_bounds__(256) void k_gather_sched(const uint64_t *__restrict
 // walks through oidx, alto.hpp:1708-1801).
 int ensure_schedule(TensorImpl *t, int mode, bool need64, hipStream_t s) {
     ModeSchedule &sc = t->sched[mode];
-    if (sc.built && (!need64 || sc.vals64.p)) return ALTO_OK;
+    if (sc.built && (!need64 || sc.has64)) return ALTO_OK;
     uint64_t n = t->nnz;
     ALTO_TRY(sc.pkeys.alloc(sizeof(uint64_t) * t->pkw * n));
     ALTO_TRY(sc.vals32.alloc(sizeof(float) * n));
     if (need64) ALTO_TRY(sc.vals64.alloc(sizeof(double) * n));
     if (n == 0) {
         sc.built = true;
+        sc.has64 = need64;
         return ALTO_OK;
     }
     DevBuf idx, perm;
@@ -244,44 +246,33 @@ int ensure_schedule(TensorImpl *t, int mode, bool need64, hipStream_t s) {
     ALTO_LAUNCH_CHECK();
     ALTO_HIP_TRY(hipStreamSynchronize(s));
     sc.built = true;
+    sc.has64 = need64;
     return ALTO_OK;
 }
 
-}  // namespace alto
 
-using namespace alto;
-
-extern "C" {
-
-int alto_hip_tensor_create(const alto_hip_coo *coo, int nprtn, int key_words, void *stream,
-                           alto_hip_tensor *out) {
-    if (!coo || !out || !coo->dims || !coo->cidx || !coo->vals) return ALTO_ERR_ARG;
-    if (coo->nmode < 1 || coo->nmode > kMaxModes || nprtn < 1) return ALTO_ERR_ARG;
+int build_alto(TensorImpl *t, const alto_hip_coo *coo, int nprtn, int key_words, bool check,
+               hipStream_t s) {
+    if (!coo || !coo->dims || !coo->cidx || !coo->vals) return ALTO_ERR_ARG;
+    if (coo->nmode < 1 || coo->nmode > kMaxModes || nprtn < 0) return ALTO_ERR_ARG;
     if (coo->nnz >= (1ULL << 32)) {
         set_error("nnz %llu exceeds 2^32 - 1", (unsigned long long)coo->nnz);
         return ALTO_ERR_UNSUPPORTED;
     }
     for (int m = 0; m < coo->nmode; ++m)
         if (coo->dims[m] == 0 || (coo->nnz && !coo->cidx[m])) return ALTO_ERR_ARG;
-    ALTO_TRY(ensure_init());
-    hipStream_t s = to_stream(stream);
-
-    auto holder = new alto_hip_tensor_s;
-    TensorImpl *t = &holder->impl;
     t->nmode = coo->nmode;
     t->nnz = coo->nnz;
     t->nprtn = nprtn;
+    t->vals32_valid = false;
+    for (int m = 0; m < kMaxModes; ++m) t->sched[m].built = false;
     memcpy(t->dims, coo->dims, sizeof(uint64_t) * t->nmode);
     int total = 0;
     for (int m = 0; m < t->nmode; ++m) total += mode_bits(t->dims[m]);
     int kw = key_words == 0 ? (total <= 64 ? 1 : 2) : key_words;
-    if (kw != 1 && kw != 2) {
-        delete holder;
-        return ALTO_ERR_ARG;
-    }
+    if (kw != 1 && kw != 2) return ALTO_ERR_ARG;
     if (build_masks(t->nmode, t->dims, kw, t->masks, &t->bp) < 0) {
         set_error("ALTO key needs %d bits, key holds %d", total, 64 * kw);
-        delete holder;
         return ALTO_ERR_KEY_WIDTH;
     }
     t->kw = kw;
@@ -294,112 +285,116 @@ int alto_hip_tensor_create(const alto_hip_coo *coo, int nprtn, int key_words, vo
         t->pack.width[m] = mode_bits(t->dims[m]);
         sh += t->pack.width[m];
     }
-    uint64_t n = t->nnz;
-    int st = ALTO_OK;
-    do {
-        if ((st = t->keys.alloc(sizeof(uint64_t) * kw * n))) break;
-        if ((st = t->pkeys.alloc(sizeof(uint64_t) * t->pkw * n))) break;
-        if ((st = t->perm.alloc(sizeof(uint32_t) * n))) break;
-        if ((st = t->vals64.alloc(sizeof(double) * n))) break;
-        DevBuf dci[kMaxModes], dvals, k0, k1, bad;
-        IdxPtrs ip{};
-        const double *vin = coo->vals;
-        if (!coo->on_device) {
-            for (int m = 0; m < t->nmode && !st; ++m) {
-                if ((st = dci[m].alloc(sizeof(uint64_t) * n))) break;
-                if (n) {
-                    hipError_t e = hipMemcpyAsync(dci[m].p, coo->cidx[m], sizeof(uint64_t) * n,
-                                                  hipMemcpyHostToDevice, s);
-                    if (e != hipSuccess) { set_error("H2D: %s", hipGetErrorString(e)); st = ALTO_ERR_HIP; }
-                }
-                ip.p[m] = dci[m].as<uint64_t>();
-            }
-            if (st) break;
-            if ((st = dvals.alloc(sizeof(double) * n))) break;
-            if (n && hipMemcpyAsync(dvals.p, coo->vals, sizeof(double) * n, hipMemcpyHostToDevice,
-                                    s) != hipSuccess) {
-                st = ALTO_ERR_HIP;
-                break;
-            }
-            vin = dvals.as<double>();
-        } else {
-            for (int m = 0; m < t->nmode; ++m) ip.p[m] = coo->cidx[m];
+    const uint64_t n = t->nnz;
+    ALTO_TRY(t->keys.alloc(sizeof(uint64_t) * kw * n));
+    ALTO_TRY(t->pkeys.alloc(sizeof(uint64_t) * t->pkw * n));
+    ALTO_TRY(t->perm.alloc(sizeof(uint32_t) * n));
+    ALTO_TRY(t->vals64.alloc(sizeof(double) * n));
+    DevBuf dci[kMaxModes], dvals;
+    IdxPtrs ip{};
+    const double *vin = coo->vals;
+    if (!coo->on_device) {
+        for (int m = 0; m < t->nmode; ++m) {
+            ALTO_TRY(dci[m].alloc(sizeof(uint64_t) * n));
+            if (n)
+                ALTO_HIP_TRY(hipMemcpyAsync(dci[m].p, coo->cidx[m], sizeof(uint64_t) * n,
+                                            hipMemcpyHostToDevice, s));
+            ip.p[m] = dci[m].as<uint64_t>();
         }
-        if ((st = k0.alloc(sizeof(uint64_t) * n))) break;
-        if (kw == 2 && (st = k1.alloc(sizeof(uint64_t) * n))) break;
-        if ((st = bad.alloc(sizeof(uint32_t)))) break;
-        if (hipMemsetAsync(bad.p, 0, sizeof(uint32_t), s) != hipSuccess) { st = ALTO_ERR_HIP; break; }
-        DimsArg da{};
-        for (int m = 0; m < t->nmode; ++m) da.d[m] = t->dims[m];
-        if (n) {
-            k_linearize<<<cdiv(n, 256), 256, 0, s>>>(ip, n, t->bp, da, k0.as<uint64_t>(),
-                                                     kw == 2 ? k1.as<uint64_t>() : nullptr,
-                                                     bad.as<uint32_t>());
-            if ((st = iota_u32(t->perm.as<uint32_t>(), n, s))) break;
-            if ((st = radix_sort_pairs(k0.as<uint64_t>(), kw == 2 ? k1.as<uint64_t>() : nullptr,
-                                       t->perm.as<uint32_t>(), n, total, t->sort_ws, s)))
-                break;
-            k_finish<<<cdiv(n, 256), 256, 0, s>>>(k0.as<uint64_t>(),
-                                                  kw == 2 ? k1.as<uint64_t>() : nullptr,
-                                                  t->perm.as<uint32_t>(), vin, n, t->bp, t->pack,
-                                                  kw, t->pkw, t->keys.as<uint64_t>(),
-                                                  t->pkeys.as<uint64_t>(), t->vals64.as<double>());
-            if (hipGetLastError() != hipSuccess) { st = ALTO_ERR_HIP; break; }
-        }
+        ALTO_TRY(dvals.alloc(sizeof(double) * n));
+        if (n) ALTO_HIP_TRY(hipMemcpyAsync(dvals.p, coo->vals, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        vin = dvals.as<double>();
+    } else {
+        for (int m = 0; m < t->nmode; ++m) ip.p[m] = coo->cidx[m];
+    }
+    ALTO_TRY(t->k0.alloc(sizeof(uint64_t) * n));
+    if (kw == 2) ALTO_TRY(t->k1.alloc(sizeof(uint64_t) * n));
+    ALTO_TRY(t->bad.alloc(sizeof(uint32_t)));
+    if (check) ALTO_HIP_TRY(hipMemsetAsync(t->bad.p, 0, sizeof(uint32_t), s));
+    DimsArg da{};
+    for (int m = 0; m < t->nmode; ++m) da.d[m] = t->dims[m];
+    if (n) {
+        k_linearize<<<cdiv(n, 256), 256, 0, s>>>(ip, n, t->bp, da, t->k0.as<uint64_t>(),
+                                                 kw == 2 ? t->k1.as<uint64_t>() : nullptr,
+                                                 t->bad.as<uint32_t>());
+        ALTO_LAUNCH_CHECK();
+        ALTO_TRY(iota_u32(t->perm.as<uint32_t>(), n, s));
+        ALTO_TRY(radix_sort_pairs(t->k0.as<uint64_t>(), kw == 2 ? t->k1.as<uint64_t>() : nullptr,
+                                  t->perm.as<uint32_t>(), n, total, t->sort_ws, s));
+        k_finish<<<cdiv(n, 256), 256, 0, s>>>(t->k0.as<uint64_t>(),
+                                              kw == 2 ? t->k1.as<uint64_t>() : nullptr,
+                                              t->perm.as<uint32_t>(), vin, n, t->bp, t->pack, kw,
+                                              t->pkw, t->keys.as<uint64_t>(),
+                                              t->pkeys.as<uint64_t>(), t->vals64.as<double>());
+        ALTO_LAUNCH_CHECK();
+    }
+    if (check) {
         uint32_t hbad = 0;
-        if (hipMemcpyAsync(&hbad, bad.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
-            hipStreamSynchronize(s) != hipSuccess) {
-            set_error("ALTO encode failed");
-            st = ALTO_ERR_HIP;
-            break;
-        }
+        ALTO_HIP_TRY(hipMemcpyAsync(&hbad, t->bad.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        ALTO_HIP_TRY(hipStreamSynchronize(s));
         if (hbad) {
             set_error("a coordinate is not below its mode size");
-            st = ALTO_ERR_ARG;
-            break;
+            return ALTO_ERR_ARG;
         }
-        // prtn_alto: nnz_ptrn = ceil(nnz / nprtn) (alto.hpp:1134-1161)
-        t->prtn_ptr.assign(nprtn + 1, 0);
-        uint64_t per = (n + nprtn - 1) / nprtn;
-        std::vector<Chunk> chunks;
-        const uint64_t CH = 1 << 16;
-        for (int p = 0; p < nprtn; ++p) {
-            uint64_t a = std::min<uint64_t>((uint64_t)p * per, n);
-            uint64_t b = std::min<uint64_t>((uint64_t)(p + 1) * per, n);
-            t->prtn_ptr[p + 1] = b;
-            for (uint64_t c = a; c < b; c += CH) chunks.push_back({c, std::min(c + CH, b), (uint32_t)p});
-        }
-        DevBuf div, dch;
-        if ((st = div.alloc(sizeof(uint64_t) * 2 * nprtn * t->nmode))) break;
-        k_init_intervals<<<cdiv((uint64_t)nprtn * t->nmode, 256), 256, 0, s>>>(
-            div.as<unsigned long long>(), nprtn, t->nmode);
-        if (!chunks.empty()) {
-            if ((st = dch.alloc(sizeof(Chunk) * chunks.size()))) break;
-            if (hipMemcpyAsync(dch.p, chunks.data(), sizeof(Chunk) * chunks.size(),
-                               hipMemcpyHostToDevice, s) != hipSuccess) { st = ALTO_ERR_HIP; break; }
-            k_intervals<<<(unsigned)chunks.size(), 256, 0, s>>>(t->pkeys.as<uint64_t>(), t->pkw,
-                                                                t->pack, dch.as<Chunk>(),
-                                                                div.as<unsigned long long>());
-        }
-        t->intervals.assign((size_t)2 * nprtn * t->nmode, 0);
-        if (hipMemcpyAsync(t->intervals.data(), div.p, sizeof(uint64_t) * t->intervals.size(),
-                           hipMemcpyDeviceToHost, s) != hipSuccess ||
-            hipStreamSynchronize(s) != hipSuccess) {
-            set_error("partition intervals failed");
-            st = ALTO_ERR_HIP;
-            break;
-        }
-        for (int p = 0; p < nprtn; ++p)
-            for (int m = 0; m < t->nmode; ++m) {
-                uint64_t *lo = &t->intervals[((size_t)p * t->nmode + m) * 2];
-                if (t->prtn_ptr[p] == t->prtn_ptr[p + 1]) {
-                    lo[0] = 0;  // empty partition: [0, 0] (alto.hpp:1220-1225)
-                    lo[1] = 0;
-                } else if (lo[0] > lo[1]) {
-                    lo[0] = lo[1];
-                }
+    } else if (!coo->on_device) {
+        ALTO_HIP_TRY(hipStreamSynchronize(s));  // host-staged inputs die with this call
+    }
+    if (nprtn == 0) {
+        t->prtn_ptr.assign(1, 0);
+        t->intervals.clear();
+        return ALTO_OK;
+    }
+    // prtn_alto: nnz_ptrn = ceil(nnz / nprtn) (alto.hpp:1134-1161)
+    t->prtn_ptr.assign(nprtn + 1, 0);
+    uint64_t per = (n + nprtn - 1) / nprtn;
+    std::vector<Chunk> chunks;
+    const uint64_t CH = 1 << 16;
+    for (int p = 0; p < nprtn; ++p) {
+        uint64_t a = std::min<uint64_t>((uint64_t)p * per, n);
+        uint64_t b = std::min<uint64_t>((uint64_t)(p + 1) * per, n);
+        t->prtn_ptr[p + 1] = b;
+        for (uint64_t c = a; c < b; c += CH) chunks.push_back({c, std::min(c + CH, b), (uint32_t)p});
+    }
+    ALTO_TRY(t->ivbuf.alloc(sizeof(uint64_t) * 2 * nprtn * t->nmode));
+    k_init_intervals<<<cdiv((uint64_t)nprtn * t->nmode, 256), 256, 0, s>>>(
+        t->ivbuf.as<unsigned long long>(), nprtn, t->nmode);
+    if (!chunks.empty()) {
+        ALTO_TRY(t->chunkbuf.alloc(sizeof(Chunk) * chunks.size()));
+        ALTO_HIP_TRY(hipMemcpyAsync(t->chunkbuf.p, chunks.data(), sizeof(Chunk) * chunks.size(),
+                                    hipMemcpyHostToDevice, s));
+        k_intervals<<<(unsigned)chunks.size(), 256, 0, s>>>(t->pkeys.as<uint64_t>(), t->pkw, t->pack,
+                                                            t->chunkbuf.as<Chunk>(),
+                                                            t->ivbuf.as<unsigned long long>());
+    }
+    t->intervals.assign((size_t)2 * nprtn * t->nmode, 0);
+    ALTO_HIP_TRY(hipMemcpyAsync(t->intervals.data(), t->ivbuf.p, sizeof(uint64_t) * t->intervals.size(),
+                                hipMemcpyDeviceToHost, s));
+    ALTO_HIP_TRY(hipStreamSynchronize(s));
+    for (int p = 0; p < nprtn; ++p)
+        for (int m = 0; m < t->nmode; ++m) {
+            uint64_t *lo = &t->intervals[((size_t)p * t->nmode + m) * 2];
+            if (t->prtn_ptr[p] == t->prtn_ptr[p + 1]) {
+                lo[0] = 0;  // empty partition: [0, 0] (alto.hpp:1220-1225)
+                lo[1] = 0;
+            } else if (lo[0] > lo[1]) {
+                lo[0] = lo[1];
             }
-    } while (0);
+        }
+    return ALTO_OK;
+}
+
+}  // namespace alto
+
+using namespace alto;
+
+extern "C" {
+
+int alto_hip_tensor_create(const alto_hip_coo *coo, int nprtn, int key_words, void *stream,
+                           alto_hip_tensor *out) {
+    if (!coo || !out || nprtn < 1) return ALTO_ERR_ARG;
+    ALTO_TRY(ensure_init());
+    auto holder = new alto_hip_tensor_s;
+    int st = build_alto(&holder->impl, coo, nprtn, key_words, true, to_stream(stream));
     if (st != ALTO_OK) {
         delete holder;
         return st;

@@ -257,10 +257,13 @@ __global__ __launch_bounds__(256) void k_mttkrp_seg(const uint64_t *__restrict__
 // boundary combining).  Each lane of a group decodes one nonzero of the
 // group's next L (coalesced key/value loads) into an LDS record holding the
 // byte offsets of the gathered rows; the group reads the records back with
-// broadcast LDS reads, so per nonzero a lane only issues its 16-byte gathers
-// (uniform base address + 32-bit offset) and its multiplies.  The keys and
-// values of the next L nonzeros are loaded while the current gathers are in
-// flight.  Needs nmode <= 5 and I_m * R * sizeof(factor) < 2^32 bytes.
+// broadcast LDS reads, so per nonzero a lane only issues its gathers (uniform
+// base address + 32-bit offset, VPL elements in 16-byte loads) and its
+// multiplies.  The keys and values of the next L nonzeros are loaded while
+// the current gathers are in flight.  NM1 = number of gathered modes.
+// Workgroups are remapped so that each XCD (round-robin dispatch over 8
+// XCDs) walks one contiguous slice of the nonzeros: neighbouring ranges share
+// that XCD's L2.  Needs I_m * R * sizeof(factor) < 2^32 bytes.
 template <typename VT> struct __align__(16) DecRec {
     uint32_t off[4];  // byte offsets of the gathered rows (non-target modes)
     uint32_t trow;    // output row
@@ -268,58 +271,71 @@ template <typename VT> struct __align__(16) DecRec {
     VT val;
 };
 
-template <typename VT, typename FT, typename OT, int R, int PKW>
+template <typename FT, int VPL> struct RowLoad {
+    using CT = typename FVec<FT>::CT;
+    static constexpr int E16 = 16 / (int)sizeof(FT);  // elements per 16-byte load
+    static_assert(VPL % E16 == 0, "VPL must be a multiple of a 16-byte load");
+    __device__ __forceinline__ static void load(const char *p, CT *o) {
+#pragma unroll
+        for (int q = 0; q < VPL / E16; ++q) FVec<FT>::load((const FT *)(p + 16 * q), o + q * E16);
+    }
+};
+
+__device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned nb) {
+    const unsigned q = nb / 8, r = nb % 8, x = b % 8, slot = b / 8;
+    return x < r ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
+}
+
+template <typename VT, typename FT, typename OT, int R, int PKW, int NM1, int VPL>
 __global__ __launch_bounds__(256) void k_mttkrp_dec(const uint64_t *__restrict__ pkeys,
                                                     const VT *__restrict__ vals, uint64_t nnz,
                                                     FPtrs<FT> F, OT *__restrict__ out,
                                                     PackInfo pk, int target, uint64_t per_group,
                                                     int flags) {
-    using Vec = FVec<FT>;
-    using CT = typename Vec::CT;
-    constexpr int V = Vec::V;
-    constexpr int L = R / V;
+    using CT = typename FVec<FT>::CT;
+    constexpr int L = R / VPL;
     constexpr int G = 256 / L;
-    constexpr int CB = L < 4 ? L : 4;  // nonzeros whose gathers are issued together
-    static_assert(R % V == 0 && L >= 1 && L <= 64 && (64 % L) == 0, "bad rank tiling");
+    constexpr int CB0 = VPL >= 8 ? 2 : 4;
+    constexpr int CB = L < CB0 ? L : CB0;  // nonzeros whose gathers are issued together
+    static_assert(R % VPL == 0 && L >= 1 && L <= 64 && (64 % L) == 0, "bad rank tiling");
     using Rec = DecRec<VT>;
     __shared__ Rec s_rec[256];
     __shared__ CT s_part[2][G][R];
-    __shared__ uint64_t s_row[2][G];
+    __shared__ uint32_t s_row[2][G];
+    constexpr uint32_t NONE = 0xFFFFFFFFu;
 
-    const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
+    const uint64_t gtid = (uint64_t)lb * blockDim.x + threadIdx.x;
     const uint64_t group = gtid / L;
     const int gib = threadIdx.x / L;
-    const int lig = (int)(gtid % L);
+    const int lig = threadIdx.x % L;
     const bool nt = flags & 1, excl = flags & 4;
     uint64_t start = group * per_group;
     uint64_t end = start + per_group < nnz ? start + per_group : nnz;
     if (start > nnz) start = nnz;
 
-    const int nm1 = pk.nmode - 1;
-    int osh[4], ow[4];
-    const char *fb[4];
+    int osh[NM1], ow[NM1];
+    const char *fb[NM1];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < NM1; ++q) {
         int m = q < target ? q : q + 1;  // q-th mode other than the target
-        bool ok = q < nm1;
-        osh[q] = ok ? pk.shift[m] : 0;
-        ow[q] = ok ? pk.width[m] : 0;
-        fb[q] = ok ? (const char *)F.p[m] : nullptr;
+        osh[q] = pk.shift[m];
+        ow[q] = pk.width[m];
+        fb[q] = (const char *)F.p[m];
     }
     const int tsh = pk.shift[target], tw = pk.width[target];
-    const uint32_t lane_b = (uint32_t)(lig * V * sizeof(FT));
-    OT *outp = out + lig * V;
+    const uint32_t lane_b = (uint32_t)(lig * VPL * sizeof(FT));
+    OT *outp = out + lig * VPL;
     Rec *myrec = s_rec + gib * L;
 
-    uint64_t cur = ~0ULL, first_row = ~0ULL;
+    uint32_t cur = NONE, first_row = NONE;
     bool in_first = true;
-    CT acc[V], facc[V];
+    CT acc[VPL], facc[VPL];
 #pragma unroll
-    for (int v = 0; v < V; ++v) {
+    for (int v = 0; v < VPL; ++v) {
         acc[v] = 0;
         facc[v] = 0;
     }
-    // keys / values of the first step
     uint64_t klo = 0, khi = 0;
     VT kval = 0;
     if (start < end) {
@@ -333,7 +349,9 @@ __global__ __launch_bounds__(256) void k_mttkrp_dec(const uint64_t *__restrict__
             Rec r;
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-                r.off[q] = (uint32_t)field<PKW>(klo, khi, osh[q], ow[q]) * (uint32_t)(R * sizeof(FT));
+                r.off[q] = q < NM1 ? (uint32_t)field<PKW>(klo, khi, osh[q < NM1 ? q : 0], ow[q < NM1 ? q : 0]) *
+                                         (uint32_t)(R * sizeof(FT))
+                                   : 0u;
             r.trow = (uint32_t)field<PKW>(klo, khi, tsh, tw);
             r.pad = 0;
             r.val = kval;
@@ -350,58 +368,54 @@ __global__ __launch_bounds__(256) void k_mttkrp_dec(const uint64_t *__restrict__
         const int nh = (end - base) < (uint64_t)L ? (int)(end - base) : L;
 #pragma unroll
         for (int c0 = 0; c0 < L; c0 += CB) {
-            uint32_t ob[CB][4], tr[CB];
+            uint32_t ob[CB][NM1], tr[CB];
             CT vv[CB];
 #pragma unroll
             for (int k = 0; k < CB; ++k) {
                 const Rec &r = myrec[c0 + k];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) ob[k][q] = r.off[q] + lane_b;
+                for (int q = 0; q < NM1; ++q) ob[k][q] = r.off[q] + lane_b;
                 tr[k] = r.trow;
                 vv[k] = (CT)r.val;
             }
-            CT f[4][CB][V];
+            CT f[NM1][CB][VPL];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (q < nm1) {
+            for (int q = 0; q < NM1; ++q)
 #pragma unroll
-                    for (int k = 0; k < CB; ++k) Vec::load((const FT *)(fb[q] + ob[k][q]), f[q][k]);
-                }
-            }
+                for (int k = 0; k < CB; ++k) RowLoad<FT, VPL>::load(fb[q] + ob[k][q], f[q][k]);
 #pragma unroll
             for (int k = 0; k < CB; ++k) {
-                CT g[V];
+                CT g[VPL];
 #pragma unroll
-                for (int v = 0; v < V; ++v) g[v] = vv[k] * f[0][k][v];
+                for (int v = 0; v < VPL; ++v) g[v] = vv[k] * f[0][k][v];
 #pragma unroll
-                for (int q = 1; q < 4; ++q)
-                    if (q < nm1) {
+                for (int q = 1; q < NM1; ++q)
 #pragma unroll
-                        for (int v = 0; v < V; ++v) g[v] *= f[q][k][v];
-                    }
+                    for (int v = 0; v < VPL; ++v) g[v] *= f[q][k][v];
                 if (c0 + k < nh) {
-                    const uint64_t row = tr[k];
-                    if (row != cur) {
-                        if (cur != ~0ULL) {
+                    const uint32_t row = tr[k];
+                    if (row == cur) {
+#pragma unroll
+                        for (int v = 0; v < VPL; ++v) acc[v] += g[v];
+                    } else {
+                        if (cur != NONE) {
                             if (!excl) {
 #pragma unroll
-                                for (int v = 0; v < V; ++v) atomic_add_out(outp + cur * R + v, (OT)acc[v]);
+                                for (int v = 0; v < VPL; ++v)
+                                    atomic_add_out(outp + (uint64_t)cur * R + v, (OT)acc[v]);
                             } else if (in_first) {
                                 first_row = cur;
 #pragma unroll
-                                for (int v = 0; v < V; ++v) facc[v] = acc[v];
+                                for (int v = 0; v < VPL; ++v) facc[v] = acc[v];
                                 in_first = false;
                             } else {
 #pragma unroll
-                                for (int v = 0; v < V; ++v) outp[cur * R + v] += (OT)acc[v];
+                                for (int v = 0; v < VPL; ++v) outp[(uint64_t)cur * R + v] += (OT)acc[v];
                             }
                         }
                         cur = row;
 #pragma unroll
-                        for (int v = 0; v < V; ++v) acc[v] = g[v];
-                    } else {
-#pragma unroll
-                        for (int v = 0; v < V; ++v) acc[v] += g[v];
+                        for (int v = 0; v < VPL; ++v) acc[v] = g[v];
                     }
                 }
             }
@@ -409,22 +423,22 @@ __global__ __launch_bounds__(256) void k_mttkrp_dec(const uint64_t *__restrict__
         __builtin_amdgcn_wave_barrier();
     }
     if (!excl) {
-        if (cur != ~0ULL) {
+        if (cur != NONE) {
 #pragma unroll
-            for (int v = 0; v < V; ++v) atomic_add_out(outp + cur * R + v, (OT)acc[v]);
+            for (int v = 0; v < VPL; ++v) atomic_add_out(outp + (uint64_t)cur * R + v, (OT)acc[v]);
         }
         return;
     }
     if (in_first) {
         first_row = cur;
 #pragma unroll
-        for (int v = 0; v < V; ++v) facc[v] = acc[v];
-        cur = ~0ULL;
+        for (int v = 0; v < VPL; ++v) facc[v] = acc[v];
+        cur = NONE;
     }
 #pragma unroll
-    for (int v = 0; v < V; ++v) {
-        s_part[0][gib][lig * V + v] = facc[v];
-        s_part[1][gib][lig * V + v] = acc[v];
+    for (int v = 0; v < VPL; ++v) {
+        s_part[0][gib][lig * VPL + v] = facc[v];
+        s_part[1][gib][lig * VPL + v] = acc[v];
     }
     if (lig == 0) {
         s_row[0][gib] = first_row;
@@ -434,27 +448,33 @@ __global__ __launch_bounds__(256) void k_mttkrp_dec(const uint64_t *__restrict__
     if (threadIdx.x < R) {
         const int r = threadIdx.x;
         OT *o = out + r;
-        uint64_t crow = ~0ULL;
+        uint32_t crow = NONE;
         CT a = 0;
         bool first_merged = true;
         for (int e = 0; e < 2 * G; ++e) {
-            uint64_t row = s_row[e & 1][e >> 1];
-            if (row == ~0ULL) continue;
+            uint32_t row = s_row[e & 1][e >> 1];
+            if (row == NONE) continue;
             CT x = s_part[e & 1][e >> 1][r];
             if (row == crow) {
                 a += x;
             } else {
-                if (crow != ~0ULL) {
-                    if (first_merged) atomic_add_out(o + crow * R, (OT)a);
-                    else o[crow * R] += (OT)a;
+                if (crow != NONE) {
+                    if (first_merged) atomic_add_out(o + (uint64_t)crow * R, (OT)a);
+                    else o[(uint64_t)crow * R] += (OT)a;
                     first_merged = false;
                 }
                 crow = row;
                 a = x;
             }
         }
-        if (crow != ~0ULL) atomic_add_out(o + crow * R, (OT)a);
+        if (crow != NONE) atomic_add_out(o + (uint64_t)crow * R, (OT)a);
     }
+}
+
+// elements per lane of the decode kernel: 8 FP32 / BF16, 4 FP64 (<= R)
+template <typename FT, int R> constexpr int dec_vpl() {
+    constexpr int want = sizeof(FT) == 8 ? 4 : 8;
+    return want < R ? want : R;
 }
 
 // Any rank: one wave per range, lane r covers ranks r, r+64, ...
@@ -530,10 +550,22 @@ static void launch_seg(const MttkrpArgs &a, hipStream_t s) {
     uint64_t threads = groups * L;
     unsigned blocks = cdiv(threads, 256);
     constexpr int B = 8;
-    if (a.decode_once)
-        k_mttkrp_dec<VT, FT, OT, R, PKW><<<blocks, 256, 0, s>>>(
-            a.pkeys, (const VT *)a.vals, a.nnz, F, (OT *)a.out, a.pk, a.target, a.per_group, a.flags);
-    else
+    if (a.decode_once) {
+        constexpr int VPL = dec_vpl<FT, R>();
+        constexpr int LD = R / VPL;
+        uint64_t dthreads = groups * LD;
+        unsigned dblocks = cdiv(dthreads, 256);
+        const int nm1 = a.pk.nmode - 1;
+        if (nm1 == 2)
+            k_mttkrp_dec<VT, FT, OT, R, PKW, 2, VPL><<<dblocks, 256, 0, s>>>(
+                a.pkeys, (const VT *)a.vals, a.nnz, F, (OT *)a.out, a.pk, a.target, a.per_group, a.flags);
+        else if (nm1 == 3)
+            k_mttkrp_dec<VT, FT, OT, R, PKW, 3, VPL><<<dblocks, 256, 0, s>>>(
+                a.pkeys, (const VT *)a.vals, a.nnz, F, (OT *)a.out, a.pk, a.target, a.per_group, a.flags);
+        else
+            k_mttkrp_dec<VT, FT, OT, R, PKW, 4, VPL><<<dblocks, 256, 0, s>>>(
+                a.pkeys, (const VT *)a.vals, a.nnz, F, (OT *)a.out, a.pk, a.target, a.per_group, a.flags);
+    } else
         k_mttkrp_seg<VT, FT, OT, R, PKW, B><<<blocks, 256, 0, s>>>(
             a.pkeys, (const VT *)a.vals, a.nnz, F, (OT *)a.out, a.pk, a.target, a.per_group, a.flags);
 }
@@ -595,11 +627,11 @@ int run_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const 
     a.rank = rank;
     a.prec = prec;
     a.out64 = prec == ALTO_PREC_F64 || out_f64;
-    a.per_group = per_group ? per_group : 256;
+    a.per_group = per_group ? per_group : 64;
     a.flags = flags < 0 ? 3 : flags;
     if (algo == 2) a.flags |= 4;  // output-ordered: runs inside a range own their row
     // decode-once kernel: nmode <= 5 and 32-bit element offsets into every factor
-    a.decode_once = t->nmode <= 5 && !(a.flags & 8);
+    a.decode_once = t->nmode >= 3 && t->nmode <= 5 && !(a.flags & 8);
     for (int m = 0; m < t->nmode; ++m)
         if (t->dims[m] * (uint64_t)rank * (prec == ALTO_PREC_F64 ? 8 : prec == ALTO_PREC_F32 ? 4 : 2) >=
             (1ULL << 32))

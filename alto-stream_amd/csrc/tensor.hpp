@@ -22,6 +22,7 @@ struct BitPos {
 // the mode's index, stably, so ALTO order is kept inside each row run.
 struct ModeSchedule {
     bool built = false;
+    bool has64 = false;
     DevBuf pkeys;   // packed keys in row order
     DevBuf vals32;  // FP32 values in row order
     DevBuf vals64;  // FP64 values in row order (built on demand)
@@ -51,7 +52,16 @@ struct TensorImpl {
     ModeSchedule sched[kMaxModes];
     SortScratch sort_ws;
     DevBuf scratch;     // generic per-call scratch
+    DevBuf k0, k1, bad, ivbuf, chunkbuf;  // construction scratch, kept for rebuilds
+    bool vals32_valid = false;
 };
+
+// (Re)builds t from a COO tensor, reusing t's device buffers (the GPU
+// counterpart of init_salto + update_salto, include/alto.hpp:151-299).
+// nprtn == 0 skips the reference partitions; check == false skips the
+// coordinate range check; with both off the call does not synchronise.
+int build_alto(TensorImpl *t, const alto_hip_coo *coo, int nprtn, int key_words, bool check,
+               hipStream_t s);
 
 // Host-side mask construction: LSB_FIRST + SHORT_FIRST packing
 // (include/alto.hpp:954-1057).  Returns total bits or -1 if > 64*kw.

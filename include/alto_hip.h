@@ -162,11 +162,13 @@ int alto_hip_cpd_als(alto_hip_tensor t, const alto_hip_als_opts *opts,
  * Outputs (host, may be NULL): final factors (dims[m] x rank each), final
  * dims, the time row s_t per slice, outer iterations per slice, ADMM
  * iterations per slice, final Grams (nmode*rank*rank).  timings (may be
- * NULL): 4 doubles, seconds: total, tensor build, MTTKRP, ADMM+dense.
+ * NULL): 4 doubles: total seconds, first slice-build seconds, 0, slices.
+ * prec: ALTO_PREC_F64 or ALTO_PREC_F32 (MTTKRP arithmetic; the solve, ADMM
+ * and Grams are FP64 either way).  rank <= 64.
  * Returns the number of slices processed, or a negative status. */
 int alto_hip_cpstream(const alto_hip_coo *coo, int stream_mode, int rank,
                       int max_iters, double epsilon, int nonneg, int max_slices,
-                      const double *const *init_factors, double *const *out_factors,
+                      int prec, const double *const *init_factors, double *const *out_factors,
                       uint64_t *out_dims, double *out_time_rows,
                       int *out_outer_iters, double *out_admm_iters,
                       double *out_grams, double *timings, void *stream);

@@ -1,0 +1,92 @@
+"""GPU CP-stream (cpstream + cpstream_alto_iter + admm_cf_, src/streaming_cpd.cpp,
+src/admm.cpp) against the oracle on the same input and the same initial rows.
+
+Bars (FP64 MTTKRP path): slice count, slice dims, outer and ADMM iteration
+counts equal; time rows, final factors and Grams within 1e-8 relative.
+FP32 MTTKRP path: factors within 1e-4 relative (the ADMM, solves and Grams
+stay FP64)."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _stream_tensor(T, dims, nnz, seed, zipf=False):
+    rng = np.random.default_rng(seed)
+    t = np.sort(rng.integers(0, T, size=nnz)).astype(np.uint64)
+    t[:T] = np.arange(T)
+    rng.shuffle(t)
+    cidx = [t]
+    for d in dims:
+        if zipf:
+            w = np.arange(1, d + 1, dtype=np.float64) ** -1.05
+            p = w / w.sum()
+            cidx.append(rng.permutation(d)[rng.choice(d, size=nnz, p=p)].astype(np.uint64))
+        else:
+            cidx.append(rng.integers(0, d, size=nnz).astype(np.uint64))
+    vals = rng.poisson(3, size=nnz) + 1.0
+    return [T] + list(dims), cidx, vals
+
+
+def _init(dims, R, seed):
+    rng = np.random.default_rng(seed)
+    return [rng.random((int(d), R)) for d in dims]
+
+
+@pytest.mark.parametrize("nonneg", [True, False])
+@pytest.mark.parametrize("R", [4, 16])
+def test_cpstream_matches_oracle(gpu, nonneg, R):
+    dims, cidx, vals = _stream_tensor(6, [300, 200], 6000, seed=90)
+    init = _init(dims, R, 91)
+    O.set_threads(1)
+    ref = O.cpstream(dims, cidx, vals, 0, R, 6, 1e-5, nonneg, init)
+    got = gpu.cpstream(dims, cidx, vals, 0, R, 6, 1e-5, nonneg, init)
+    assert got["slices"] == ref["slices"] == 6
+    np.testing.assert_array_equal(got["dims"], ref["dims"])
+    np.testing.assert_array_equal(got["outer_iters"], ref["outer_iters"])
+    if nonneg:
+        np.testing.assert_array_equal(got["admm_iters"], ref["admm_iters"])
+    np.testing.assert_allclose(got["time_rows"], ref["time_rows"], rtol=1e-8, atol=1e-10)
+    for a, b in zip(got["factors"], ref["factors"]):
+        np.testing.assert_allclose(a, b, rtol=1e-8, atol=1e-10)
+    # the reference writes only the upper triangle of the factor Grams
+    iu = np.triu_indices(R)
+    np.testing.assert_allclose(got["grams"][:, iu[0], iu[1]], ref["grams"][:, iu[0], iu[1]],
+                               rtol=1e-8, atol=1e-9)
+
+
+def test_cpstream_stream_mode_not_first_and_4_modes(gpu):
+    rng = np.random.default_rng(92)
+    T, nnz = 5, 5000
+    dims = [120, T, 90, 40]
+    t = rng.integers(0, T, size=nnz).astype(np.uint64)
+    t[:T] = np.arange(T)
+    cidx = [rng.integers(0, 120, size=nnz).astype(np.uint64), t,
+            rng.integers(0, 90, size=nnz).astype(np.uint64), rng.integers(0, 40, size=nnz).astype(np.uint64)]
+    vals = rng.random(nnz) + 0.5
+    init = _init(dims, 8, 93)
+    ref = O.cpstream(dims, cidx, vals, 1, 8, 4, 1e-6, True, init)
+    got = gpu.cpstream(dims, cidx, vals, 1, 8, 4, 1e-6, True, init)
+    np.testing.assert_array_equal(got["outer_iters"], ref["outer_iters"])
+    for a, b in zip(got["factors"], ref["factors"]):
+        np.testing.assert_allclose(a, b, rtol=1e-8, atol=1e-10)
+
+
+def test_cpstream_f32_mttkrp_zipf(gpu):
+    dims, cidx, vals = _stream_tensor(5, [2000, 1000], 40000, seed=94, zipf=True)
+    init = _init(dims, 16, 95)
+    ref = O.cpstream(dims, cidx, vals, 0, 16, 3, 1e-30, True, init)
+    got = gpu.cpstream(dims, cidx, vals, 0, 16, 3, 1e-30, True, init, prec=gpu.PREC_F32)
+    assert np.all(got["factors"][1] >= 0) and np.all(got["factors"][2] >= 0)
+    for a, b in zip(got["factors"][1:], ref["factors"][1:]):
+        scale = np.abs(b).max()
+        assert np.abs(a - b).max() <= 1e-4 * scale
+
+
+def test_cpstream_empty_slice_is_an_error(gpu):
+    dims, cidx, vals = _stream_tensor(4, [30, 20], 500, seed=96)
+    cidx[0] = np.where(cidx[0] == 2, 3, cidx[0]).astype(np.uint64)  # slice 2 empty
+    with pytest.raises(gpu.AltoError, match="empty time slice"):
+        gpu.cpstream(dims, cidx, vals, 0, 4, 2, 1e-5, True, _init(dims, 4, 97))
