@@ -1,0 +1,235 @@
+// Host-side implementation of alto_amd.hpp: tensor I/O, the Kruskal model
+// container and thin wrappers over the C ABI.
+#include "alto_amd.hpp"
+
+#include <string.h>
+
+#include <string>
+
+using altoamd::check;
+
+// ---- sptensor.cpp:74-242 ----
+static void read_text(FILE *fp, SparseTensor *X, IType zero_based) {
+    // pass 1: modes, dims (max index, +1 when zero based), nnz
+    char *line = nullptr;
+    size_t len = 0;
+    ssize_t nread;
+    int nmodes = 0;
+    while ((nread = getline(&line, &len, fp)) != -1) {
+        if (nread > 1 && line[0] != '#') {
+            char *save = nullptr;
+            for (char *p = strtok_r(line, " \t\n", &save); p; p = strtok_r(nullptr, " \t\n", &save)) ++nmodes;
+            break;
+        }
+    }
+    --nmodes;
+    if (nmodes < 1 || nmodes > ALTO_MAX_MODES) {
+        fprintf(stderr, "ALTO: bad .tns header (%d modes)\n", nmodes);
+        exit(1);
+    }
+    std::vector<IType> dims(nmodes, 0);
+    IType nnz = 0;
+    rewind(fp);
+    while ((nread = getline(&line, &len, fp)) != -1) {
+        if (nread > 1 && line[0] != '#') {
+            char *p = line;
+            for (int m = 0; m < nmodes; ++m) {
+                IType v = strtoull(p, &p, 10) + zero_based;
+                if (v > dims[m]) dims[m] = v;
+            }
+            ++nnz;
+        }
+    }
+    X->nmodes = nmodes;
+    X->nnz = nnz;
+    X->dims = (IType *)malloc(sizeof(IType) * nmodes);
+    X->cidx = (IType **)malloc(sizeof(IType *) * nmodes);
+    for (int m = 0; m < nmodes; ++m) {
+        X->dims[m] = dims[m];
+        X->cidx[m] = (IType *)malloc(sizeof(IType) * (nnz ? nnz : 1));
+    }
+    X->vals = (FType *)malloc(sizeof(FType) * (nnz ? nnz : 1));
+    rewind(fp);
+    IType k = 0, one_based = zero_based ? 0 : 1;
+    while ((nread = getline(&line, &len, fp)) != -1) {
+        if (nread > 1 && line[0] != '#') {
+            char *p = line;
+            for (int m = 0; m < nmodes; ++m) X->cidx[m][k] = strtoull(p, &p, 10) - one_based;
+            X->vals[k] = strtod(p, &p);
+            ++k;
+        }
+    }
+    free(line);
+}
+
+void ImportSparseTensor(const char *file_path, FileFormat f, SparseTensor **X_, IType zero_based) {
+    FILE *fp = fopen(file_path, "r");
+    if (!fp) {
+        fprintf(stderr, "ALTO: cannot open %s\n", file_path);
+        exit(1);
+    }
+    SparseTensor *X = (SparseTensor *)calloc(1, sizeof(SparseTensor));
+    if (f == TEXT_FORMAT) {
+        read_text(fp, X, zero_based);
+    } else {
+        // .bin: u64 nmodes, u64 dims[nmodes], u64 nnz, u64 cidx per mode, f64 vals
+        IType nm = 0;
+        if (fread(&nm, sizeof(IType), 1, fp) != 1 || nm < 1 || nm > ALTO_MAX_MODES) {
+            fprintf(stderr, "ALTO: bad .bin header in %s\n", file_path);
+            exit(1);
+        }
+        X->nmodes = (int)nm;
+        X->dims = (IType *)malloc(sizeof(IType) * nm);
+        size_t ok = fread(X->dims, sizeof(IType), nm, fp);
+        ok += fread(&X->nnz, sizeof(IType), 1, fp);
+        X->cidx = (IType **)malloc(sizeof(IType *) * nm);
+        for (IType m = 0; m < nm; ++m) {
+            X->cidx[m] = (IType *)malloc(sizeof(IType) * (X->nnz ? X->nnz : 1));
+            ok += fread(X->cidx[m], sizeof(IType), X->nnz, fp);
+        }
+        X->vals = (FType *)malloc(sizeof(FType) * (X->nnz ? X->nnz : 1));
+        ok += fread(X->vals, sizeof(FType), X->nnz, fp);
+        if (ok != nm + 1 + (nm + 1) * X->nnz) {
+            fprintf(stderr, "ALTO: truncated .bin file %s\n", file_path);
+            exit(1);
+        }
+    }
+    fclose(fp);
+    *X_ = X;
+}
+
+void DestroySparseTensor(SparseTensor *X) {
+    if (!X) return;
+    for (int m = 0; m < X->nmodes; ++m) free(X->cidx[m]);
+    free(X->cidx);
+    free(X->dims);
+    free(X->vals);
+    free(X);
+}
+
+// ---- kruskal_model.cpp ----
+void CreateKruskalModel(int mode, IType *dims, IType rank, KruskalModel **M_) {
+    KruskalModel *M = (KruskalModel *)calloc(1, sizeof(KruskalModel));
+    M->mode = mode;
+    M->rank = rank;
+    M->dims = (IType *)malloc(sizeof(IType) * mode);
+    M->U = (FType **)malloc(sizeof(FType *) * mode);
+    for (int m = 0; m < mode; ++m) {
+        M->dims[m] = dims[m];
+        M->U[m] = (FType *)calloc(dims[m] * rank, sizeof(FType));
+    }
+    M->lambda = (FType *)malloc(sizeof(FType) * rank);
+    for (IType r = 0; r < rank; ++r) M->lambda[r] = 1.0;
+    *M_ = M;
+}
+
+static FType rand_val() {  // util.cpp:28-35
+    FType v = 3.0 * ((FType)rand() / (FType)RAND_MAX);
+    if (rand() % 2 == 0) v *= -1;
+    return v;
+}
+
+void KruskalModelRandomInit(KruskalModel *M, unsigned int /*seed: ignored, util.cpp:37-43*/) {
+    for (IType r = 0; r < M->rank; ++r) M->lambda[r] = 1.0;
+    for (int m = 0; m < M->mode; ++m)
+        for (IType i = 0; i < M->dims[m] * M->rank; ++i) M->U[m][i] = rand_val();
+}
+
+void DestroyKruskalModel(KruskalModel *M) {
+    if (!M) return;
+    for (int m = 0; m < M->mode; ++m) free(M->U[m]);
+    free(M->U);
+    free(M->dims);
+    free(M->lambda);
+    free(M);
+}
+
+// ---- device factors ----
+static size_t esize(int prec) { return prec == ALTO_PREC_F64 ? 8 : prec == ALTO_PREC_F32 ? 4 : 2; }
+
+DeviceFactors::DeviceFactors(int nmode_, const IType *dims_, IType rank_, int prec_)
+    : nmode(nmode_), prec(prec_), rank(rank_), U(nmode_, nullptr), dims(dims_, dims_ + nmode_) {
+    for (int m = 0; m < nmode; ++m) check(alto_hip_malloc(&U[m], esize(prec) * dims[m] * rank), "malloc");
+}
+
+DeviceFactors::~DeviceFactors() {
+    for (auto p : U) alto_hip_free(p);
+}
+
+void DeviceFactors::upload(int m, const FType *host) {
+    size_t n = dims[m] * rank;
+    if (prec == ALTO_PREC_F64) {
+        check(alto_hip_memcpy_h2d(U[m], host, 8 * n, nullptr), "h2d");
+        return;
+    }
+    void *tmp = nullptr;
+    check(alto_hip_malloc(&tmp, 8 * n), "malloc");
+    check(alto_hip_memcpy_h2d(tmp, host, 8 * n, nullptr), "h2d");
+    check(alto_hip_convert(U[m], prec, tmp, ALTO_PREC_F64, n, nullptr), "convert");
+    check(alto_hip_stream_sync(nullptr), "sync");
+    alto_hip_free(tmp);
+}
+
+void DeviceFactors::download(int m, FType *host) {
+    size_t n = dims[m] * rank;
+    if (prec == ALTO_PREC_F64) {
+        check(alto_hip_memcpy_d2h(host, U[m], 8 * n, nullptr), "d2h");
+        return;
+    }
+    void *tmp = nullptr;
+    check(alto_hip_malloc(&tmp, 8 * n), "malloc");
+    check(alto_hip_convert(tmp, ALTO_PREC_F64, U[m], prec, n, nullptr), "convert");
+    check(alto_hip_memcpy_d2h(host, tmp, 8 * n, nullptr), "d2h");
+    alto_hip_free(tmp);
+}
+
+void DeviceFactors::zero(int m) { check(alto_hip_memset(U[m], 0, esize(prec) * dims[m] * rank, nullptr), "memset"); }
+
+// ---- mttkrp.cpp:122-156 on the GPU ----
+void mttkrp(SparseTensor *X, KruskalModel *M, IType mode) {
+    std::vector<void *> dc(X->nmodes);
+    void *dv = nullptr;
+    for (int m = 0; m < X->nmodes; ++m) {
+        check(alto_hip_malloc(&dc[m], 8 * X->nnz), "malloc");
+        check(alto_hip_memcpy_h2d(dc[m], X->cidx[m], 8 * X->nnz, nullptr), "h2d");
+    }
+    check(alto_hip_malloc(&dv, 8 * X->nnz), "malloc");
+    check(alto_hip_memcpy_h2d(dv, X->vals, 8 * X->nnz, nullptr), "h2d");
+    DeviceFactors F(M->mode, M->dims, M->rank);
+    for (int m = 0; m < M->mode; ++m) F.upload(m, M->U[m]);
+    alto_hip_coo c{X->nmodes, (const uint64_t *)X->dims, X->nnz, (const uint64_t *const *)dc.data(),
+                   (const double *)dv, 1};
+    check(alto_hip_mttkrp_coo(&c, (int)mode, (int)M->rank, (const double *const *)F.U.data(),
+                              (double *)F.U[mode], nullptr),
+          "mttkrp");
+    F.download((int)mode, M->U[mode]);
+    for (auto p : dc) alto_hip_free(p);
+    alto_hip_free(dv);
+}
+
+// ---- streaming_cpd.cpp:26-366 ----
+int cpstream(SparseTensor *X, int rank, int max_iters, int streaming_mode, FType epsilon, IType seed,
+             bool nonneg, int max_slices, int prec) {
+    (void)seed;  // the reference seeds once with srand (main.cpp:188)
+    const int N = X->nmodes;
+    std::vector<std::vector<double>> init(N);
+    std::vector<const double *> ip(N);
+    for (int m = 0; m < N; ++m) {
+        init[m].resize(X->dims[m] * rank);
+        for (auto &v : init[m]) v = rand_val();
+        ip[m] = init[m].data();
+    }
+    alto_hip_coo c{N, (const uint64_t *)X->dims, X->nnz, (const uint64_t *const *)X->cidx, X->vals, 0};
+    std::vector<double> trows(X->dims[streaming_mode] * rank);
+    std::vector<int> outer(X->dims[streaming_mode]);
+    std::vector<double> admm(X->dims[streaming_mode]);
+    double tim[4] = {0, 0, 0, 0};
+    int n = alto_hip_cpstream(&c, streaming_mode, rank, max_iters, epsilon, nonneg ? 1 : 0, max_slices, prec,
+                              ip.data(), nullptr, nullptr, trows.data(), outer.data(), admm.data(), nullptr,
+                              tim, nullptr);
+    if (n < 0) altoamd::fail("cpstream", n);
+    for (int k = 0; k < n; ++k)
+        fprintf(stderr, "slice %d: outer iterations %d, ADMM iterations %.0f\n", k, outer[k], admm[k]);
+    printf("Total CPSTREAM time: %f s (%d slices, %.2f slices/s)\n", tim[0], n, n / (tim[0] > 0 ? tim[0] : 1));
+    return n;
+}

@@ -66,6 +66,8 @@ def main():
     ap.add_argument("--no-als", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=25_000_000)
+    ap.add_argument("--counters", action="store_true",
+                    help="untimed pass for rocprofv3 --pmc: each mode twice in FP32, BF16 and FP64")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -93,6 +95,16 @@ def main():
 
     rng = np.random.default_rng(W["seed"])
     F64 = [rng.random((d, R)) for d in dims]
+    if args.counters:
+        for p in (A.PREC_F32, A.PREC_BF16, A.PREC_F64):
+            dF = [A.to_prec(f, p) for f in F64]
+            os_ = [A.DeviceArray.zeros((d, R), np.float64 if p == A.PREC_F64 else np.float32) for d in dims]
+            for _ in range(2):
+                for n in range(N):
+                    A.mttkrp_alto_par(n, dF, R, at, os_[n], prec=p, out_f64=False, algo=args.algo)
+            A.synchronize()
+        print(json.dumps({"counters_pass": True}), flush=True)
+        return
     devF = [A.to_prec(f, prec) for f in F64]
     odt = np.float64 if prec == A.PREC_F64 else np.float32
     outs = [A.DeviceArray.zeros((d, R), odt) for d in dims]
@@ -146,7 +158,8 @@ def main():
                    "prec": args.prec, "mode_ms": [round(x, 4) for x in mode_ms],
                    "alto_create_s": round(t_create, 3), "device": dev},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(args.workload, args.prec),
                      "alg_bytes_per_launch": int(tot / N)},
     }
     for d in outs:
@@ -168,6 +181,23 @@ def main():
         result["cpu_baseline"] = cpu_baseline(A, at, dims, R, F64, args.cpu_sample)
     at.close()
     print(json.dumps(result), flush=True)
+
+
+def measured_traffic(workload, prec):
+    """HBM bytes per MTTKRP launch (FETCH_SIZE + WRITE_SIZE, scaled by a copy
+    kernel of known size) from the committed rocprofv3 summary of this
+    workload, or None when there is none."""
+    if workload != "c2":
+        return None
+    path = os.path.join(ROOT, "profiles", "mttkrp_c2_rank32.md")
+    key = "traffic_bytes_per_launch_%s:" % {"f32": "fp32", "bf16": "bf16", "f64": "fp64"}[prec]
+    try:
+        for line in open(path):
+            if line.startswith(key):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(A, at, dims, R, F64, sample):

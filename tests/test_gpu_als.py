@@ -74,3 +74,16 @@ def test_als_epsilon_stop_and_exact_fit_is_residual(gpu):
     np.add.at(X, tuple(c.astype(np.int64) for c in cidx), vals)
     M = np.einsum("ir,jr,kr,r->ijk", gf[0], gf[1], gf[2], gl)
     assert abs(fe[-1] - np.linalg.norm(X - M) / np.linalg.norm(X)) < 1e-9
+
+
+def test_als_f64_signed_init_matches_oracle(gpu):
+    # the reference initialises with +-3 U(0,1) (util.cpp:28-35): signed
+    # factors exercise the signed max-norm (kruskal_model.cpp:277-317)
+    dims, cidx, vals = rand_tensor([150, 100, 80], 20000, seed=85)
+    F = rand_factors(dims, 8, seed=86, lo=-3, hi=3)
+    ref = O.AltoTensor(dims, cidx, vals, nprtn=4)
+    rf, rl, rfits, rfe = ref.cpd_als(F, 8, 5, fit_mode=O.FIT_EXACT)
+    at = gpu.create_alto(dims, cidx, vals, nprtn=4)
+    gf, gl, gfits, gfe, _ = gpu.cpd_alto(at, F, 8, 5, prec=gpu.PREC_F64, fit_mode=gpu.FIT_EXACT)
+    np.testing.assert_allclose(gfe, rfe, rtol=0, atol=1e-8)
+    np.testing.assert_allclose(gl, rl, rtol=1e-6)
