@@ -38,10 +38,12 @@ __global__ __launch_bounds__(256) void k_fit(const double *__restrict__ grams, i
         for (int m = 0; m < nmode; ++m) v *= grams[(size_t)m * RR + e];
         tg[e] = v;
     }
-    for (int c = t; c < R; c += blockDim.x) {
+    for (int c = t >> 6; c < R; c += blockDim.x >> 6) {
         double s = 0.0;
-        for (int b = 0; b < nblk; ++b) s += partials[(size_t)b * 3 * R + 2 * R + c];
-        ipc[c] = s;
+        const int lane = t & 63;
+        for (int b = lane; b < nblk; b += 64) s += partials[(size_t)b * 3 * R + 2 * R + c];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+        if (lane == 0) ipc[c] = s;
     }
     __syncthreads();
     if (t == 0) {

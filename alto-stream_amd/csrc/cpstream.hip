@@ -199,15 +199,12 @@ __global__ __launch_bounds__(256) void k_admm_iter(const double *__restrict__ mt
 __global__ void k_admm_check(const double *__restrict__ part, int nblk,
                              const double *__restrict__ part0, int it, int *__restrict__ flag,
                              double *__restrict__ iters) {
-    if (threadIdx.x != 0 || *flag) return;
-    double pn = 0, dn = 0, pr = 0, dr = 0;
-    for (int b = 0; b < nblk; ++b) {
-        pn += part[b * 3 + 0];
-        dn += part[b * 3 + 1];
-        pr += part[b * 3 + 2];
-    }
-    if (it == 0)
-        for (int b = 0; b < nblk; ++b) dr += part0[b];
+    if (*flag) return;
+    double pn = wave_reduce_entry(part, nblk, 3, 0, 0);
+    double dn = wave_reduce_entry(part, nblk, 3, 1, 0);
+    double pr = wave_reduce_entry(part, nblk, 3, 2, 0);
+    double dr = it == 0 ? wave_reduce_entry(part0, nblk, 1, 0, 0) : 0.0;
+    if (threadIdx.x != 0) return;
     *iters += 1.0;
     const double tol = (double)1e-6f;
     if (pr <= tol * pn && dr <= tol * dn) *flag = 1;
@@ -269,13 +266,9 @@ __global__ __launch_bounds__(256) void k_diff_norms(const double *__restrict__ P
 
 // delta += ||P - U|| / (||U|| + 1e-12) (streaming_cpd.cpp:1461).
 __global__ void k_delta_add(const double *__restrict__ part, int nblk, double *__restrict__ delta) {
-    if (threadIdx.x != 0) return;
-    double a = 0.0, b = 0.0;
-    for (int k = 0; k < nblk; ++k) {
-        a += part[2 * k];
-        b += part[2 * k + 1];
-    }
-    *delta += sqrt(a) / (sqrt(b) + 1e-12);
+    double a = wave_reduce_entry(part, nblk, 2, 0, 0);
+    double b = wave_reduce_entry(part, nblk, 2, 1, 0);
+    if (threadIdx.x == 0) *delta += sqrt(a) / (sqrt(b) + 1e-12);
 }
 
 __global__ void k_zero_u64(uint64_t *p, uint64_t n) {

@@ -7,14 +7,15 @@
 // (src/gram.cpp:29-42).
 //
 // Every reduction is two-level with a fixed order (per-block partials, then
-// one block summing the partials in block order), so results do not depend
-// on scheduling.
+// one wave per entry summing the partials in a fixed lane/shuffle order), so
+// results do not depend on scheduling.
 #include <float.h>
 #include <math.h>
 
 #include "dense.hpp"
 
 namespace alto {
+
 
 __global__ void k_fill_zero(double *p, uint64_t n) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -29,7 +30,7 @@ int launch_fill_zero(double *p, uint64_t n, hipStream_t s) {
 }
 
 int row_blocks(uint64_t rows) {
-    uint64_t b = (rows + 31) / 32;
+    uint64_t b = (rows + 63) / 64;
     if (b < 1) b = 1;
     if (b > (uint64_t)kMaxBlocks) b = kMaxBlocks;
     return (int)b;
@@ -250,32 +251,27 @@ int launch_solve(double *M, uint64_t rows, int R, const double *vinv, double *pa
     return ALTO_OK;
 }
 
-__global__ void k_lambda(const double *__restrict__ partials, int nblk, int R, int which,
-                         double *__restrict__ lambda) {
-    int c = threadIdx.x;
+__global__ __launch_bounds__(256) void k_lambda(const double *__restrict__ partials, int nblk,
+                                                int R, int which, double *__restrict__ lambda) {
+    const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (c >= R) return;
     if (which == 0) {
-        double s = 0.0;
-        for (int b = 0; b < nblk; ++b) s += partials[(size_t)b * 3 * R + c];
-        lambda[c] = sqrt(s > 1e-12 ? s : 1e-12);
+        double s = wave_reduce_entry(partials, nblk, 3 * (size_t)R, c, 0);
+        if ((threadIdx.x & 63) == 0) lambda[c] = sqrt(s > 1e-12 ? s : 1e-12);
     } else {
-        double m = 1.0;
-        for (int b = 0; b < nblk; ++b) {
-            double v = partials[(size_t)b * 3 * R + R + c];
-            m = v > m ? v : m;
-        }
-        lambda[c] = m;
+        double m = wave_reduce_entry(partials, nblk, 3 * (size_t)R, R + c, 1);
+        if ((threadIdx.x & 63) == 0) lambda[c] = m > 1.0 ? m : 1.0;
     }
 }
 
 int launch_lambda(const double *partials, int nblk, int R, int which, double *lambda,
                   hipStream_t s) {
-    k_lambda<<<1, 256, 0, s>>>(partials, nblk, R, which, lambda);
+    k_lambda<<<cdiv(R, 4), 256, 0, s>>>(partials, nblk, R, which, lambda);
     ALTO_LAUNCH_CHECK();
     return ALTO_OK;
 }
 
-template <int MODE>  // 0: scale f64 u by 1/lambda and store; 1: read stored factor
+template <int MODE, int NQ>  // MODE 0: scale f64 u by 1/lambda and store; 1: read stored factor
 __global__ __launch_bounds__(256) void k_scale_gram(const double *__restrict__ u,
                                                     const double *__restrict__ lambda,
                                                     const void *__restrict__ Uin,
@@ -286,10 +282,9 @@ __global__ __launch_bounds__(256) void k_scale_gram(const double *__restrict__ u
     constexpr int TR = 32;
     const int t = threadIdx.x;
     const int RR = R * R;
-    const int nq = (RR + 255) / 256;
-    double g[64];
+    double g[NQ];
 #pragma unroll
-    for (int q = 0; q < 64; ++q) g[q] = 0.0;
+    for (int q = 0; q < NQ; ++q) g[q] = 0.0;
     uint64_t ntiles = (rows + TR - 1) / TR;
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         uint64_t r0 = tile * TR;
@@ -322,33 +317,43 @@ __global__ __launch_bounds__(256) void k_scale_gram(const double *__restrict__ u
         }
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < 64; ++q) {
-            if (q < nq) {
-                int e = t + q * 256;
-                if (e < RR) {
-                    int a = e / R, b = e % R;
-                    double s = g[q];
-                    for (int r = 0; r < nr; ++r) s += T[r * R + a] * T[r * R + b];
-                    g[q] = s;
-                }
+        for (int q = 0; q < NQ; ++q) {
+            int e = t + q * 256;
+            if (e < RR) {
+                int a = e / R, b = e % R;
+                double s = g[q];
+                for (int r = 0; r < nr; ++r) s += T[r * R + a] * T[r * R + b];
+                g[q] = s;
             }
         }
     }
     double *p = gpart + (size_t)blockIdx.x * RR;
 #pragma unroll
-    for (int q = 0; q < 64; ++q) {
-        if (q < nq) {
-            int e = t + q * 256;
-            if (e < RR) p[e] = g[q];
-        }
+    for (int q = 0; q < NQ; ++q) {
+        int e = t + q * 256;
+        if (e < RR) p[e] = g[q];
     }
+}
+
+template <int MODE>
+static void launch_sg(const double *u, const double *lambda, const void *Uin, void *Uout, int prec,
+                      uint64_t rows, int R, double *gpart, int nblk, hipStream_t s) {
+    size_t lds = sizeof(double) * 32 * R;
+    int nq = (R * R + 255) / 256;
+    if (nq <= 1)
+        k_scale_gram<MODE, 1><<<nblk, 256, lds, s>>>(u, lambda, Uin, Uout, prec, rows, R, gpart);
+    else if (nq <= 4)
+        k_scale_gram<MODE, 4><<<nblk, 256, lds, s>>>(u, lambda, Uin, Uout, prec, rows, R, gpart);
+    else if (nq <= 16)
+        k_scale_gram<MODE, 16><<<nblk, 256, lds, s>>>(u, lambda, Uin, Uout, prec, rows, R, gpart);
+    else
+        k_scale_gram<MODE, 64><<<nblk, 256, lds, s>>>(u, lambda, Uin, Uout, prec, rows, R, gpart);
 }
 
 int launch_scale_gram(const double *u, const double *lambda, uint64_t rows, int R, void *U,
                       int prec, double *gpart, int nblk, hipStream_t s) {
     if (R > kMaxRank) return ALTO_ERR_UNSUPPORTED;
-    size_t lds = sizeof(double) * 32 * R;
-    k_scale_gram<0><<<nblk, 256, lds, s>>>(u, lambda, nullptr, U, prec, rows, R, gpart);
+    launch_sg<0>(u, lambda, nullptr, U, prec, rows, R, gpart, nblk, s);
     ALTO_LAUNCH_CHECK();
     return ALTO_OK;
 }
@@ -356,24 +361,23 @@ int launch_scale_gram(const double *u, const double *lambda, uint64_t rows, int 
 int launch_gram(const void *U, int prec, uint64_t rows, int R, double *gpart, int nblk,
                 hipStream_t s) {
     if (R > kMaxRank) return ALTO_ERR_UNSUPPORTED;
-    size_t lds = sizeof(double) * 32 * R;
-    k_scale_gram<1><<<nblk, 256, lds, s>>>(nullptr, nullptr, U, nullptr, prec, rows, R, gpart);
+    launch_sg<1>(nullptr, nullptr, U, nullptr, prec, rows, R, gpart, nblk, s);
     ALTO_LAUNCH_CHECK();
     return ALTO_OK;
 }
 
 // Per-block partials of P^T U (R x R) over rows (matmul(prev_M->U[m], true,
 // M->U[m], false, ...), streaming_cpd.cpp:1387-1388).
+template <int NQ>
 __global__ __launch_bounds__(256) void k_cross_gram(const double *__restrict__ P,
                                                     const double *__restrict__ U, uint64_t rows,
                                                     int R, double *__restrict__ gpart) {
     extern __shared__ double T[];  // 2 x 32 x R
     double *TP = T, *TU = T + 32 * R;
     const int t = threadIdx.x, RR = R * R;
-    const int nq = (RR + 255) / 256;
-    double g[64];
+    double g[NQ];
 #pragma unroll
-    for (int q = 0; q < 64; ++q) g[q] = 0.0;
+    for (int q = 0; q < NQ; ++q) g[q] = 0.0;
     uint64_t ntiles = (rows + 31) / 32;
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         uint64_t r0 = tile * 32;
@@ -385,32 +389,37 @@ __global__ __launch_bounds__(256) void k_cross_gram(const double *__restrict__ P
         }
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < 64; ++q) {
-            if (q < nq) {
-                int e = t + q * 256;
-                if (e < RR) {
-                    int a = e / R, b = e % R;
-                    double sacc = g[q];
-                    for (int r = 0; r < nr; ++r) sacc += TP[r * R + a] * TU[r * R + b];
-                    g[q] = sacc;
-                }
+        for (int q = 0; q < NQ; ++q) {
+            int e = t + q * 256;
+            if (e < RR) {
+                int a = e / R, b = e % R;
+                double sacc = g[q];
+                for (int r = 0; r < nr; ++r) sacc += TP[r * R + a] * TU[r * R + b];
+                g[q] = sacc;
             }
         }
     }
     double *p = gpart + (size_t)blockIdx.x * RR;
 #pragma unroll
-    for (int q = 0; q < 64; ++q) {
-        if (q < nq) {
-            int e = t + q * 256;
-            if (e < RR) p[e] = g[q];
-        }
+    for (int q = 0; q < NQ; ++q) {
+        int e = t + q * 256;
+        if (e < RR) p[e] = g[q];
     }
 }
 
 int launch_cross_gram(const double *P, const double *U, uint64_t rows, int R, double *gpart,
                       int nblk, hipStream_t s) {
     if (R > kMaxRank) return ALTO_ERR_UNSUPPORTED;
-    k_cross_gram<<<nblk, 256, sizeof(double) * 64 * R, s>>>(P, U, rows, R, gpart);
+    size_t lds = sizeof(double) * 64 * R;
+    int nq = (R * R + 255) / 256;
+    if (nq <= 1)
+        k_cross_gram<1><<<nblk, 256, lds, s>>>(P, U, rows, R, gpart);
+    else if (nq <= 4)
+        k_cross_gram<4><<<nblk, 256, lds, s>>>(P, U, rows, R, gpart);
+    else if (nq <= 16)
+        k_cross_gram<16><<<nblk, 256, lds, s>>>(P, U, rows, R, gpart);
+    else
+        k_cross_gram<64><<<nblk, 256, lds, s>>>(P, U, rows, R, gpart);
     ALTO_LAUNCH_CHECK();
     return ALTO_OK;
 }
@@ -442,18 +451,17 @@ int launch_colsumsq(const double *U, uint64_t rows, int R, double *partials, int
     return ALTO_OK;
 }
 
-__global__ void k_gram_reduce(const double *__restrict__ gpart, int nblk, int RR,
-                              double *__restrict__ G) {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_gram_reduce(const double *__restrict__ gpart, int nblk,
+                                                     int RR, double *__restrict__ G) {
+    const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (e >= RR) return;
-    double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += gpart[(size_t)b * RR + e];
-    G[e] = s;
+    double s = wave_reduce_entry(gpart, nblk, RR, e, 0);
+    if ((threadIdx.x & 63) == 0) G[e] = s;
 }
 
 int launch_gram_reduce(const double *gpart, int nblk, int R, double *G, hipStream_t s) {
     int RR = R * R;
-    k_gram_reduce<<<cdiv(RR, 256), 256, 0, s>>>(gpart, nblk, RR, G);
+    k_gram_reduce<<<cdiv(RR, 4), 256, 0, s>>>(gpart, nblk, RR, G);
     ALTO_LAUNCH_CHECK();
     return ALTO_OK;
 }

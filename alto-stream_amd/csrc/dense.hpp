@@ -1,12 +1,32 @@
 // Dense R x R / I x R kernels shared by CP-ALS and CP-stream.
 #pragma once
+#include <math.h>
+
 #include "common.hpp"
 
 namespace alto {
 
 constexpr int kDenseThreads = 256;
-constexpr int kMaxBlocks = 512;   // row blocks per reduction (partials buffer size)
+constexpr int kMaxBlocks = 2048;  // row blocks per reduction (partials buffer size)
 constexpr int kMaxRank = 128;
+
+// One wave per output entry: lanes sum (op 0) or max (op 1) part[b * stride +
+// off] over blocks b = lane, lane + 64, ..., then a fixed shuffle tree; the
+// result is valid in lane 0.  Deterministic for a given nblk.
+__device__ __forceinline__ double wave_reduce_entry(const double *__restrict__ part, int nblk,
+                                                    size_t stride, size_t off, int op) {
+    const int lane = threadIdx.x & 63;
+    double a = op == 0 ? 0.0 : -INFINITY;
+    for (int b = lane; b < nblk; b += 64) {
+        double v = part[(size_t)b * stride + off];
+        a = op == 0 ? a + v : (v > a ? v : a);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        double v = __shfl_down(a, o, 64);
+        a = op == 0 ? a + v : (v > a ? v : a);
+    }
+    return a;
+}
 
 // Launch helpers (all FP64 R x R math, row-major).
 // grams: nmode consecutive R x R matrices.

@@ -190,61 +190,147 @@ int ensure_vals32(TensorImpl *t, hipStream_t s) {
     return ALTO_OK;
 }
 
-__global__ __launch_bounds__(256) void k_mode_index(const uint64_t *__restrict__ pkeys, int pkw,
-                                                    PackInfo pk, int mode, uint64_t n,
-                                                    uint64_t *__restrict__ out) {
+struct FiberOrder {
+    int n;
+    int mode[kMaxModes];  // most significant first
+};
+
+// Composite sort key of a nonzero: its fields concatenated in `ord` order
+// (first mode in the top bits), at most 128 bits.
+__global__ __launch_bounds__(256) void k_fiber_key(const uint64_t *__restrict__ pkeys, int pkw,
+                                                   PackInfo pk, FiberOrder ord, uint64_t n,
+                                                   uint64_t *__restrict__ k0,
+                                                   uint64_t *__restrict__ k1) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint64_t a = pkeys[i * pkw], b = pkw == 2 ? pkeys[i * pkw + 1] : 0;
-    out[i] = extract_field(a, b, pk.shift[mode], pk.width[mode]);
+    unsigned __int128 c = 0;
+    for (int q = 0; q < ord.n; ++q) {
+        int m = ord.mode[q], w = pk.width[m];
+        if (w == 0) continue;
+        c = (c << w) | extract_field(a, b, pk.shift[m], w);
+    }
+    k0[i] = (uint64_t)c;
+    if (k1) k1[i] = (uint64_t)(c >> 64);
 }
 
 __global__ __launch_bounds__(256) void k_gather_sched(const uint64_t *__restrict__ pkeys, int pkw,
                                                       const double *__restrict__ v64,
                                                       const uint32_t *__restrict__ perm, uint64_t n,
                                                       uint64_t *__restrict__ opk,
-                                                      float *__restrict__ o32,
                                                       double *__restrict__ o64) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint32_t j = perm[i];
     opk[i * pkw] = pkeys[(uint64_t)j * pkw];
     if (pkw == 2) opk[i * pkw + 1] = pkeys[(uint64_t)j * pkw + 1];
-    double v = v64[j];
-    o32[i] = (float)v;
-    if (o64) o64[i] = v;
+    o64[i] = v64[j];
 }
 
-// Output-ordered copy of the tensor for one mode: stable sort of the ALTO
-// order by that mode's index (the order the reference's op_sched variant
-// walks through oidx, alto.hpp:1708-1801).
+__device__ __forceinline__ bool same_key(const uint64_t *k, int pkw, uint64_t i, uint64_t j) {
+    return k[i * pkw] == k[j * pkw] && (pkw == 1 || k[i * pkw + 1] == k[j * pkw + 1]);
+}
+
+// head[i] = 1 where a run of equal coordinates starts.
+__global__ __launch_bounds__(256) void k_run_heads(const uint64_t *__restrict__ k, int pkw,
+                                                   uint64_t n, uint32_t *__restrict__ head) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    head[i] = (i == 0 || !same_key(k, pkw, i, i - 1)) ? 1u : 0u;
+}
+
+// One thread per run head: the run's values summed in order in FP64 (the
+// run keeps the ALTO order of the duplicates).  pos = exclusive scan of the
+// head flags.
+__global__ __launch_bounds__(256) void k_run_merge(const uint64_t *__restrict__ k, int pkw,
+                                                   const double *__restrict__ v, uint64_t n,
+                                                   const uint32_t *__restrict__ pos,
+                                                   uint64_t *__restrict__ ok, double *__restrict__ ov64,
+                                                   float *__restrict__ ov32) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || (i > 0 && same_key(k, pkw, i, i - 1))) return;
+    double sum = v[i];
+    for (uint64_t j = i + 1; j < n && same_key(k, pkw, j, i); ++j) sum += v[j];
+    const uint64_t o = pos[i];
+    ok[o * pkw] = k[i * pkw];
+    if (pkw == 2) ok[o * pkw + 1] = k[i * pkw + 1];
+    ov32[o] = (float)sum;
+    if (ov64) ov64[o] = sum;
+}
+
+// Output-ordered copy of the tensor for one mode (the row order the
+// reference's op_sched variant walks through oidx, alto.hpp:1708-1801).
+// Inside a row the nonzeros are in fiber order: sorted by the other modes'
+// indices, smallest mode first (ties by mode number), so nonzeros that share
+// a prefix of those indices are adjacent and the MTTKRP gathers the prefix
+// rows once per fiber.  Nonzeros with equal coordinates (duplicates kept by
+// the reference's ALTO tensor) are merged into one entry whose value is
+// their FP64 sum in ALTO order: the MTTKRP is linear in the values, so this
+// only reorders the additions.
+void fiber_levels(const TensorImpl *t, int mode, int *lvl) {
+    int k = 0;
+    for (int m = 0; m < t->nmode; ++m)
+        if (m != mode) lvl[k++] = m;
+    std::stable_sort(lvl, lvl + k, [&](int a, int b) { return t->dims[a] < t->dims[b]; });
+}
+
 int ensure_schedule(TensorImpl *t, int mode, bool need64, hipStream_t s) {
     ModeSchedule &sc = t->sched[mode];
     if (sc.built && (!need64 || sc.has64)) return ALTO_OK;
-    uint64_t n = t->nnz;
-    ALTO_TRY(sc.pkeys.alloc(sizeof(uint64_t) * t->pkw * n));
-    ALTO_TRY(sc.vals32.alloc(sizeof(float) * n));
-    if (need64) ALTO_TRY(sc.vals64.alloc(sizeof(double) * n));
+    const uint64_t n = t->nnz;
+    const int pkw = t->pkw;
+    fiber_levels(t, mode, sc.lvl);
+    sc.n = 0;
     if (n == 0) {
+        ALTO_TRY(sc.pkeys.alloc(sizeof(uint64_t) * pkw));
+        ALTO_TRY(sc.vals32.alloc(sizeof(float)));
+        if (need64) ALTO_TRY(sc.vals64.alloc(sizeof(double)));
         sc.built = true;
         sc.has64 = need64;
         return ALTO_OK;
     }
-    DevBuf idx, perm;
-    ALTO_TRY(idx.alloc(sizeof(uint64_t) * n));
+    FiberOrder ord{};
+    ord.n = t->nmode;
+    ord.mode[0] = mode;
+    for (int q = 0; q < t->nmode - 1; ++q) ord.mode[q + 1] = sc.lvl[q];
+    const bool wide = t->total_bits > 64;
+    DevBuf k0, k1, perm, sk, sv;
+    ALTO_TRY(k0.alloc(sizeof(uint64_t) * n));
+    if (wide) ALTO_TRY(k1.alloc(sizeof(uint64_t) * n));
     ALTO_TRY(perm.alloc(sizeof(uint32_t) * n));
-    k_mode_index<<<cdiv(n, 256), 256, 0, s>>>(t->pkeys.as<uint64_t>(), t->pkw, t->pack, mode, n,
-                                              idx.as<uint64_t>());
+    k_fiber_key<<<cdiv(n, 256), 256, 0, s>>>(t->pkeys.as<uint64_t>(), pkw, t->pack, ord, n,
+                                             k0.as<uint64_t>(), wide ? k1.as<uint64_t>() : nullptr);
     ALTO_LAUNCH_CHECK();
     ALTO_TRY(iota_u32(perm.as<uint32_t>(), n, s));
-    ALTO_TRY(radix_sort_pairs(idx.as<uint64_t>(), nullptr, perm.as<uint32_t>(), n,
-                              t->pack.width[mode], t->sort_ws, s));
-    k_gather_sched<<<cdiv(n, 256), 256, 0, s>>>(t->pkeys.as<uint64_t>(), t->pkw,
+    ALTO_TRY(radix_sort_pairs(k0.as<uint64_t>(), wide ? k1.as<uint64_t>() : nullptr,
+                              perm.as<uint32_t>(), n, t->total_bits, t->sort_ws, s));
+    k1.release();
+    ALTO_TRY(sk.alloc(sizeof(uint64_t) * pkw * n));
+    ALTO_TRY(sv.alloc(sizeof(double) * n));
+    k_gather_sched<<<cdiv(n, 256), 256, 0, s>>>(t->pkeys.as<uint64_t>(), pkw,
                                                 t->vals64.as<double>(), perm.as<uint32_t>(), n,
-                                                sc.pkeys.as<uint64_t>(), sc.vals32.as<float>(),
-                                                need64 ? sc.vals64.as<double>() : nullptr);
+                                                sk.as<uint64_t>(), sv.as<double>());
+    ALTO_LAUNCH_CHECK();
+    // merge duplicate coordinates (perm reused as the head flags / positions)
+    uint32_t *pos = perm.as<uint32_t>();
+    k_run_heads<<<cdiv(n, 256), 256, 0, s>>>(sk.as<uint64_t>(), pkw, n, pos);
+    ALTO_LAUNCH_CHECK();
+    uint32_t last_head = 0, last_pos = 0;
+    ALTO_HIP_TRY(hipMemcpyAsync(&last_head, pos + n - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    ALTO_TRY(exclusive_scan_u32(pos, n, t->sort_ws.scan_tmp, s));
+    ALTO_HIP_TRY(hipMemcpyAsync(&last_pos, pos + n - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    ALTO_HIP_TRY(hipStreamSynchronize(s));
+    const uint64_t m = (uint64_t)last_pos + last_head;
+    ALTO_TRY(sc.pkeys.alloc(sizeof(uint64_t) * pkw * m));
+    ALTO_TRY(sc.vals32.alloc(sizeof(float) * m));
+    if (need64) ALTO_TRY(sc.vals64.alloc(sizeof(double) * m));
+    k_run_merge<<<cdiv(n, 256), 256, 0, s>>>(sk.as<uint64_t>(), pkw, sv.as<double>(), n, pos,
+                                             sc.pkeys.as<uint64_t>(),
+                                             need64 ? sc.vals64.as<double>() : nullptr,
+                                             sc.vals32.as<float>());
     ALTO_LAUNCH_CHECK();
     ALTO_HIP_TRY(hipStreamSynchronize(s));
+    sc.n = m;
     sc.built = true;
     sc.has64 = need64;
     return ALTO_OK;

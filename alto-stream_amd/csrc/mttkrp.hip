@@ -81,7 +81,8 @@ template <typename T> __device__ __forceinline__ T stream_load(const T *p, bool 
 // starts and ends inside one lane group's range owns its row (plain
 // read-add-store), and only the runs at range boundaries are combined -- in
 // LDS across the workgroup, then with atomics for the workgroup's first and
-// last rows.
+// last rows; bit 3 = no decode-once kernel; bit 4 = no fiber kernel (output
+// order only).
 template <typename VT, typename FT, typename OT, int R, int PKW, int B>
 __global__ __launch_bounds__(256) void k_mttkrp_seg(const uint64_t *__restrict__ pkeys,
                                                     const VT *__restrict__ vals, uint64_t nnz,
@@ -471,6 +472,218 @@ __global__ __launch_bounds__(256) void k_mttkrp_dec(const uint64_t *__restrict__
     }
 }
 
+// Fiber variant of k_mttkrp_dec for output-ordered schedules whose rows are
+// sorted by the other modes' indices (levels 1..NM1, leaf last).  Nonzeros
+// that share the index prefix (target, l1, ..., lq) are adjacent, so the
+// product factors like a CSF tree walk:
+//   s      += val                       (equal coordinates: leaf gathered once)
+//   t[leaf] += s * A_leaf[l_leaf]        at the end of a coordinate
+//   t[q]    += t[q+1] * A_q[l_q]         at the end of a level-q fiber
+//   out[n]  += t[1]                      at the end of the row
+// Each nonzero carries the depth d of the first level whose index differs in
+// the next nonzero (0: the row ends, NM1 + 1: same coordinates), computed
+// from the two packed keys; a level's row is gathered only when its fiber
+// ends, so gathers per nonzero drop from NM1 to (#fibers of all levels) /
+// nnz.  Range boundaries end every fiber.  Row flushing and the boundary
+// combine are those of k_mttkrp_dec in exclusive mode.
+struct FibInfo {
+    int mode[4];                  // mode of level q + 1, q = 0 .. NM1 - 1
+    uint64_t mlo[5], mhi[5];      // packed-key bits of level 0 (target) .. NM1
+};
+
+template <int PKW, int NM1>
+__device__ __forceinline__ uint32_t fib_depth(uint64_t alo, uint64_t ahi, uint64_t blo,
+                                              uint64_t bhi, const FibInfo &fi) {
+    const uint64_t xl = alo ^ blo, xh = ahi ^ bhi;
+    uint32_t d = NM1 + 1;
+#pragma unroll
+    for (int q = NM1; q >= 0; --q) {
+        uint64_t x = xl & fi.mlo[q];
+        if (PKW == 2) x |= xh & fi.mhi[q];
+        if (x) d = q;
+    }
+    return d;
+}
+
+template <typename VT, typename FT, typename OT, int R, int PKW, int NM1, int VPL, int CB>
+__global__ __launch_bounds__(256) void k_mttkrp_fib(const uint64_t *__restrict__ pkeys,
+                                                    const VT *__restrict__ vals, uint64_t nnz,
+                                                    FPtrs<FT> F, OT *__restrict__ out,
+                                                    PackInfo pk, int target, FibInfo fi,
+                                                    uint64_t per_group, int flags) {
+    using CT = typename FVec<FT>::CT;
+    constexpr int L = R / VPL;
+    constexpr int G = 256 / L;
+    static_assert(R % VPL == 0 && L >= 1 && L <= 64 && (64 % L) == 0, "bad rank tiling");
+    static_assert(L % CB == 0, "batch must divide the group");
+    using Rec = DecRec<VT>;
+    __shared__ Rec s_rec[256];
+    __shared__ CT s_part[2][G][R];
+    __shared__ uint32_t s_row[2][G];
+    constexpr uint32_t NONE = 0xFFFFFFFFu;
+
+    const unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
+    const uint64_t gtid = (uint64_t)lb * blockDim.x + threadIdx.x;
+    const uint64_t group = gtid / L;
+    const int gib = threadIdx.x / L;
+    const int lig = threadIdx.x % L;
+    const bool nt = flags & 1;
+    uint64_t start = group * per_group;
+    uint64_t end = start + per_group < nnz ? start + per_group : nnz;
+    if (start > nnz) start = nnz;
+
+    int osh[NM1], ow[NM1];
+    const char *fb[NM1];
+#pragma unroll
+    for (int q = 0; q < NM1; ++q) {
+        const int m = fi.mode[q];
+        osh[q] = pk.shift[m];
+        ow[q] = pk.width[m];
+        fb[q] = (const char *)F.p[m];
+    }
+    const int tsh = pk.shift[target], tw = pk.width[target];
+    const uint32_t lane_b = (uint32_t)(lig * VPL * sizeof(FT));
+    OT *outp = out + lig * VPL;
+    Rec *myrec = s_rec + gib * L;
+
+    // The group's first and last row partials go straight to LDS (slot 0:
+    // first flushed row, slot 1: the row still open at the range end).
+    bool in_first = true;
+    if (lig == 0) {
+        s_row[0][gib] = NONE;
+        s_row[1][gib] = NONE;
+    }
+    CT t[NM1][VPL];
+    CT sv = 0;
+#pragma unroll
+    for (int v = 0; v < VPL; ++v)
+#pragma unroll
+        for (int q = 0; q < NM1; ++q) t[q][v] = 0;
+    uint64_t klo = 0, khi = 0, nlo = 0, nhi = 0;
+    VT kval = 0;
+    if (start < end) {
+        const uint64_t i = start + lig < end ? start + lig : end - 1;
+        const uint64_t j = i + 1 < nnz ? i + 1 : i;
+        klo = stream_load(pkeys + i * PKW, nt);
+        nlo = stream_load(pkeys + j * PKW, nt);
+        if (PKW == 2) {
+            khi = stream_load(pkeys + i * PKW + 1, nt);
+            nhi = stream_load(pkeys + j * PKW + 1, nt);
+        }
+        kval = stream_load(vals + i, nt);
+    }
+    for (uint64_t base = start; base < end; base += L) {
+        {
+            Rec r;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                r.off[q] = q < NM1 ? (uint32_t)field<PKW>(klo, khi, osh[q < NM1 ? q : 0], ow[q < NM1 ? q : 0]) *
+                                         (uint32_t)(R * sizeof(FT))
+                                   : 0u;
+            r.trow = (uint32_t)field<PKW>(klo, khi, tsh, tw);
+            r.pad = base + lig + 1 >= end ? 0u : fib_depth<PKW, NM1>(klo, khi, nlo, nhi, fi);
+            r.val = kval;
+            myrec[lig] = r;
+        }
+        const uint64_t nb = base + L;
+        if (nb < end) {  // prefetch the next step
+            const uint64_t i = nb + lig < end ? nb + lig : end - 1;
+            const uint64_t j = i + 1 < nnz ? i + 1 : i;
+            klo = stream_load(pkeys + i * PKW, nt);
+            nlo = stream_load(pkeys + j * PKW, nt);
+            if (PKW == 2) {
+                khi = stream_load(pkeys + i * PKW + 1, nt);
+                nhi = stream_load(pkeys + j * PKW + 1, nt);
+            }
+            kval = stream_load(vals + i, nt);
+        }
+        __builtin_amdgcn_wave_barrier();
+        const int nh = (end - base) < (uint64_t)L ? (int)(end - base) : L;
+#pragma unroll
+        for (int c0 = 0; c0 < L; c0 += CB) {
+            uint32_t ob[CB][NM1], tr[CB], dd[CB];
+            CT vv[CB];
+#pragma unroll
+            for (int k = 0; k < CB; ++k) {
+                const Rec &r = myrec[c0 + k];
+#pragma unroll
+                for (int q = 0; q < NM1; ++q) ob[k][q] = r.off[q] + lane_b;
+                tr[k] = r.trow;
+                dd[k] = c0 + k < nh ? r.pad : (uint32_t)(NM1 + 2);
+                vv[k] = (CT)r.val;
+            }
+            CT f[NM1][CB][VPL];
+#pragma unroll
+            for (int q = 0; q < NM1; ++q)
+#pragma unroll
+                for (int k = 0; k < CB; ++k)
+                    if (dd[k] <= (uint32_t)(q + 1)) RowLoad<FT, VPL>::load(fb[q] + ob[k][q], f[q][k]);
+#pragma unroll
+            for (int k = 0; k < CB; ++k) {
+                const uint32_t d = dd[k];
+                if (d > (uint32_t)(NM1 + 1)) continue;
+                sv += vv[k];
+                if (d <= (uint32_t)NM1) {
+#pragma unroll
+                    for (int v = 0; v < VPL; ++v) t[NM1 - 1][v] += sv * f[NM1 - 1][k][v];
+                    sv = 0;
+                }
+#pragma unroll
+                for (int q = NM1 - 2; q >= 0; --q) {
+                    if (d <= (uint32_t)(q + 1)) {
+#pragma unroll
+                        for (int v = 0; v < VPL; ++v) {
+                            t[q][v] += t[q + 1][v] * f[q][k][v];
+                            t[q + 1][v] = 0;
+                        }
+                    }
+                }
+                if (d == 0) {
+                    const uint32_t row = tr[k];
+                    if (in_first || base + c0 + k + 1 == end) {
+                        const int slot = in_first ? 0 : 1;
+#pragma unroll
+                        for (int v = 0; v < VPL; ++v) s_part[slot][gib][lig * VPL + v] = t[0][v];
+                        if (lig == 0) s_row[slot][gib] = row;
+                        in_first = false;
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < VPL; ++v) outp[(uint64_t)row * R + v] += (OT)t[0][v];
+                    }
+#pragma unroll
+                    for (int v = 0; v < VPL; ++v) t[0][v] = 0;
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    if (threadIdx.x < R) {
+        const int r = threadIdx.x;
+        OT *o = out + r;
+        uint32_t crow = NONE;
+        CT a = 0;
+        bool first_merged = true;
+        for (int e = 0; e < 2 * G; ++e) {
+            uint32_t row = s_row[e & 1][e >> 1];
+            if (row == NONE) continue;
+            CT x = s_part[e & 1][e >> 1][r];
+            if (row == crow) {
+                a += x;
+            } else {
+                if (crow != NONE) {
+                    if (first_merged) atomic_add_out(o + (uint64_t)crow * R, (OT)a);
+                    else o[(uint64_t)crow * R] += (OT)a;
+                    first_merged = false;
+                }
+                crow = row;
+                a = x;
+            }
+        }
+        if (crow != NONE) atomic_add_out(o + (uint64_t)crow * R, (OT)a);
+    }
+}
+
 // elements per lane of the decode kernel: 8 FP32 / BF16, 4 FP64 (<= R)
 template <typename FT, int R> constexpr int dec_vpl() {
     constexpr int want = sizeof(FT) == 8 ? 4 : 8;
@@ -538,7 +751,29 @@ struct MttkrpArgs {
     uint64_t per_group;
     int flags;
     bool decode_once;
+    bool fiber;
+    FibInfo fi;
 };
+
+template <typename VT, typename FT, typename OT, int R, int PKW, int VPL, int CB>
+static void launch_fib(const MttkrpArgs &a, const FPtrs<FT> &F, hipStream_t s) {
+    constexpr int L = R / VPL;
+    const uint64_t groups = (a.nnz + a.per_group - 1) / a.per_group;
+    const unsigned blocks = cdiv(groups * L, 256);
+    const int nm1 = a.pk.nmode - 1;
+    if (nm1 == 2)
+        k_mttkrp_fib<VT, FT, OT, R, PKW, 2, VPL, CB><<<blocks, 256, 0, s>>>(
+            a.pkeys, (const VT *)a.vals, a.nnz, F, (OT *)a.out, a.pk, a.target, a.fi, a.per_group,
+            a.flags);
+    else if (nm1 == 3)
+        k_mttkrp_fib<VT, FT, OT, R, PKW, 3, VPL, CB><<<blocks, 256, 0, s>>>(
+            a.pkeys, (const VT *)a.vals, a.nnz, F, (OT *)a.out, a.pk, a.target, a.fi, a.per_group,
+            a.flags);
+    else
+        k_mttkrp_fib<VT, FT, OT, R, PKW, 4, VPL, CB><<<blocks, 256, 0, s>>>(
+            a.pkeys, (const VT *)a.vals, a.nnz, F, (OT *)a.out, a.pk, a.target, a.fi, a.per_group,
+            a.flags);
+}
 
 template <typename VT, typename FT, typename OT, int R, int PKW>
 static void launch_seg(const MttkrpArgs &a, hipStream_t s) {
@@ -556,6 +791,7 @@ static void launch_seg(const MttkrpArgs &a, hipStream_t s) {
         uint64_t dthreads = groups * LD;
         unsigned dblocks = cdiv(dthreads, 256);
         const int nm1 = a.pk.nmode - 1;
+        if (a.fiber) return launch_fib<VT, FT, OT, R, PKW, VPL, 1>(a, F, s);
         if (nm1 == 2)
             k_mttkrp_dec<VT, FT, OT, R, PKW, 2, VPL><<<dblocks, 256, 0, s>>>(
                 a.pkeys, (const VT *)a.vals, a.nnz, F, (OT *)a.out, a.pk, a.target, a.per_group, a.flags);
@@ -646,8 +882,25 @@ int run_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const 
         }
     } else if (algo == 2) {
         ALTO_TRY(ensure_schedule(t, target, prec == ALTO_PREC_F64, s));
-        a.pkeys = t->sched[target].pkeys.as<uint64_t>();
-        a.vals = prec == ALTO_PREC_F64 ? t->sched[target].vals64.p : t->sched[target].vals32.p;
+        const ModeSchedule &sc = t->sched[target];
+        a.nnz = sc.n;
+        if (a.nnz == 0) return ALTO_OK;
+        a.pkeys = sc.pkeys.as<uint64_t>();
+        a.vals = prec == ALTO_PREC_F64 ? sc.vals64.p : sc.vals32.p;
+        // fiber kernel: FP64 and FP32 factors (BF16 rows are cheaper to gather
+        // per nonzero than the fiber bookkeeping)
+        a.fiber = a.decode_once && !(a.flags & 16) && prec != ALTO_PREC_BF16;
+        auto bits = [&](int m, uint64_t *lo, uint64_t *hi) {
+            const int sh = t->pack.shift[m], w = t->pack.width[m];
+            unsigned __int128 mk = w ? ((((unsigned __int128)1) << w) - 1) << sh : 0;
+            *lo = (uint64_t)mk;
+            *hi = (uint64_t)(mk >> 64);
+        };
+        bits(target, &a.fi.mlo[0], &a.fi.mhi[0]);
+        for (int q = 0; q < t->nmode - 1; ++q) {
+            a.fi.mode[q] = sc.lvl[q];
+            bits(sc.lvl[q], &a.fi.mlo[q + 1], &a.fi.mhi[q + 1]);
+        }
     } else {
         return ALTO_ERR_ARG;
     }

@@ -23,6 +23,8 @@ struct BitPos {
 struct ModeSchedule {
     bool built = false;
     bool has64 = false;
+    int lvl[kMaxModes] = {};  // fiber levels: the other modes, shallow to leaf
+    uint64_t n = 0;           // entries (distinct coordinates)
     DevBuf pkeys;   // packed keys in row order
     DevBuf vals32;  // FP32 values in row order
     DevBuf vals64;  // FP64 values in row order (built on demand)
@@ -70,6 +72,7 @@ int mode_bits(uint64_t dim);
 
 int ensure_vals32(TensorImpl *t, hipStream_t s);
 int ensure_schedule(TensorImpl *t, int mode, bool need64, hipStream_t s);
+void fiber_levels(const TensorImpl *t, int mode, int *lvl);
 
 }  // namespace alto
 
