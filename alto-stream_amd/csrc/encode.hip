@@ -239,23 +239,65 @@ __global__ __launch_bounds__(256) void k_run_heads(const uint64_t *__restrict__ 
     head[i] = (i == 0 || !same_key(k, pkw, i, i - 1)) ? 1u : 0u;
 }
 
-// One thread per run head: the run's values summed in order in FP64 (the
-// run keeps the ALTO order of the duplicates).  pos = exclusive scan of the
-// head flags.
-__global__ __launch_bounds__(256) void k_run_merge(const uint64_t *__restrict__ k, int pkw,
-                                                   const double *__restrict__ v, uint64_t n,
-                                                   const uint32_t *__restrict__ pos,
-                                                   uint64_t *__restrict__ ok, double *__restrict__ ov64,
-                                                   float *__restrict__ ov32) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || (i > 0 && same_key(k, pkw, i, i - 1))) return;
-    double sum = v[i];
-    for (uint64_t j = i + 1; j < n && same_key(k, pkw, j, i); ++j) sum += v[j];
-    const uint64_t o = pos[i];
+// One thread per run head: the values of a run of at most kShortRun equal
+// coordinates are summed in order in FP64 (the run keeps the ALTO order of
+// the duplicates); longer runs are appended to `longs` for k_run_merge_long.
+// pos = exclusive scan of the head flags.
+constexpr int kShortRun = 64;
+
+__device__ __forceinline__ void put_merged(const uint64_t *k, int pkw, uint64_t i, uint64_t o,
+                                           double sum, uint64_t *ok, double *ov64, float *ov32) {
     ok[o * pkw] = k[i * pkw];
     if (pkw == 2) ok[o * pkw + 1] = k[i * pkw + 1];
     ov32[o] = (float)sum;
     if (ov64) ov64[o] = sum;
+}
+
+__global__ __launch_bounds__(256) void k_run_merge(const uint64_t *__restrict__ k, int pkw,
+                                                   const double *__restrict__ v, uint64_t n,
+                                                   const uint32_t *__restrict__ pos,
+                                                   uint64_t *__restrict__ ok, double *__restrict__ ov64,
+                                                   float *__restrict__ ov32,
+                                                   uint64_t *__restrict__ longs,
+                                                   unsigned long long *__restrict__ nlong) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || (i > 0 && same_key(k, pkw, i, i - 1))) return;
+    double sum = v[i];
+    uint64_t j = i + 1;
+    for (; j < n && j < i + kShortRun && same_key(k, pkw, j, i); ++j) sum += v[j];
+    if (j < n && j == i + kShortRun && same_key(k, pkw, j, i)) {
+        longs[atomicAdd(nlong, 1ULL)] = i;
+        return;
+    }
+    put_merged(k, pkw, i, pos[i], sum, ok, ov64, ov32);
+}
+
+// One wave per long run: lane l sums elements head + l, head + l + 64, ...
+// of the run, then a fixed shuffle tree (deterministic).
+__global__ __launch_bounds__(256) void k_run_merge_long(const uint64_t *__restrict__ k, int pkw,
+                                                        const double *__restrict__ v, uint64_t n,
+                                                        const uint32_t *__restrict__ pos,
+                                                        uint64_t *__restrict__ ok,
+                                                        double *__restrict__ ov64,
+                                                        float *__restrict__ ov32,
+                                                        const uint64_t *__restrict__ longs,
+                                                        const unsigned long long *__restrict__ nlong) {
+    const uint64_t nl = *nlong;
+    const int lane = threadIdx.x & 63;
+    for (uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64; w < nl;
+         w += ((uint64_t)gridDim.x * blockDim.x) / 64) {
+        const uint64_t h = longs[w];
+        double a = 0.0;
+        bool more = true;
+        for (uint64_t b = h; more; b += 64) {
+            const uint64_t j = b + lane;
+            const bool in = j < n && same_key(k, pkw, j, h);
+            if (in) a += v[j];
+            more = __any(in);
+        }
+        for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
+        if (lane == 0) put_merged(k, pkw, h, pos[h], a, ok, ov64, ov32);
+    }
 }
 
 // Output-ordered copy of the tensor for one mode (the row order the
@@ -324,10 +366,19 @@ int ensure_schedule(TensorImpl *t, int mode, bool need64, hipStream_t s) {
     ALTO_TRY(sc.pkeys.alloc(sizeof(uint64_t) * pkw * m));
     ALTO_TRY(sc.vals32.alloc(sizeof(float) * m));
     if (need64) ALTO_TRY(sc.vals64.alloc(sizeof(double) * m));
+    DevBuf longs;
+    ALTO_TRY(longs.alloc(sizeof(uint64_t) * (n / kShortRun + 2)));
+    unsigned long long *nlong = (unsigned long long *)(longs.as<uint64_t>() + n / kShortRun + 1);
+    ALTO_HIP_TRY(hipMemsetAsync(nlong, 0, sizeof(unsigned long long), s));
     k_run_merge<<<cdiv(n, 256), 256, 0, s>>>(sk.as<uint64_t>(), pkw, sv.as<double>(), n, pos,
                                              sc.pkeys.as<uint64_t>(),
                                              need64 ? sc.vals64.as<double>() : nullptr,
-                                             sc.vals32.as<float>());
+                                             sc.vals32.as<float>(), longs.as<uint64_t>(), nlong);
+    ALTO_LAUNCH_CHECK();
+    k_run_merge_long<<<1024, 256, 0, s>>>(sk.as<uint64_t>(), pkw, sv.as<double>(), n, pos,
+                                          sc.pkeys.as<uint64_t>(),
+                                          need64 ? sc.vals64.as<double>() : nullptr,
+                                          sc.vals32.as<float>(), longs.as<uint64_t>(), nlong);
     ALTO_LAUNCH_CHECK();
     ALTO_HIP_TRY(hipStreamSynchronize(s));
     sc.n = m;

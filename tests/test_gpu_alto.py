@@ -35,7 +35,22 @@ def _cases():
     out.append(("one_nnz",) + rand_tensor([4, 5, 6], 1, seed=33))
     out.append(("u128",) + rand_tensor([1 << 20, 1 << 20, 1 << 18, 1 << 16, 1 << 12], 30000, seed=34, dup_frac=0.1))
     out.append(("big_tiles",) + rand_tensor([3000, 2000, 1000], 300000, seed=35, dup_frac=0.05))
+    out.append(("hot_dups",) + _hot_dups())
     return out
+
+
+def _hot_dups():
+    """Runs of equal coordinates around the 64-entry short-run limit of the
+    schedule merge (63, 64, 65) and one of 5000, mixed with random entries."""
+    dims, cidx, vals = rand_tensor([50, 40, 30], 3000, seed=38)
+    rng = np.random.default_rng(39)
+    extra = [((1, 2, 3), 5000), ((4, 5, 6), 65), ((7, 8, 9), 64), ((10, 11, 12), 63)]
+    add_c = [np.concatenate([np.full(k, c[m], np.uint64) for c, k in extra]) for m in range(3)]
+    add_v = rng.uniform(-1, 1, sum(k for _, k in extra))
+    perm = rng.permutation(len(vals) + len(add_v))
+    cidx = [np.concatenate([cidx[m], add_c[m]])[perm] for m in range(3)]
+    vals = np.concatenate([vals, add_v])[perm]
+    return dims, cidx, vals
 
 
 CASES = _cases()
@@ -98,7 +113,7 @@ def _run_gpu_mttkrp(gpu, at, F, mode, prec, rank, algo, out_f64=True):
 
 
 MT_CASES = [c for c in CASES if c[0] in ("fixture:med.tns", "fixture:med4.tns", "fixture:med5.tns", "dups3",
-                                         "size1", "u128")]
+                                         "size1", "u128", "hot_dups")]
 
 
 @pytest.mark.parametrize("case", MT_CASES, ids=[c[0] for c in MT_CASES])
