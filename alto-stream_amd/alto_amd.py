@@ -307,6 +307,15 @@ def cpstream(dims, cidx, vals, stream_mode, rank, max_iters, eps, nonneg, init_f
                 admm_iters=admm[:n], grams=grams, slices=n, timings=tim)
 
 
+def gather_probe(table_bytes, gathers=1 << 26, seed=1, stream=None):
+    """ms per launch of the random 128-byte row gather probe over a table of
+    table_bytes (power of two)."""
+    ms = ctypes.c_float()
+    check(lib().alto_hip_gather_probe(ctypes.c_uint64(table_bytes), ctypes.c_uint64(gathers),
+                                      ctypes.c_uint64(seed), ctypes.byref(ms), _vp(stream)), "gather_probe")
+    return ms.value
+
+
 def generate(dims, nnz, seed, mode_dist, zipf_s=None, value_dist=0, p1=0.0, p2=1.0, block_frac=0.0,
              block=0, stream=None):
     """Synthetic tensor in device memory -> (cidx device arrays, vals device array)."""

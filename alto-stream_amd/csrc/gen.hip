@@ -91,6 +91,28 @@ __global__ __launch_bounds__(256) void k_generate(GenArgs a, uint64_t **cidx, do
     vals[i] = v;
 }
 
+// Random-row gather rate: each group of 8 lanes reads `per` rows of 128
+// bytes (16 bytes per lane) at uniformly random row indices of a table of
+// `rows` rows (a power of two), as the MTTKRP gathers factor rows.
+__global__ __launch_bounds__(256) void k_gather_probe(const float4 *__restrict__ tab, uint64_t rows,
+                                                      uint64_t groups, int per, uint64_t seed,
+                                                      float *__restrict__ out) {
+    const uint64_t g = ((uint64_t)blockIdx.x * 256 + threadIdx.x) / 8;
+    const int l = threadIdx.x & 7;
+    if (g >= groups) return;
+    float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll 4
+    for (int k = 0; k < per; ++k) {
+        const uint64_t r = mix64(seed + g * per + k) & (rows - 1);
+        const float4 v = tab[r * 8 + l];
+        a.x += v.x;
+        a.y += v.y;
+        a.z += v.z;
+        a.w += v.w;
+    }
+    out[g * 8 + l] = a.x + a.y + a.z + a.w;
+}
+
 }  // namespace alto
 
 using namespace alto;
@@ -170,5 +192,39 @@ extern "C" int alto_hip_generate(int nmode, const uint64_t *dims, uint64_t nnz, 
         ALTO_LAUNCH_CHECK();
     }
     ALTO_HIP_TRY(hipStreamSynchronize(s));
+    return ALTO_OK;
+}
+
+extern "C" int alto_hip_gather_probe(uint64_t table_bytes, uint64_t gathers, uint64_t seed,
+                                     float *ms, void *stream) {
+    using namespace alto;
+    ALTO_TRY(ensure_init());
+    if (!ms || table_bytes < 128 || gathers < 64 || (table_bytes & (table_bytes - 1)))
+        return ALTO_ERR_ARG;
+    hipStream_t s = to_stream(stream);
+    const uint64_t rows = table_bytes / 128;
+    const int per = 64;
+    const uint64_t groups = gathers / per;
+    DevBuf tab, out;
+    ALTO_TRY(tab.alloc(table_bytes));
+    ALTO_TRY(out.alloc(sizeof(float) * 8 * groups));
+    ALTO_HIP_TRY(hipMemsetAsync(tab.p, 0, table_bytes, s));
+    const unsigned blocks = cdiv(groups * 8, 256);
+    hipEvent_t e0, e1;
+    ALTO_HIP_TRY(hipEventCreate(&e0));
+    ALTO_HIP_TRY(hipEventCreate(&e1));
+    k_gather_probe<<<blocks, 256, 0, s>>>(tab.as<float4>(), rows, groups, per, seed, out.as<float>());
+    ALTO_HIP_TRY(hipEventRecord(e0, s));
+    const int reps = 5;
+    for (int r = 0; r < reps; ++r)
+        k_gather_probe<<<blocks, 256, 0, s>>>(tab.as<float4>(), rows, groups, per, seed + r + 1,
+                                              out.as<float>());
+    ALTO_HIP_TRY(hipEventRecord(e1, s));
+    ALTO_HIP_TRY(hipEventSynchronize(e1));
+    float t = 0.f;
+    ALTO_HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = t / reps;
     return ALTO_OK;
 }

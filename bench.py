@@ -66,6 +66,8 @@ def main():
     ap.add_argument("--no-als", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=25_000_000)
+    ap.add_argument("--probe", action="store_true",
+                    help="random 128-byte row gather rate vs table size (the MTTKRP's miss-service bound)")
     ap.add_argument("--counters", action="store_true",
                     help="untimed pass for rocprofv3 --pmc: each mode twice in FP32, BF16 and FP64")
     args = ap.parse_args()
@@ -76,6 +78,13 @@ def main():
     import alto_amd as A
     A.init(0)
     dev = A.device_name()
+    if args.probe:
+        for lg in range(21, 32, 2):
+            g = 1 << 26
+            ms = A.gather_probe(1 << lg, g)
+            print(json.dumps({"table_MiB": (1 << lg) >> 20, "ms": round(ms, 4), "G_rows_per_s": round(g / ms / 1e6, 2),
+                              "GB_per_s": round(g * 128 / ms / 1e6, 1)}), flush=True)
+        return
     W = workload(args.workload)
     dims, nnz, R = W["dims"], W["nnz"], W["rank"]
     N = len(dims)

@@ -185,6 +185,13 @@ int alto_hip_generate(int nmode, const uint64_t *dims, uint64_t nnz, uint64_t se
                       double p1, double p2, double block_frac, uint64_t block,
                       uint64_t *const *cidx_dev, double *vals_dev, void *stream);
 
+/* ---- gather-rate probe (no reference counterpart: roofline calibration) ----
+ * Average ms of one launch gathering `gathers` 128-byte rows at uniformly
+ * random positions of a device table of table_bytes (a power of two), the
+ * access pattern of the MTTKRP's factor-row gathers. */
+int alto_hip_gather_probe(uint64_t table_bytes, uint64_t gathers, uint64_t seed, float *ms,
+                          void *stream);
+
 #ifdef __cplusplus
 }
 #endif
