@@ -233,8 +233,20 @@ def _ptrs(devs):
     return arr
 
 
+def _check_out(out, rows, rank, f64):
+    """The kernels write rows x rank elements of FP64 (f64) or FP32: refuse an
+    output array of another element type or a smaller shape."""
+    want = np.dtype(np.float64 if f64 else np.float32)
+    if out.dtype != want:
+        raise AltoError("output array is %s, the MTTKRP writes %s" % (out.dtype, want))
+    if out.nbytes < rows * rank * want.itemsize:
+        raise AltoError("output array holds %d bytes, the MTTKRP writes %d x %d" % (out.nbytes, rows, rank))
+
+
 def mttkrp_alto_par(target, factors, rank, at, out, prec=PREC_F64, out_f64=True, algo=0, stream=None):
-    """out += MTTKRP(target) on device arrays (mttkrp_alto_par, alto.hpp:902-908)."""
+    """out += MTTKRP(target) on device arrays (mttkrp_alto_par, alto.hpp:902-908).
+    out: FP64 when prec is PREC_F64 or out_f64, else FP32."""
+    _check_out(out, int(at.dims[target]), rank, out_f64 or prec == PREC_F64)
     check(lib().alto_hip_mttkrp_algo(_vp(at.handle), ctypes.c_int(target), ctypes.c_int(prec), ctypes.c_int(rank),
                                      _ptrs(factors), _vp(out.ptr), ctypes.c_int(1 if out_f64 else 0),
                                      ctypes.c_int(algo), _vp(stream)), "mttkrp_alto_par")

@@ -346,7 +346,6 @@ int ensure_schedule(TensorImpl *t, int mode, bool need64, hipStream_t s) {
     ALTO_TRY(iota_u32(perm.as<uint32_t>(), n, s));
     ALTO_TRY(radix_sort_pairs(k0.as<uint64_t>(), wide ? k1.as<uint64_t>() : nullptr,
                               perm.as<uint32_t>(), n, t->total_bits, t->sort_ws, s));
-    k1.release();
     ALTO_TRY(sk.alloc(sizeof(uint64_t) * pkw * n));
     ALTO_TRY(sv.alloc(sizeof(double) * n));
     k_gather_sched<<<cdiv(n, 256), 256, 0, s>>>(t->pkeys.as<uint64_t>(), pkw,

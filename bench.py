@@ -66,6 +66,9 @@ def main():
     ap.add_argument("--no-als", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=25_000_000)
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the other workloads (plain CP-ALS, 5-mode BF16, CP-stream sample)")
+    ap.add_argument("--c4-slices", type=int, default=24, help="CP-stream sample: number of time slices")
     ap.add_argument("--probe", action="store_true",
                     help="random 128-byte row gather rate vs table size (the MTTKRP's miss-service bound)")
     ap.add_argument("--counters", action="store_true",
@@ -189,6 +192,10 @@ def main():
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(A, at, dims, R, F64, args.cpu_sample)
     at.close()
+    if not args.no_extra:
+        result["other_workloads"] = {"c1_plain_cp_als": leg_c1(A, not args.no_cpu),
+                                     "c3_5mode_bf16": leg_c3(A),
+                                     "c4_cp_stream": leg_c4(A, args.c4_slices)}
     print(json.dumps(result), flush=True)
 
 
@@ -248,6 +255,146 @@ def cpu_baseline(A, at, dims, R, F64, sample):
                       "the first %d nonzeros in ALTO order: %.3f s; extrapolated %.1f ms per all-mode step "
                       "at 1e8 nnz" % (S, dt, dt * 1e3 * 1e8 / S)}
 
+
+
+def leg_c1(A, with_cpu):
+    """Plain CP-ALS as the reference runs it: 3-mode 1000x800x600, 1e6
+    distinct uniform coordinates, U(0,1) values, rank 16, 20 iterations;
+    plus 100 mode-0 MTTKRPs (the reference's -p benchmark)."""
+    dims, nnz, R = [1000, 800, 600], 1_000_000, 16
+    log("C1: plain CP-ALS, 1000x800x600, 1e6 nnz, rank 16")
+    rng = np.random.default_rng(0xA1700000)
+    total = dims[0] * dims[1] * dims[2]
+    lin = np.unique(rng.integers(0, total, size=int(nnz * 1.05)))
+    lin = rng.permutation(lin)[:nnz]
+    cidx = [(lin // (dims[1] * dims[2])).astype(np.uint64), ((lin // dims[2]) % dims[1]).astype(np.uint64),
+            (lin % dims[2]).astype(np.uint64)]
+    vals = rng.random(nnz)
+    F = [rng.random((d, R)) for d in dims]
+    at = A.create_alto(dims, cidx, vals, nprtn=256)
+    out = {"workload": "3-mode 1000x800x600, 1e6 distinct nnz, U(0,1), rank 16"}
+    for name, p in (("f64", A.PREC_F64), ("f32", A.PREC_F32)):
+        _, _, fits, _, tim = A.cpd_alto(at, F, R, 20, eps=1e-30, prec=p)
+        out["cp_als_%s_s_per_iter" % name] = round(float(tim[7]), 6)
+        out["cp_als_%s_fit" % name] = float(fits[-1])
+    dF = [A.to_prec(f, A.PREC_F64) for f in F]
+    o = A.DeviceArray.zeros((dims[0], R), np.float64)
+    A.mttkrp_alto_par(0, dF, R, at, o, prec=A.PREC_F64)
+    A.synchronize()
+    with A.Timer() as tm:
+        for _ in range(100):
+            A.mttkrp_alto_par(0, dF, R, at, o, prec=A.PREC_F64)
+    out["mttkrp_mode0_f64_ms"] = round(tm.ms / 100, 5)
+    for d in dF:
+        d.close()
+    o.close()
+    at.close()
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        threads = min(16, os.cpu_count() or 1)
+        O.set_threads(threads)
+        ref = O.AltoTensor(dims, cidx, vals, nprtn=threads)
+        t0 = time.time()
+        _, _, rfits, _ = ref.cpd_als(F, R, 5, eps=1e-30)
+        dt = (time.time() - t0) / 5
+        out["cpu_port_cp_als_s_per_iter"] = round(dt, 5)
+        out["cpu_cores"] = threads
+        out["cpu_fit_after_5"] = float(rfits[-1])
+    log("C1: CP-ALS f64 %.5f s/iter, f32 %.5f s/iter, MTTKRP mode 0 %.4f ms" %
+        (out["cp_als_f64_s_per_iter"], out["cp_als_f32_s_per_iter"], out["mttkrp_mode0_f64_ms"]))
+    return out
+
+
+def leg_c3(A):
+    """5-mode 2^20 x 2^20 x 2^18 x 2^16 x 2^12, 5e7 nnz (70% uniform, 30% in a
+    256^5 sub-block), N(0,1) values, rank 64, 128-bit keys: all-mode MTTKRP
+    with BF16-stored factors (FP32 accumulate) and FP64, and 10 CP-ALS
+    iterations BF16 vs FP64 with the fit difference."""
+    dims, nnz, R = [1 << 20, 1 << 20, 1 << 18, 1 << 16, 1 << 12], 50_000_000, 64
+    seed = 0xA1700003
+    log("C3: 5-mode, 5e7 nnz, rank 64, BF16 / FP64")
+    t0 = time.time()
+    cidx, vals = A.generate(dims, nnz, seed, [0] * 5, None, 2, 0.0, 1.0, block_frac=0.3, block=256)
+    at = A.create_alto(dims, cidx, vals, nprtn=256, on_device=True)
+    A.synchronize()
+    t_create = time.time() - t0
+    for c in cidx:
+        c.close()
+    vals.close()
+    rng = np.random.default_rng(seed)
+    F = [rng.random((d, R)) for d in dims]
+    out = {"workload": "5-mode 2^20x2^20x2^18x2^16x2^12, 5e7 nnz (30%% in a 256^5 block), N(0,1), rank 64, "
+                       "%d-bit keys" % (64 * at.kw), "alto_create_s": round(t_create, 3)}
+    for name, p in (("bf16", A.PREC_BF16), ("f64", A.PREC_F64)):
+        dF = [A.to_prec(f, p) for f in F]
+        odt = np.float64 if p == A.PREC_F64 else np.float32
+        outs = [A.DeviceArray.zeros((d, R), odt) for d in dims]
+        for n in range(5):
+            A.mttkrp_alto_par(n, dF, R, at, outs[n], prec=p, out_f64=False)
+        A.synchronize()
+        with A.Timer() as tm:
+            for _ in range(3):
+                for n in range(5):
+                    A.mttkrp_alto_par(n, dF, R, at, outs[n], prec=p, out_f64=False)
+        ms = tm.ms / 3
+        k, v = 8 * at.kw, 8 if p == A.PREC_F64 else 4
+        f, o = (8, 8) if p == A.PREC_F64 else (2, 4)
+        tot, _ = alg_bytes(dims, nnz, R, k, v, f, o)
+        out["mttkrp_%s_ms_all_mode" % name] = round(ms, 4)
+        out["mttkrp_%s_GBs_algorithmic" % name] = round(tot / (ms * 1e-3) / 1e9, 1)
+        for d in dF:
+            d.close()
+        for d in outs:
+            d.close()
+    fits = {}
+    for name, p in (("bf16", A.PREC_BF16), ("f64", A.PREC_F64)):
+        _, _, ft, fe, tim = A.cpd_alto(at, F, R, 10, eps=1e-30, prec=p, fit_mode=A.FIT_EXACT)
+        out["cp_als_%s_s_per_iter" % name] = round(float(tim[7]), 5)
+        fits[name] = float(ft[-1])
+        out["cp_als_%s_fit" % name] = fits[name]
+    out["cp_als_speedup_bf16_vs_f64"] = round(out["cp_als_f64_s_per_iter"] / out["cp_als_bf16_s_per_iter"], 3)
+    out["fit_abs_diff_bf16_vs_f64"] = abs(fits["bf16"] - fits["f64"])
+    at.close()
+    log("C3: MTTKRP bf16 %.3f ms, f64 %.3f ms all-mode; CP-ALS bf16 %.4f s/iter, f64 %.4f s/iter, |dfit| %.2e" %
+        (out["mttkrp_bf16_ms_all_mode"], out["mttkrp_f64_ms_all_mode"], out["cp_als_bf16_s_per_iter"],
+         out["cp_als_f64_s_per_iter"], out["fit_abs_diff_bf16_vs_f64"]))
+    return out
+
+
+def leg_c4(A, slices):
+    """CP-stream with non-negative ADMM on the first `slices` time slices of
+    the 1000-slice workload: slices of 200000 x 100000, per-slice nnz
+    lognormal(ln 2e5, 1) clipped to [2e4, 2e6], Zipf(1.05) indices,
+    Poisson(3)+1 values, rank 16, at most 10 outer iterations per slice;
+    FP32 MTTKRP, FP64 ADMM/solve."""
+    seed = 0xA1700004
+    log("C4: CP-stream, %d slices" % slices)
+    rng = np.random.default_rng(seed)
+    sizes = np.clip(np.round(rng.lognormal(np.log(2e5), 1.0, 1000)), 2e4, 2e6).astype(np.int64)[:slices]
+    nnz = int(sizes.sum())
+    dims = [slices, 200000, 100000]
+    cidx, vals = A.generate([1, 200000, 100000], nnz, seed, [0, 1, 1], [1.0, 1.05, 1.05], 3, 3.0, 0.0)
+    hc = [c.numpy() for c in cidx]
+    hv = vals.numpy()
+    for c in cidx:
+        c.close()
+    vals.close()
+    hc[0] = np.repeat(np.arange(slices, dtype=np.uint64), sizes)
+    R = 16
+    init = [rng.random((d, R)) for d in dims]
+    t0 = time.time()
+    res = A.cpstream(dims, hc, hv, 0, R, 10, 1e-6, True, init, prec=A.PREC_F32)
+    wall = time.time() - t0
+    tim = res["timings"]
+    out = {"workload": "CP-stream, first %d of 1000 slices of 200000x100000 (%d nnz), Zipf(1.05), Poisson(3)+1, "
+                       "rank 16, nonneg ADMM, <= 10 outer iterations" % (slices, nnz),
+           "slices_per_s": round(slices / float(tim[0]), 2), "slice_loop_s": round(float(tim[0]), 3),
+           "wall_s_incl_host_sort": round(wall, 3),
+           "outer_iters_mean": round(float(np.mean(res["outer_iters"])), 2),
+           "admm_iters_mean": round(float(np.mean(res["admm_iters"])), 2)}
+    log("C4: %d slices in %.3f s (%.2f slices/s), wall %.1f s" % (slices, tim[0], slices / tim[0], wall))
+    return out
 
 if __name__ == "__main__":
     main()

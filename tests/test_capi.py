@@ -5,6 +5,7 @@ import glob
 import os
 import re
 
+import numpy as np
 import pytest
 
 import alto_amd
@@ -46,3 +47,21 @@ def test_no_device_is_reported_not_faked():
     assert lib.alto_hip_init(0) == -5  # ALTO_ERR_NO_DEVICE
     with pytest.raises(alto_amd.AltoError):
         alto_amd.init(0)
+
+
+def test_mttkrp_output_type_is_checked_before_launch():
+    """The binding refuses an output array whose element type or size does
+    not match what the kernel writes (FP64 rows for out_f64 / PREC_F64)."""
+    import alto_amd as A
+
+    class Fake:
+        def __init__(self, dtype, n):
+            self.dtype = np.dtype(dtype)
+            self.nbytes = n * self.dtype.itemsize
+
+    A._check_out(Fake(np.float64, 40 * 16), 40, 16, True)
+    A._check_out(Fake(np.float32, 40 * 16), 40, 16, False)
+    with pytest.raises(A.AltoError, match="writes float64"):
+        A._check_out(Fake(np.float32, 40 * 16), 40, 16, True)
+    with pytest.raises(A.AltoError, match="holds"):
+        A._check_out(Fake(np.float64, 39 * 16), 40, 16, True)
