@@ -319,6 +319,35 @@ def cpstream(dims, cidx, vals, stream_mode, rank, max_iters, eps, nonneg, init_f
                 admm_iters=admm[:n], grams=grams, slices=n, timings=tim)
 
 
+def rowsparse_mttkrp(dims, cidx, vals, mode, stream_mode, rank, rsp_vals, ridx, stream=None):
+    """Row-sparse MTTKRP (rowsparse_mttkrp, src/rowsparse_mttkrp.cpp:97-187) on
+    host arrays.  rsp_vals[m]: stored rows (k x rank) of mode m; ridx[m]:
+    int32 map index -> stored row (-1 absent); the stream mode uses its row 0.
+    Returns (rowind, rows)."""
+    c, keep = _coo(dims, cidx, vals, False)
+    N = len(dims)
+    I = int(dims[mode])
+    rv = [np.ascontiguousarray(rsp_vals[m] if m != mode and rsp_vals[m] is not None else np.zeros((1, rank)),
+                               dtype=np.float64) for m in range(N)]
+    rx = [np.ascontiguousarray(ridx[m] if m not in (mode, stream_mode) else np.zeros(1, np.int32), dtype=np.int32)
+          for m in range(N)]
+    varr = (_f64p * N)()
+    xarr = (ctypes.POINTER(ctypes.c_int32) * N)()
+    for m in range(N):
+        varr[m] = rv[m].ctypes.data_as(_f64p)
+        xarr[m] = rx[m].ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    rows = np.ascontiguousarray([r.shape[0] for r in rv], dtype=np.uint64)
+    rowind = np.zeros(I, dtype=np.uint64)
+    out = np.zeros((I, rank))
+    nnzr = ctypes.c_uint64()
+    check(lib().alto_hip_rowsparse_mttkrp(ctypes.byref(c), ctypes.c_int(mode), ctypes.c_int(stream_mode),
+                                          ctypes.c_int(rank), varr, rows.ctypes.data_as(_u64p), xarr,
+                                          rowind.ctypes.data_as(_u64p), out.ctypes.data_as(_f64p),
+                                          ctypes.byref(nnzr), _vp(stream)), "rowsparse_mttkrp")
+    n = nnzr.value
+    return rowind[:n], out[:n]
+
+
 def gather_probe(table_bytes, gathers=1 << 26, seed=1, stream=None):
     """ms per launch of the random 128-byte row gather probe over a table of
     table_bytes (power of two)."""

@@ -207,6 +207,56 @@ void mttkrp(SparseTensor *X, KruskalModel *M, IType mode) {
     alto_hip_free(dv);
 }
 
+// ---- rowsparse_mttkrp.cpp:97-187 ----
+RowSparseMatrix *rspmat_init(IType nrows, IType ncols, IType nnzr) {
+    RowSparseMatrix *m = new RowSparseMatrix;
+    m->I = nrows;
+    m->J = ncols;
+    m->nnzr = nnzr;
+    m->rowind = (size_t *)calloc(nnzr ? nnzr : 1, sizeof(size_t));
+    m->vals = (FType *)calloc((nnzr ? nnzr : 1) * ncols, sizeof(FType));
+    return m;
+}
+
+void rspmat_free(RowSparseMatrix *mat) {
+    if (!mat) return;
+    free(mat->rowind);
+    free(mat->vals);
+    delete mat;
+}
+
+RowSparseMatrix *rowsparse_mttkrp(SparseTensor *X, RowSparseMatrix **rsp_mats, IType mode,
+                                  IType streaming_mode, std::vector<std::vector<int>> &ridx) {
+    const int N = X->nmodes;
+    const IType R = rsp_mats[0]->J;
+    std::vector<const double *> vals(N, nullptr);
+    std::vector<uint64_t> rows(N, 0);
+    std::vector<std::vector<int32_t>> maps(N);
+    std::vector<const int32_t *> mp(N, nullptr);
+    for (int m = 0; m < N; ++m) {
+        if ((IType)m == mode) continue;
+        vals[m] = rsp_mats[m]->vals;
+        rows[m] = rsp_mats[m]->nnzr;
+        if ((IType)m == streaming_mode) continue;
+        maps[m].assign(ridx[m].begin(), ridx[m].end());
+        maps[m].resize(X->dims[m], -1);
+        mp[m] = maps[m].data();
+    }
+    const IType I = X->dims[mode];
+    std::vector<uint64_t> rowind(I);
+    std::vector<double> out(I * R);
+    uint64_t nnzr = 0;
+    alto_hip_coo c{X->nmodes, (const uint64_t *)X->dims, X->nnz, (const uint64_t *const *)X->cidx,
+                   X->vals, 0};
+    check(alto_hip_rowsparse_mttkrp(&c, (int)mode, (int)streaming_mode, (int)R, vals.data(), rows.data(),
+                                    mp.data(), rowind.data(), out.data(), &nnzr, nullptr),
+          "rowsparse_mttkrp");
+    RowSparseMatrix *M = rspmat_init(I, R, nnzr);
+    for (uint64_t i = 0; i < nnzr; ++i) M->rowind[i] = (size_t)rowind[i];
+    memcpy(M->vals, out.data(), sizeof(double) * nnzr * R);
+    return M;
+}
+
 // ---- streaming_cpd.cpp:26-366 ----
 int cpstream(SparseTensor *X, int rank, int max_iters, int streaming_mode, FType epsilon, IType seed,
              bool nonneg, int max_slices, int prec) {

@@ -135,6 +135,24 @@ inline void mttkrp_par(SparseTensor *X, KruskalModel *M, IType mode, void * /*wr
     mttkrp(X, M, mode);
 }
 
+// RowSparseMatrix (include/rowsparse_matrix.hpp:21-27): nnzr stored rows of
+// an I x J matrix, row-major in `vals`, with their row indices in `rowind`.
+struct RowSparseMatrix {
+    IType I = 0, J = 0, nnzr = 0;
+    size_t *rowind = nullptr;
+    FType *vals = nullptr;
+};
+RowSparseMatrix *rspmat_init(IType nrows, IType ncols, IType nnzr);
+void rspmat_free(RowSparseMatrix *mat);
+
+// rowsparse_mttkrp, the streaming overload (include/rowsparse_mttkrp.hpp:36-41,
+// src/rowsparse_mttkrp.cpp:97-187): the factors are row-sparse (rsp_mats[m],
+// ridx[m][i] = stored row of index i or -1); the stream mode uses its stored
+// row 0.  Returns a new row-sparse matrix: the distinct `mode` indices of X
+// (ascending) and their MTTKRP rows.
+RowSparseMatrix *rowsparse_mttkrp(SparseTensor *X, RowSparseMatrix **rsp_mats, IType mode,
+                                  IType streaming_mode, std::vector<std::vector<int>> &ridx);
+
 // cpd_alto (cpd.hpp:47-191): CP-ALS on the GPU, printing the reference's lines.
 template <typename LIT>
 void cpd_alto(AltoTensor<LIT> *AT, KruskalModel *M, int max_iters, double epsilon,

@@ -173,6 +173,25 @@ int alto_hip_cpstream(const alto_hip_coo *coo, int stream_mode, int rank,
                       int *out_outer_iters, double *out_admm_iters,
                       double *out_grams, double *timings, void *stream);
 
+/* ---- row-sparse MTTKRP: rowsparse_mttkrp (include/rowsparse_mttkrp.hpp:18-41,
+ * src/rowsparse_mttkrp.cpp:97-187) ----
+ * Factor matrices are row-sparse, as the reference's RowSparseMatrix
+ * (include/rowsparse_matrix.hpp:21-27): mode m stores rsp_rows[m] rows
+ * (rsp_vals[m], row-major rsp_rows[m] x rank, FP64) and ridx[m][i] maps index
+ * i of mode m (i < dims[m]) to its stored row; -1 = no stored row (the
+ * nonzero then contributes zero; the reference would read out of bounds).
+ * stream_mode (or -1): that mode's factor is its single stored row, used for
+ * every nonzero (ridx[stream_mode] is ignored), as the reference's
+ * m_idx = 0.  Entries rsp_vals[mode] / ridx[mode] are ignored.
+ * Output, row-sparse: the distinct indices of `mode` that occur, ascending
+ * (out_rowind, capacity dims[mode]), their MTTKRP rows (out_vals, capacity
+ * dims[mode] x rank, overwritten) and *out_nnzr.  All arrays are host or
+ * device memory as coo->on_device says. */
+int alto_hip_rowsparse_mttkrp(const alto_hip_coo *coo, int mode, int stream_mode, int rank,
+                              const double *const *rsp_vals, const uint64_t *rsp_rows,
+                              const int32_t *const *ridx, uint64_t *out_rowind, double *out_vals,
+                              uint64_t *out_nnzr, void *stream);
+
 /* ---- synthetic workloads (no reference counterpart: benchmark input) ----
  * Fills device arrays cidx_dev[m] (nnz uint64 each) and vals_dev (nnz FP64).
  * mode_dist[m]: 0 = uniform over the mode, 1 = Zipf(zipf_s[m]) over a seeded

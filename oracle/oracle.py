@@ -152,6 +152,36 @@ def mttkrp_coo(cidx, vals, factors, mode, out=None):
     return out
 
 
+def rowsparse_mttkrp(cidx, vals, mode, stream_mode, rsp_vals, ridx, rank):
+    """Row-sparse MTTKRP as src/rowsparse_mttkrp.cpp:97-187 computes it after
+    idxsort_hist (8-95): nonzeros bucketed by their `mode` index (ascending),
+    per nonzero accum = val * prod_{m != mode} rsp_vals[m][m_idx] in mode
+    order (m_idx = 0 for stream_mode, else ridx[m][i_m]), summed per bucket.
+    ridx -1 contributes a zero row (the reference would read out of bounds).
+    Pure Python loops: small inputs only.  Returns (rowind, rows)."""
+    cidx = [np.asarray(c, dtype=np.int64) for c in cidx]
+    vals = np.asarray(vals, dtype=np.float64)
+    order = np.argsort(cidx[mode], kind="stable")
+    keys = cidx[mode][order]
+    starts = [0] + [i for i in range(1, len(keys)) if keys[i] != keys[i - 1]] if len(keys) else []
+    bounds = list(zip(starts, starts[1:] + [len(keys)]))
+    rowind = np.array([keys[a] for a, _ in bounds], dtype=np.uint64)
+    out = np.zeros((len(bounds), rank))
+    N = len(cidx)
+    for b, (a, e) in enumerate(bounds):
+        row_accum = np.zeros(rank)
+        for i in order[a:e]:
+            accum = np.full(rank, vals[i])
+            for m in range(N):
+                if m == mode:
+                    continue
+                r = 0 if m == stream_mode else int(ridx[m][cidx[m][i]])
+                accum = accum * (rsp_vals[m][r] if r >= 0 else 0.0)
+            row_accum = row_accum + accum
+        out[b] += row_accum
+    return rowind, out
+
+
 class AltoTensor:
     """Oracle-side ALTO tensor: masks, sorted keys/values, partitions."""
 

@@ -325,3 +325,54 @@ def test_cpstream_batching_and_invariants():
     assert np.all((res["outer_iters"] >= 1) & (res["outer_iters"] <= 5))
     G = res["grams"]
     np.testing.assert_allclose(G[0], G[0].T, rtol=1e-12)
+
+
+def _rowsparse_case(dims, nnz, R, seed, stream_mode=-1, absent=0.0):
+    rng = np.random.default_rng(seed)
+    cidx = [rng.integers(0, d, size=nnz).astype(np.uint64) for d in dims]
+    vals = rng.uniform(-1, 1, nnz)
+    rsp, ridx = [], []
+    for m, d in enumerate(dims):
+        perm = rng.permutation(d).astype(np.int32)          # stored-row order differs from index order
+        keep = rng.random(d) >= absent
+        r = np.where(keep, perm, -1).astype(np.int32)
+        ridx.append(r)
+        rsp.append(rng.uniform(-1, 1, (d if m != stream_mode else 1, R)))
+    return cidx, vals, rsp, ridx
+
+
+@pytest.mark.parametrize("stream_mode", [-1, 0])
+def test_rowsparse_mttkrp_oracle_is_dense_mttkrp_restricted(stream_mode):
+    """rowsparse_mttkrp == the dense COO MTTKRP (mttkrp.cpp:122-156, pinned
+    above) over the expanded factors, restricted to the occurring rows."""
+    dims, R = [6, 9, 7, 5], 4
+    cidx, vals, rsp, ridx = _rowsparse_case(dims, 300, R, 77, stream_mode, absent=0.2)
+    for mode in (1, 3):
+        rowind, rows = O.rowsparse_mttkrp(cidx, vals, mode, stream_mode, rsp, ridx, R)
+        dense = []
+        for m, d in enumerate(dims):
+            if m == stream_mode:
+                dense.append(np.repeat(rsp[m][:1], d, axis=0))
+            else:
+                D = np.zeros((d, R))
+                ok = ridx[m] >= 0
+                D[ok] = rsp[m][ridx[m][ok]]
+                dense.append(D)
+        full = O.mttkrp_coo(cidx, vals, dense, mode)
+        np.testing.assert_array_equal(rowind, np.unique(cidx[mode]))
+        np.testing.assert_allclose(rows, full[rowind.astype(np.int64)], rtol=1e-13, atol=1e-14)
+
+
+def test_rowsparse_mttkrp_oracle_brute_force_tiny():
+    """One output row by hand: sum over its nonzeros of val * product of the
+    stored rows the map selects."""
+    cidx = [np.array([0, 2, 0], np.uint64), np.array([1, 0, 1], np.uint64), np.array([0, 0, 1], np.uint64)]
+    vals = np.array([2.0, 3.0, 5.0])
+    rsp = [None, np.array([[1.0, 2.0], [3.0, 4.0]]), np.array([[0.5, 1.0], [2.0, -1.0]])]
+    ridx = [None, np.array([1, 0], np.int32), np.array([0, 1], np.int32)]
+    rowind, rows = O.rowsparse_mttkrp(cidx, vals, 0, -1, rsp, ridx, 2)
+    np.testing.assert_array_equal(rowind, [0, 2])
+    # row 0: nonzeros 0 and 2; mode-1 index 1 -> stored row 0 = [1, 2]
+    exp0 = 2.0 * np.array([1.0, 2.0]) * np.array([0.5, 1.0]) + 5.0 * np.array([1.0, 2.0]) * np.array([2.0, -1.0])
+    exp2 = 3.0 * np.array([3.0, 4.0]) * np.array([0.5, 1.0])
+    np.testing.assert_allclose(rows, [exp0, exp2], rtol=0, atol=0)
