@@ -284,6 +284,33 @@ def cpd_alto(at, factors, rank, max_iters, eps=1e-30, prec=PREC_F64, fit_mode=FI
     return fs, lam, fits[:n], fits_exact[:n], tim
 
 
+def cpd_alto_batch(tensors, factors, ranks, max_iters, eps=1e-30, prec=PREC_F64, fit_mode=FIT_REFERENCE_DIAG,
+                   nstreams=8):
+    """CP-ALS on many independent ALTO tensors concurrently
+    (alto_hip_cpd_als_batch).  factors[k]: host FP64 initial model of tensor k;
+    ranks[k]: its rank.  Returns (factors, lambdas, fits, wall_seconds)."""
+    n = len(tensors)
+    fs = [[np.ascontiguousarray(f, dtype=np.float64).copy() for f in fk] for fk in factors]
+    lams = [np.zeros(int(r)) for r in ranks]
+    fits = [np.zeros(max(int(max_iters), 1)) for _ in range(n)]
+    iters = np.zeros(n, dtype=np.int32)
+    handles = (_vp * n)(*[t.handle for t in tensors])
+    opts = (_AlsOpts * n)(*[_AlsOpts(int(r), int(max_iters), eps, prec, fit_mode, 0) for r in ranks])
+    farr = (ctypes.POINTER(_f64p) * n)()
+    keep = []
+    for k in range(n):
+        a = (_f64p * len(fs[k]))(*[f.ctypes.data_as(_f64p) for f in fs[k]])
+        keep.append(a)
+        farr[k] = ctypes.cast(a, ctypes.POINTER(_f64p))
+    larr = (_f64p * n)(*[l.ctypes.data_as(_f64p) for l in lams])
+    fitarr = (_f64p * n)(*[f.ctypes.data_as(_f64p) for f in fits])
+    tim = np.zeros(1)
+    check(lib().alto_hip_cpd_als_batch(ctypes.c_int(n), handles, opts, farr, larr, fitarr,
+                                       iters.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), ctypes.c_int(nstreams),
+                                       tim.ctypes.data_as(_f64p)), "cpd_als_batch")
+    return fs, lams, [f[: iters[k]] for k, f in enumerate(fits)], float(tim[0])
+
+
 def cpstream(dims, cidx, vals, stream_mode, rank, max_iters, eps, nonneg, init_factors, max_slices=0,
              prec=PREC_F64, stream=None):
     """CP-stream (cpstream, src/streaming_cpd.cpp:26-366), host COO input."""

@@ -12,6 +12,12 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <mutex>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "dense.hpp"
@@ -90,22 +96,33 @@ static float elapsed(hipEvent_t a, hipEvent_t b) {
 
 using namespace alto;
 
-extern "C" int alto_hip_cpd_als(alto_hip_tensor th, const alto_hip_als_opts *o,
-                                double *const *factors, double *lambda, double *fits,
-                                double *fits_exact, int *iters, double *timings, void *stream) {
+namespace alto {
+
+// Device buffers of one CP-ALS run; reused across runs (buffers only grow).
+struct AlsWork {
+    DevBuf Uf[kMaxModes], mt, grams, vinv, flag, part, gpart, lam, normX, fitd, sumsq;
+};
+
+static int check_als_args(alto_hip_tensor th, const alto_hip_als_opts *o, double *const *factors) {
     if (!th || !o || !factors) return ALTO_ERR_ARG;
-    TensorImpl *t = &th->impl;
-    const int R = o->rank, N = t->nmode, prec = o->prec;
+    const int R = o->rank, prec = o->prec;
     if (R < 1 || R > kMaxRank || o->max_iters < 0) return ALTO_ERR_ARG;
     if (prec != ALTO_PREC_F64 && prec != ALTO_PREC_F32 && prec != ALTO_PREC_BF16) return ALTO_ERR_ARG;
-    for (int m = 0; m < N; ++m)
+    for (int m = 0; m < th->impl.nmode; ++m)
         if (!factors[m]) return ALTO_ERR_ARG;
-    ALTO_TRY(ensure_init());
-    hipStream_t s = to_stream(stream);
+    return ALTO_OK;
+}
 
+// cpd_alto on stream s with workspace w (arguments already checked).
+static int als_run(TensorImpl *t, const alto_hip_als_opts *o, double *const *factors,
+                   double *lambda, double *fits, double *fits_exact, int *iters, double *timings,
+                   AlsWork &w, hipStream_t s) {
+    const int R = o->rank, N = t->nmode, prec = o->prec;
     uint64_t maxI = 1;
     for (int m = 0; m < N; ++m) maxI = t->dims[m] > maxI ? t->dims[m] : maxI;
-    DevBuf Uf[kMaxModes], mt, grams, vinv, flag, part, gpart, lam, normX, fitd, tmp64;
+    DevBuf *Uf = w.Uf;
+    DevBuf &mt = w.mt, &grams = w.grams, &vinv = w.vinv, &flag = w.flag, &part = w.part,
+           &gpart = w.gpart, &lam = w.lam, &normX = w.normX, &fitd = w.fitd;
     const void *fptr[kMaxModes];
     for (int m = 0; m < N; ++m) {
         ALTO_TRY(Uf[m].alloc(prec_size(prec) * t->dims[m] * R));
@@ -131,7 +148,8 @@ extern "C" int alto_hip_cpd_als(alto_hip_tensor th, const alto_hip_als_opts *o,
                                    gpart.as<double>(), nb, s));
         ALTO_TRY(launch_gram_reduce(gpart.as<double>(), nb, R, grams.as<double>() + (size_t)m * R * R, s));
     }
-    ALTO_TRY(launch_sumsq(t->vals64.as<double>(), t->nnz, normX.as<double>(), s));
+    ALTO_TRY(w.sumsq.alloc(sizeof(double) * kSumsqBlocks));
+    ALTO_TRY(launch_sumsq(t->vals64.as<double>(), t->nnz, normX.as<double>(), w.sumsq.as<double>(), s));
     // Build the per-mode output-ordered schedules outside the timed loop
     // (tensor preparation, like create_alto).
     for (int m = 0; m < N; ++m) ALTO_TRY(ensure_schedule(t, m, prec == ALTO_PREC_F64, s));
@@ -236,5 +254,89 @@ extern "C" int alto_hip_cpd_als(alto_hip_tensor th, const alto_hip_als_opts *o,
         ALTO_HIP_TRY(hipMemcpyAsync(lambda, lam.p, sizeof(double) * R, hipMemcpyDeviceToHost, s));
         ALTO_HIP_TRY(hipStreamSynchronize(s));
     }
+    return ALTO_OK;
+}
+
+}  // namespace alto
+
+extern "C" int alto_hip_cpd_als(alto_hip_tensor th, const alto_hip_als_opts *o,
+                                double *const *factors, double *lambda, double *fits,
+                                double *fits_exact, int *iters, double *timings, void *stream) {
+    ALTO_TRY(check_als_args(th, o, factors));
+    ALTO_TRY(ensure_init());
+    AlsWork w;
+    return als_run(&th->impl, o, factors, lambda, fits, fits_exact, iters, timings, w,
+                   to_stream(stream));
+}
+
+extern "C" int alto_hip_cpd_als_batch(int ntensors, const alto_hip_tensor *tensors,
+                                      const alto_hip_als_opts *opts, double *const *const *factors,
+                                      double *const *lambdas, double *const *fits, int *iters,
+                                      int nstreams, double *timings) {
+    if (ntensors < 0 || (ntensors > 0 && (!tensors || !opts || !factors)) || nstreams < 1 ||
+        nstreams > 64)
+        return ALTO_ERR_ARG;
+    for (int k = 0; k < ntensors; ++k) {
+        ALTO_TRY(check_als_args(tensors[k], &opts[k], factors[k]));
+        if (opts[k].verbose) return ALTO_ERR_ARG;
+    }
+    ALTO_TRY(ensure_init());
+    int dev = 0;
+    ALTO_HIP_TRY(hipGetDevice(&dev));
+    const auto t0 = std::chrono::steady_clock::now();
+    // Schedules first, on one stream: they allocate and free device memory,
+    // which would serialise the concurrent runs.
+    hipStream_t s0 = default_stream();
+    for (int k = 0; k < ntensors; ++k)
+        for (int m = 0; m < tensors[k]->impl.nmode; ++m)
+            ALTO_TRY(ensure_schedule(&tensors[k]->impl, m, opts[k].prec == ALTO_PREC_F64, s0));
+    // Largest tensors first.
+    std::vector<int> order(ntensors);
+    for (int k = 0; k < ntensors; ++k) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return tensors[a]->impl.nnz * opts[a].max_iters > tensors[b]->impl.nnz * opts[b].max_iters;
+    });
+    std::atomic<int> next{0};
+    std::atomic<int> status{ALTO_OK};
+    std::mutex err_mu;
+    std::string err_msg;
+    auto worker = [&]() {
+        if (hipSetDevice(dev) != hipSuccess) {
+            status = ALTO_ERR_HIP;
+            return;
+        }
+        hipStream_t s = nullptr;
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+            status = ALTO_ERR_HIP;
+            return;
+        }
+        {
+            AlsWork w;
+            for (int i = next++; i < ntensors && status == ALTO_OK; i = next++) {
+                const int k = order[i];
+                int st = als_run(&tensors[k]->impl, &opts[k], factors[k], lambdas ? lambdas[k] : nullptr,
+                                 fits ? fits[k] : nullptr, nullptr, iters ? iters + k : nullptr, nullptr, w, s);
+                if (st != ALTO_OK) {
+                    std::lock_guard<std::mutex> g(err_mu);
+                    if (status == ALTO_OK) {
+                        status = st;
+                        err_msg = last_error();
+                    }
+                }
+            }
+            (void)hipStreamSynchronize(s);
+        }
+        (void)hipStreamDestroy(s);
+    };
+    const int nw = nstreams < ntensors ? nstreams : (ntensors > 0 ? ntensors : 1);
+    std::vector<std::thread> pool;
+    for (int i = 0; i < nw; ++i) pool.emplace_back(worker);
+    for (auto &th : pool) th.join();
+    if (status != ALTO_OK) {
+        set_error("%s", err_msg.c_str());
+        return status;
+    }
+    if (timings)
+        timings[0] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return ALTO_OK;
 }

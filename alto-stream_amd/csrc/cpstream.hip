@@ -17,6 +17,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -434,14 +435,13 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
         return ALTO_ERR_UNSUPPORTED;
     }
     {
-        static bool attr = false;
-        if (!attr) {
+        static std::once_flag attr;
+        std::call_once(attr, [] {
             (void)hipFuncSetAttribute((const void *)k_admm_first,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 256);
             (void)hipFuncSetAttribute((const void *)k_admm_iter,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 256);
-            attr = true;
-        }
+        });
     }
     hipEvent_t ev[4];
     for (auto &e : ev) ALTO_HIP_TRY(hipEventCreate(&e));

@@ -12,6 +12,8 @@
 #include <float.h>
 #include <math.h>
 
+#include <mutex>
+
 #include "dense.hpp"
 
 namespace alto {
@@ -159,12 +161,11 @@ int launch_chol_inv(const double *grams, int nmode, int mode, int R, double diag
     (void)copy_first;  // 1.0 * G == G exactly: both Hadamard orders agree bitwise
     if (R < 1 || R > kMaxRank) return ALTO_ERR_UNSUPPORTED;
     size_t lds = sizeof(double) * ((size_t)R * R + R);
-    static bool attr = false;
-    if (!attr) {
+    static std::once_flag attr;
+    std::call_once(attr, [] {
         (void)hipFuncSetAttribute((const void *)k_chol_inv,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 256);
-        attr = true;
-    }
+    });
     k_chol_inv<<<1, 256, lds, s>>>(grams, nmode, mode, R, diag_add, vinv, flag, phi_out, fail_mode,
                                    rho_out);
     ALTO_LAUNCH_CHECK();
@@ -240,12 +241,11 @@ int launch_solve(double *M, uint64_t rows, int R, const double *vinv, double *pa
     size_t tile = (size_t)TR * R;
     if (tile < 768) tile = 768;
     size_t lds = sizeof(double) * ((size_t)R * R + tile);
-    static bool attr = false;
-    if (!attr) {
+    static std::once_flag attr;
+    std::call_once(attr, [] {
         (void)hipFuncSetAttribute((const void *)k_solve, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024 - 256);
-        attr = true;
-    }
+    });
     k_solve<<<nblk, 256, lds, s>>>(M, rows, R, RP, vinv, partials);
     ALTO_LAUNCH_CHECK();
     return ALTO_OK;
@@ -486,13 +486,10 @@ __global__ void k_sum_final(const double *__restrict__ part, int n, double *__re
     }
 }
 
-int launch_sumsq(const double *v, uint64_t n, double *out, hipStream_t s) {
-    DevBuf part;
-    ALTO_TRY(part.alloc(sizeof(double) * 1024));
-    k_sumsq_part<<<1024, 256, 0, s>>>(v, n, part.as<double>());
-    k_sum_final<<<1, 64, 0, s>>>(part.as<double>(), 1024, out);
+int launch_sumsq(const double *v, uint64_t n, double *out, double *part, hipStream_t s) {
+    k_sumsq_part<<<kSumsqBlocks, 256, 0, s>>>(v, n, part);
+    k_sum_final<<<1, 64, 0, s>>>(part, kSumsqBlocks, out);
     ALTO_LAUNCH_CHECK();
-    ALTO_HIP_TRY(hipStreamSynchronize(s));
     return ALTO_OK;
 }
 

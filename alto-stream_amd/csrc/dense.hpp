@@ -60,6 +60,8 @@ int launch_scale_gram(const double *u, const double *lambda, uint64_t rows, int 
 int launch_gram(const void *U, int prec, uint64_t rows, int R, double *gpart, int nblk,
                 hipStream_t s);
 int launch_gram_reduce(const double *gpart, int nblk, int R, double *G, hipStream_t s);
-int launch_sumsq(const double *v, uint64_t n, double *out, hipStream_t s);
+constexpr int kSumsqBlocks = 1024;
+// out = sum of v^2 (fixed order); part: kSumsqBlocks doubles of scratch.
+int launch_sumsq(const double *v, uint64_t n, double *out, double *part, hipStream_t s);
 
 }  // namespace alto

@@ -69,6 +69,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the other workloads (plain CP-ALS, 5-mode BF16, CP-stream sample)")
     ap.add_argument("--c4-slices", type=int, default=24, help="CP-stream sample: number of time slices")
+    ap.add_argument("--c5-tensors", type=int, default=512, help="batched CP-ALS: number of tensors")
     ap.add_argument("--probe", action="store_true",
                     help="random 128-byte row gather rate vs table size (the MTTKRP's miss-service bound)")
     ap.add_argument("--counters", action="store_true",
@@ -195,7 +196,8 @@ def main():
     if not args.no_extra:
         result["other_workloads"] = {"c1_plain_cp_als": leg_c1(A, not args.no_cpu),
                                      "c3_5mode_bf16": leg_c3(A),
-                                     "c4_cp_stream": leg_c4(A, args.c4_slices)}
+                                     "c4_cp_stream": leg_c4(A, args.c4_slices),
+                                     "c5_batched_cp_als": leg_c5(A, args.c5_tensors)}
     print(json.dumps(result), flush=True)
 
 
@@ -394,6 +396,44 @@ def leg_c4(A, slices):
            "outer_iters_mean": round(float(np.mean(res["outer_iters"])), 2),
            "admm_iters_mean": round(float(np.mean(res["admm_iters"])), 2)}
     log("C4: %d slices in %.3f s (%.2f slices/s), wall %.1f s" % (slices, tim[0], slices / tim[0], wall))
+    return out
+
+
+def leg_c5(A, ntensors):
+    """Batched CP-ALS of uneven instances: 3-mode tensors, each mode size
+    uniform in [500, 50000], nnz log-uniform in [1e4, 5e6], uniform
+    coordinates, U(0,1) values; ranks 8/16/32 round robin; 25 iterations
+    each; FP32 MTTKRP, FP64 solve.  Timed as one batch on 8 streams and,
+    for comparison, on 1 stream (the tensors one after another)."""
+    seed = 0xA1700005
+    log("C5: batched CP-ALS, %d tensors" % ntensors)
+    rng = np.random.default_rng(seed)
+    ats, Fs, ranks, total = [], [], [], 0
+    for k in range(ntensors):
+        dims = [int(x) for x in rng.integers(500, 50001, size=3)]
+        nnz = int(round(10 ** rng.uniform(4, np.log10(5e6))))
+        R = (8, 16, 32)[k % 3]
+        cidx, vals = A.generate(dims, nnz, seed + k, [0, 0, 0], None, 0)
+        ats.append(A.create_alto(dims, cidx, vals, nprtn=1, on_device=True))
+        A.synchronize()
+        for c in cidx:
+            c.close()
+        vals.close()
+        Fs.append([rng.random((d, R)) for d in dims])
+        ranks.append(R)
+        total += nnz
+    out = {"workload": "%d 3-mode tensors, modes U[500, 50000], nnz log-uniform [1e4, 5e6] (%d total), "
+                       "ranks 8/16/32, 25 iterations, FP32 MTTKRP + FP64 solve" % (ntensors, total)}
+    for ns in (8, 1):
+        _, _, fits, wall = A.cpd_alto_batch(ats, Fs, ranks, 25, prec=A.PREC_F32, nstreams=ns)
+        out["wall_s_%d_streams" % ns] = round(wall, 3)
+        out["tensors_per_s_%d_streams" % ns] = round(ntensors / wall, 2)
+        out["nnz_iterations_per_s_%d_streams" % ns] = round(total * 25 / wall / 1e9, 3)
+    out["unit_nnz_iterations"] = "1e9 nonzero-iterations/s (nnz x 25 ALS iterations / wall)"
+    for a in ats:
+        a.close()
+    log("C5: %d tensors (%d nnz): %.3f s on 8 streams, %.3f s on 1 stream" %
+        (ntensors, total, out["wall_s_8_streams"], out["wall_s_1_streams"]))
     return out
 
 if __name__ == "__main__":

@@ -155,6 +155,21 @@ int alto_hip_cpd_als(alto_hip_tensor t, const alto_hip_als_opts *opts,
                      double *fits_exact, int *iters, double *timings,
                      void *stream);
 
+/* ---- batched CP-ALS: cpd_alto (cpd.hpp:47-191) on many independent tensors ----
+ * Runs alto_hip_cpd_als on ntensors distinct tensors concurrently: nstreams
+ * (1..64) worker threads, each with its own HIP stream and a device
+ * workspace reused from tensor to tensor, take the tensors in order of
+ * decreasing nnz * max_iters.  The output-ordered schedules of every tensor
+ * are built first.  Per tensor k: opts[k] (verbose must be 0), factors[k][m]
+ * (host FP64 dims x rank: initial model in, final model out), lambdas[k]
+ * (rank), fits[k] (max_iters) and iters[k]; each may be NULL.  Each tensor's
+ * results are those of alto_hip_cpd_als on it alone.  timings (may be NULL):
+ * timings[0] = wall seconds of the batch.  Returns the first failing status. */
+int alto_hip_cpd_als_batch(int ntensors, const alto_hip_tensor *tensors,
+                           const alto_hip_als_opts *opts, double *const *const *factors,
+                           double *const *lambdas, double *const *fits, int *iters,
+                           int nstreams, double *timings);
+
 /* ---- CP-stream: cpstream + cpstream_alto_iter (src/streaming_cpd.cpp:26-366,
  * 1198-1519) with admm_cf_ (src/admm.cpp:216-429), CPSTREAM_ALTO model,
  * timeslice limit 1.  init_factors: host, nmode arrays of dims[m] x rank

@@ -1,0 +1,66 @@
+"""Batched CP-ALS (alto_hip_cpd_als_batch): many independent tensors on
+several streams give each tensor the result of cpd_alto on it alone
+(cpd.hpp:47-191), which the single-run tests pin to the oracle.
+
+Bars: FP64 fits within 1e-10 and factors within 1e-9 relative of the
+single runs (boundary rows are summed with atomics, so the last bits may
+differ); FP32 fits within 1e-6; one tensor also against the oracle."""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import rand_factors, rand_tensor
+
+pytestmark = pytest.mark.gpu
+
+
+def _instances(n, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        nm = 3 if k % 4 else 4
+        dims = [int(x) for x in rng.integers(20, 400, size=nm)]
+        nnz = int(10 ** rng.uniform(3, 4.5))
+        R = (8, 16, 32)[k % 3]
+        d, c, v = rand_tensor(dims, nnz, seed=seed + k, dup_frac=0.05)
+        out.append((d, c, v, R, rand_factors(d, R, seed=seed + 100 + k)))
+    return out
+
+
+@pytest.mark.parametrize("prec,ftol", [(0, 1e-10), (1, 1e-6)])
+@pytest.mark.parametrize("nstreams", [1, 4])
+def test_batch_equals_single_runs(gpu, prec, ftol, nstreams):
+    inst = _instances(9, 300)
+    ats = [gpu.create_alto(d, c, v, nprtn=4) for d, c, v, _, _ in inst]
+    iters = 6
+    bf, bl, bfits, wall = gpu.cpd_alto_batch(ats, [x[4] for x in inst], [x[3] for x in inst], iters,
+                                             prec=prec, nstreams=nstreams)
+    assert wall > 0
+    for k, (d, c, v, R, F) in enumerate(inst):
+        sf, sl, sfits, _, _ = gpu.cpd_alto(ats[k], F, R, iters, prec=prec)
+        assert len(bfits[k]) == len(sfits) == iters
+        np.testing.assert_allclose(bfits[k], sfits, rtol=0, atol=ftol)
+        if prec == 0:
+            np.testing.assert_allclose(bl[k], sl, rtol=1e-9)
+            for a, b in zip(bf[k], sf):
+                np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-12)
+    for a in ats:
+        a.close()
+
+
+def test_batch_instance_matches_oracle(gpu):
+    d, c, v, R, F = _instances(1, 310)[0]
+    ref = O.AltoTensor(d, c, v, nprtn=4)
+    _, _, rfits, _ = ref.cpd_als(F, R, 5)
+    at = gpu.create_alto(d, c, v, nprtn=4)
+    _, _, bfits, _ = gpu.cpd_alto_batch([at], [F], [R], 5, prec=0, nstreams=2)
+    np.testing.assert_allclose(bfits[0], rfits, rtol=0, atol=1e-8)
+    at.close()
+
+
+def test_batch_rejects_verbose_and_bad_stream_count(gpu):
+    d, c, v, R, F = _instances(1, 320)[0]
+    at = gpu.create_alto(d, c, v)
+    with pytest.raises(gpu.AltoError, match="invalid argument"):
+        gpu.cpd_alto_batch([at], [F], [R], 2, nstreams=0)
+    at.close()
