@@ -332,18 +332,19 @@ def cpstream(dims, cidx, vals, stream_mode, rank, max_iters, eps, nonneg, init_f
     admm = np.zeros(nsl)
     grams = np.zeros((N, rank, rank))
     tim = np.zeros(4)
+    ferr = np.zeros(1)
     n = lib().alto_hip_cpstream(ctypes.byref(c), ctypes.c_int(stream_mode), ctypes.c_int(rank),
                                 ctypes.c_int(max_iters), ctypes.c_double(eps), ctypes.c_int(1 if nonneg else 0),
                                 ctypes.c_int(max_slices), ctypes.c_int(prec), iarr, oarr,
                                 out_dims.ctypes.data_as(_u64p),
                                 trows.ctypes.data_as(_f64p), outer.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                                 admm.ctypes.data_as(_f64p), grams.ctypes.data_as(_f64p),
-                                tim.ctypes.data_as(_f64p), _vp(stream))
+                                ferr.ctypes.data_as(_f64p), tim.ctypes.data_as(_f64p), _vp(stream))
     if n < 0:
         check(n, "cpstream")
     outs = [o[: int(out_dims[m])] for m, o in enumerate(outs)]
     return dict(factors=outs, dims=out_dims, time_rows=trows[:n], outer_iters=outer[:n],
-                admm_iters=admm[:n], grams=grams, slices=n, timings=tim)
+                admm_iters=admm[:n], grams=grams, slices=n, timings=tim, final_error=float(ferr[0]))
 
 
 def rowsparse_mttkrp(dims, cidx, vals, mode, stream_mode, rank, rsp_vals, ridx, stream=None):
@@ -373,6 +374,25 @@ def rowsparse_mttkrp(dims, cidx, vals, mode, stream_mode, rank, rsp_vals, ridx, 
                                           ctypes.byref(nnzr), _vp(stream)), "rowsparse_mttkrp")
     n = nnzr.value
     return rowind[:n], out[:n]
+
+
+def cpd_error(dims, cidx, vals, factors, stream=None):
+    """cpd_error (src/streaming_cpd.cpp:368-464) on host arrays: relative
+    residual of the Kruskal model (weights 1).  Returns (err, (||X||^2,
+    ||M||^2, <X, M>))."""
+    c, keep = _coo(dims, cidx, vals, False)
+    N = len(dims)
+    R = int(np.asarray(factors[0]).shape[1])
+    fs = [np.ascontiguousarray(f, dtype=np.float64) for f in factors]
+    for m, f in enumerate(fs):
+        if f.shape != (int(dims[m]), R):
+            raise AltoError("factor %d has shape %s, expected (%d, %d)" % (m, f.shape, int(dims[m]), R))
+    arr = (_f64p * N)(*[f.ctypes.data_as(_f64p) for f in fs])
+    err = ctypes.c_double()
+    parts = np.zeros(3)
+    check(lib().alto_hip_cpd_error(ctypes.byref(c), ctypes.c_int(R), arr, ctypes.byref(err),
+                                   parts.ctypes.data_as(_f64p), _vp(stream)), "cpd_error")
+    return err.value, tuple(parts)
 
 
 def gather_probe(table_bytes, gathers=1 << 26, seed=1, stream=None):

@@ -28,6 +28,9 @@ namespace alto {
 
 int run_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const *factors,
                void *out, int out_f64, int algo, uint64_t per_group, int flags, hipStream_t s);
+int cpd_error_dev(int N, const uint64_t *rows, uint64_t nnz, const uint64_t *const *cidx,
+                  const double *vals, const double *const *factors, int R, hipStream_t s,
+                  double *out);
 
 static inline int pow2(int r) {
     int p = 1;
@@ -292,7 +295,8 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
                                  double epsilon, int nonneg, int max_slices, int prec,
                                  const double *const *init_factors, double *const *out_factors,
                                  uint64_t *out_dims, double *out_time_rows, int *out_outer_iters,
-                                 double *out_admm_iters, double *out_grams, double *timings,
+                                 double *out_admm_iters, double *out_grams,
+                                 double *out_final_error, double *timings,
                                  void *stream) {
     if (!coo || coo->on_device || !coo->dims || !coo->cidx || !coo->vals || !init_factors)
         return ALTO_ERR_ARG;
@@ -426,6 +430,9 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
     ALTO_TRY(admm_it.alloc(sizeof(double)));
     ALTO_TRY(colp.alloc(sizeof(double) * 3 * R * kMaxBlocks));
     ALTO_TRY(lam.alloc(sizeof(double) * R));
+    DevBuf trows;
+    ALTO_TRY(trows.alloc(sizeof(double) * std::max(nsl, 1) * R));
+    ALTO_TRY(launch_fill_zero(trows.as<double>(), (uint64_t)std::max(nsl, 1) * R, st));
     TensorImpl at;
     const TileShape ts = tile_shape(R);
     const size_t lds_first = sizeof(double) * (2 * RR + 2 * (size_t)ts.TR * R);
@@ -608,6 +615,9 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
         if (out_time_rows)
             ALTO_HIP_TRY(hipMemcpyAsync(out_time_rows + (size_t)k * R, U[s].p, sizeof(double) * R,
                                         hipMemcpyDeviceToHost, st));
+        if (out_final_error)
+            ALTO_HIP_TRY(hipMemcpyAsync(trows.as<double>() + (size_t)k * R, U[s].p, sizeof(double) * R,
+                                        hipMemcpyDeviceToDevice, st));
         ALTO_HIP_TRY(hipMemcpyAsync(&ait, admm_it.p, sizeof(double), hipMemcpyDeviceToHost, st));
         ALTO_HIP_TRY(hipStreamSynchronize(st));
         if (out_outer_iters) out_outer_iters[k] = outer;
@@ -617,6 +627,26 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
     ALTO_HIP_TRY(hipEventSynchronize(ev[1]));
     float tot_ms = 0;
     (void)hipEventElapsedTime(&tot_ms, ev[0], ev[1]);
+    if (out_final_error) {
+        // cpd_error(X, factored) (streaming_cpd.cpp:307-345): the processed
+        // nonzeros in relabeled coordinates, the time factor = the s_t rows.
+        DevBuf tidx;
+        ALTO_TRY(tidx.alloc(sizeof(uint64_t) * std::max<uint64_t>(used, 1)));
+        if (used)
+            ALTO_HIP_TRY(hipMemcpyAsync(tidx.p, cid[s].data(), sizeof(uint64_t) * used,
+                                        hipMemcpyHostToDevice, st));
+        const uint64_t *cp[kMaxModes];
+        const double *fp[kMaxModes];
+        uint64_t rows[kMaxModes];
+        for (int m = 0; m < N; ++m) {
+            cp[m] = m == s ? tidx.as<uint64_t>() : dci[m].as<uint64_t>();
+            fp[m] = m == s ? trows.as<double>() : U[m].as<double>();
+            rows[m] = m == s ? (uint64_t)std::max(nsl, 1) : std::max<uint64_t>(cur[m], 1);
+        }
+        double eo[4];
+        ALTO_TRY(cpd_error_dev(N, rows, used, cp, dvals.as<double>(), fp, R, st, eo));
+        *out_final_error = eo[0];
+    }
     for (int m = 0; m < N; ++m) {
         if (out_dims) out_dims[m] = cur[m];
         if (out_factors && out_factors[m] && cur[m])

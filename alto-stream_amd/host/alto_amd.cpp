@@ -274,12 +274,15 @@ int cpstream(SparseTensor *X, int rank, int max_iters, int streaming_mode, FType
     std::vector<int> outer(X->dims[streaming_mode]);
     std::vector<double> admm(X->dims[streaming_mode]);
     double tim[4] = {0, 0, 0, 0};
+    double err = 0.0;
     int n = alto_hip_cpstream(&c, streaming_mode, rank, max_iters, epsilon, nonneg ? 1 : 0, max_slices, prec,
                               ip.data(), nullptr, nullptr, trows.data(), outer.data(), admm.data(), nullptr,
-                              tim, nullptr);
+                              &err, tim, nullptr);
     if (n < 0) altoamd::fail("cpstream", n);
     for (int k = 0; k < n; ++k)
         fprintf(stderr, "slice %d: outer iterations %d, ADMM iterations %.0f\n", k, outer[k], admm[k]);
+    fprintf(stderr, "Total CPSTREAM time (#it, total, iter)\n%d\t%f\t%f\n", n, tim[0], tim[0]);
     printf("Total CPSTREAM time: %f s (%d slices, %.2f slices/s)\n", tim[0], n, n / (tim[0] > 0 ? tim[0] : 1));
+    printf("final fit error: %f\n", err * err);  // streaming_cpd.cpp:345
     return n;
 }

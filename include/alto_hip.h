@@ -170,14 +170,29 @@ int alto_hip_cpd_als_batch(int ntensors, const alto_hip_tensor *tensors,
                            double *const *lambdas, double *const *fits, int *iters,
                            int nstreams, double *timings);
 
+/* ---- model error: cpd_error (src/streaming_cpd.cpp:368-464) with
+ * kruskal_norm (src/kruskal_model.cpp:400-442) ----
+ * err = sqrt(||X||^2 + ||M||^2 - 2 <X, M>) / ||X|| for the Kruskal model of
+ * the given factors with all weights 1 (cpd_error sets them to 1):
+ * <X, M> from the COO MTTKRP of the mode with the fewest rows (dims[m] rows
+ * per factor, rank columns, FP64), ||X||^2 = sum of squared stored values.
+ * The reference prints err^2 as "final fit error".  parts (may be NULL):
+ * ||X||^2, ||M||^2, <X, M>.  Arrays host or device as coo->on_device says;
+ * a coordinate >= dims[m] is ALTO_ERR_ARG. */
+int alto_hip_cpd_error(const alto_hip_coo *coo, int rank, const double *const *factors,
+                       double *err, double *parts, void *stream);
+
 /* ---- CP-stream: cpstream + cpstream_alto_iter (src/streaming_cpd.cpp:26-366,
  * 1198-1519) with admm_cf_ (src/admm.cpp:216-429), CPSTREAM_ALTO model,
  * timeslice limit 1.  init_factors: host, nmode arrays of dims[m] x rank
  * rows used when the model grows (row order = first-appearance labels).
  * Outputs (host, may be NULL): final factors (dims[m] x rank each), final
  * dims, the time row s_t per slice, outer iterations per slice, ADMM
- * iterations per slice, final Grams (nmode*rank*rank).  timings (may be
- * NULL): 4 doubles: total seconds, first slice-build seconds, 0, slices.
+ * iterations per slice, final Grams (nmode*rank*rank), and the final error
+ * of cpstream's closing cpd_error call (streaming_cpd.cpp:307-345; see
+ * alto_hip_cpd_error) over the processed slices in the relabeled index
+ * space, the time factor being the s_t rows.  timings (may be NULL): 4
+ * doubles: slice-loop seconds, slice-build seconds, MTTKRP seconds, slices.
  * prec: ALTO_PREC_F64 or ALTO_PREC_F32 (MTTKRP arithmetic; the solve, ADMM
  * and Grams are FP64 either way).  rank <= 64.
  * Returns the number of slices processed, or a negative status. */
@@ -186,7 +201,8 @@ int alto_hip_cpstream(const alto_hip_coo *coo, int stream_mode, int rank,
                       int prec, const double *const *init_factors, double *const *out_factors,
                       uint64_t *out_dims, double *out_time_rows,
                       int *out_outer_iters, double *out_admm_iters,
-                      double *out_grams, double *timings, void *stream);
+                      double *out_grams, double *out_final_error, double *timings,
+                      void *stream);
 
 /* ---- row-sparse MTTKRP: rowsparse_mttkrp (include/rowsparse_mttkrp.hpp:18-41,
  * src/rowsparse_mttkrp.cpp:97-187) ----

@@ -289,6 +289,58 @@ def cpstream(dims, cidx, vals, stream_mode, rank, max_iters, eps, nonneg, init_f
                 admm_iters=admm[:n], grams=grams, slices=n)
 
 
+def cpd_error(cidx, vals, factors):
+    """cpd_error (src/streaming_cpd.cpp:368-464) with kruskal_norm
+    (src/kruskal_model.cpp:400-442), weights 1: the mode with the fewest
+    factor rows (first such) gives <X, M> = sum_r sum_i U[i, r] MTTKRP[i, r]
+    (per-column sums first, then over columns, as the reference's accumF);
+    ||X||^2 = sum of squared values; ||M||^2 = sum_{i<=j} (2 - [i==j]) *
+    prod_m (U_m^T U_m)[i, j] (absolute value).  Returns (err, parts)."""
+    factors = [np.ascontiguousarray(f, dtype=np.float64) for f in factors]
+    rows = [f.shape[0] for f in factors]
+    mn = int(np.argmin(rows))
+    mt = mttkrp_coo(cidx, vals, factors, mn, out=np.zeros_like(factors[mn]))
+    acc = np.zeros(factors[mn].shape[1])
+    for i in range(rows[mn]):
+        acc += factors[mn][i] * mt[i]
+    inner = 0.0
+    for r in range(len(acc)):
+        inner += acc[r]
+    xx = float(np.dot(vals, vals))
+    R = factors[0].shape[1]
+    scratch = np.ones((R, R))
+    for f in factors:
+        scratch *= f.T @ f
+    zz = 0.0
+    for i in range(R):
+        zz += scratch[i, i]
+        for j in range(i + 1, R):
+            zz += scratch[i, j] * 2
+    zz = abs(zz)
+    resid = np.sqrt(xx + zz - 2 * inner)
+    return resid / np.sqrt(xx), (xx, zz, inner)
+
+
+def stream_relabel(cidx, vals, stream_mode):
+    """The tensor as StreamingSparseTensor leaves it (src/streaming_sptensor.cpp:69-124):
+    sorted by (stream_mode, stream_mode+1, ...) mod N (tt_sort, sort.cpp:477-519;
+    ties keep input order here) and every other mode relabelled by first
+    appearance in that order.  Returns (cidx, vals)."""
+    N = len(cidx)
+    keys = [np.asarray(cidx[(stream_mode + t) % N], dtype=np.uint64) for t in range(N)]
+    order = np.lexsort(keys[::-1])
+    out = [np.asarray(c, dtype=np.uint64)[order].copy() for c in cidx]
+    for m in range(N):
+        if m == stream_mode:
+            continue
+        _, first = np.unique(out[m], return_index=True)
+        lab = np.empty(len(first), dtype=np.uint64)
+        lab[np.argsort(first, kind="stable")] = np.arange(len(first), dtype=np.uint64)
+        uniq = np.unique(out[m])
+        out[m] = lab[np.searchsorted(uniq, out[m])]
+    return out, np.asarray(vals, dtype=np.float64)[order]
+
+
 def read_tns(path, zero_based=False):
     """Reads a FROSTT .tns file like ImportSparseTensor (sptensor.cpp:74-158):
     '#' lines skipped; dims = max index (+1 if zero based)."""

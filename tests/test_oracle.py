@@ -376,3 +376,43 @@ def test_rowsparse_mttkrp_oracle_brute_force_tiny():
     exp0 = 2.0 * np.array([1.0, 2.0]) * np.array([0.5, 1.0]) + 5.0 * np.array([1.0, 2.0]) * np.array([2.0, -1.0])
     exp2 = 3.0 * np.array([3.0, 4.0]) * np.array([0.5, 1.0])
     np.testing.assert_allclose(rows, [exp0, exp2], rtol=0, atol=0)
+
+
+def test_cpd_error_oracle_is_dense_relative_residual():
+    """cpd_error's closed form equals ||X - M|| / ||X|| computed densely
+    (weights 1), for a tensor with distinct coordinates."""
+    rng = np.random.default_rng(81)
+    dims, R = [5, 4, 6], 3
+    lin = rng.choice(np.prod(dims), size=40, replace=False)
+    cidx = list(np.unravel_index(lin, dims))
+    cidx = [c.astype(np.uint64) for c in cidx]
+    vals = rng.uniform(-1, 2, 40)
+    F = [rng.uniform(-1, 1, (d, R)) for d in dims]
+    err, (xx, zz, inner) = O.cpd_error(cidx, vals, F)
+    X = np.zeros(dims)
+    X[tuple(c.astype(np.int64) for c in cidx)] = vals
+    M = np.einsum("ir,jr,kr->ijk", *F)
+    assert abs(err - np.linalg.norm(X - M) / np.linalg.norm(X)) < 1e-12
+    assert abs(zz - np.linalg.norm(M) ** 2) < 1e-10 * zz
+    assert abs(inner - np.sum(X * M)) < 1e-12 * max(1.0, abs(inner))
+
+
+def test_stream_relabel_is_sorted_first_appearance_bijection():
+    rng = np.random.default_rng(82)
+    dims = [7, 50, 40]
+    cidx = [rng.integers(0, d, 400).astype(np.uint64) for d in dims]
+    vals = rng.random(400)
+    rc, rv = O.stream_relabel(cidx, vals, 0)
+    assert np.all(np.diff(rc[0].astype(np.int64)) >= 0)
+    for m in (1, 2):
+        seen = []
+        for x in rc[m]:
+            if int(x) not in seen:
+                seen.append(int(x))
+        assert seen == list(range(len(seen)))          # labels in order of first appearance
+    # the relabelling is a bijection per mode: the multiset of (coords, value) is kept
+    order = np.lexsort([cidx[2], cidx[1], cidx[0]])
+    np.testing.assert_array_equal(rv, vals[order])
+    for m in (1, 2):
+        pairs = set(zip(cidx[m][order].tolist(), rc[m].tolist()))
+        assert len({a for a, _ in pairs}) == len(pairs) == len({b for _, b in pairs})
