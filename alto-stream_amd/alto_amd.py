@@ -304,7 +304,7 @@ def cpd_alto_batch(tensors, factors, ranks, max_iters, eps=1e-30, prec=PREC_F64,
         farr[k] = ctypes.cast(a, ctypes.POINTER(_f64p))
     larr = (_f64p * n)(*[l.ctypes.data_as(_f64p) for l in lams])
     fitarr = (_f64p * n)(*[f.ctypes.data_as(_f64p) for f in fits])
-    tim = np.zeros(1)
+    tim = np.zeros(2)
     check(lib().alto_hip_cpd_als_batch(ctypes.c_int(n), handles, opts, farr, larr, fitarr,
                                        iters.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), ctypes.c_int(nstreams),
                                        tim.ctypes.data_as(_f64p)), "cpd_als_batch")

@@ -155,19 +155,138 @@ __global__ __launch_bounds__(256) void k_chol_inv(const double *__restrict__ gra
     if (t == 0 && flag) *flag = 0;
 }
 
+// One-wave variant of k_chol_inv for R <= 64 (lane i owns row i): the same
+// Cholesky arithmetic (A[a][b] -= A[a][j] * A[b][j], j ascending), then each
+// lane k forms column k of L^{-1} by forward substitution and row k of
+// vinv = L^{-T} L^{-1}.  Barriers of a one-wave workgroup are nearly free.
+__global__ __launch_bounds__(64) void k_chol_inv_w(const double *__restrict__ grams, int nmode,
+                                                   int mode, int R, double diag_add,
+                                                   double *__restrict__ vinv,
+                                                   int *__restrict__ flag,
+                                                   double *__restrict__ phi_out, int fail_mode,
+                                                   double *__restrict__ rho_out) {
+    extern __shared__ double lds[];
+    double *A = lds;           // R*R
+    double *X = lds + R * R;   // R*R, L^{-1}
+    __shared__ int fail;
+    __shared__ double dshift;
+    const int t = threadIdx.x;
+    const int RR = R * R;
+    for (int e = t; e < RR; e += 64) {
+        double v = 1.0;
+        for (int m = 0; m < nmode; ++m)
+            if (m != mode) v *= grams[(size_t)m * RR + e];
+        if (phi_out) phi_out[e] = v;
+        A[e] = v;
+    }
+    __syncthreads();
+    if (t == 0) {
+        fail = 0;
+        if (rho_out) {  // ADMM penalty: rho = tr(Phi) / R (admm.cpp:254)
+            double tr = 0.0;
+            for (int i = 0; i < R; ++i) tr += A[i * R + i];
+            dshift = tr / (double)R;
+            *rho_out = dshift;
+        } else {
+            dshift = diag_add;
+        }
+    }
+    __syncthreads();
+    if (t < R) A[t * R + t] += dshift;
+    __syncthreads();
+    for (int j = 0; j < R; ++j) {
+        const double d = A[j * R + j];
+        if (!(d > 0.0)) {
+            if (t == 0) fail = 1;
+            break;
+        }
+        const double l = sqrt(d);
+        __syncthreads();
+        if (t == j) A[j * R + j] = l;
+        if (t > j && t < R) A[t * R + j] /= l;
+        __syncthreads();
+        if (t > j && t < R) {
+            const double aij = A[t * R + j];
+            for (int b = j + 1; b <= t; ++b) A[t * R + b] -= aij * A[b * R + j];
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    if (fail && fail_mode == 1) {
+        for (int e = t; e < RR; e += 64) vinv[e] = (e / R == e % R) ? 1.0 : 0.0;
+        if (t == 0 && flag) *flag = 1;
+        return;
+    }
+    if (fail) {
+        if (t == 0) {
+            double mx = 0.0;
+            for (int r = 0; r < R; ++r) {
+                double v = 1.0;
+                for (int m = 0; m < nmode; ++m)
+                    if (m != mode) v *= grams[(size_t)m * RR + r * R + r];
+                mx = fabs(v) > mx ? fabs(v) : mx;
+            }
+            dshift = mx;
+        }
+        __syncthreads();
+        const double tol = dshift * R * DBL_EPSILON;
+        for (int e = t; e < RR; e += 64) {
+            int i = e / R, jj = e % R;
+            double v = 0.0;
+            if (i == jj) {
+                double dd = 1.0;
+                for (int m = 0; m < nmode; ++m)
+                    if (m != mode) dd *= grams[(size_t)m * RR + e];
+                v = fabs(dd) > tol ? 1.0 / dd : 0.0;
+            }
+            vinv[e] = v;
+        }
+        if (t == 0 && flag) *flag = 1;
+        return;
+    }
+    if (t < R) {  // column t of L^{-1}
+        const int k = t;
+        for (int i = 0; i < k; ++i) X[i * R + k] = 0.0;
+        for (int i = k; i < R; ++i) {
+            double sacc = i == k ? 1.0 : 0.0;
+            for (int p = k; p < i; ++p) sacc -= A[i * R + p] * X[p * R + k];
+            X[i * R + k] = sacc / A[i * R + i];
+        }
+    }
+    __syncthreads();
+    if (t < R) {  // row t of L^{-T} L^{-1}
+        const int a = t;
+        for (int b = 0; b < R; ++b) {
+            const int k0 = a > b ? a : b;
+            double sacc = 0.0;
+            for (int k = k0; k < R; ++k) sacc += X[k * R + a] * X[k * R + b];
+            vinv[a * R + b] = sacc;
+        }
+    }
+    if (t == 0 && flag) *flag = 0;
+}
+
 int launch_chol_inv(const double *grams, int nmode, int mode, int R, double diag_add,
                     bool copy_first, double *vinv, int *flag, double *phi_out, hipStream_t s,
                     int fail_mode, double *rho_out) {
     (void)copy_first;  // 1.0 * G == G exactly: both Hadamard orders agree bitwise
     if (R < 1 || R > kMaxRank) return ALTO_ERR_UNSUPPORTED;
-    size_t lds = sizeof(double) * ((size_t)R * R + R);
     static std::once_flag attr;
     std::call_once(attr, [] {
         (void)hipFuncSetAttribute((const void *)k_chol_inv,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 256);
+        (void)hipFuncSetAttribute((const void *)k_chol_inv_w,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 256);
     });
-    k_chol_inv<<<1, 256, lds, s>>>(grams, nmode, mode, R, diag_add, vinv, flag, phi_out, fail_mode,
-                                   rho_out);
+    if (R <= 64) {
+        const size_t lds = sizeof(double) * 2 * (size_t)R * R;
+        k_chol_inv_w<<<1, 64, lds, s>>>(grams, nmode, mode, R, diag_add, vinv, flag, phi_out,
+                                        fail_mode, rho_out);
+    } else {
+        const size_t lds = sizeof(double) * ((size_t)R * R + R);
+        k_chol_inv<<<1, 256, lds, s>>>(grams, nmode, mode, R, diag_add, vinv, flag, phi_out,
+                                       fail_mode, rho_out);
+    }
     ALTO_LAUNCH_CHECK();
     return ALTO_OK;
 }

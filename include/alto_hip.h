@@ -163,8 +163,9 @@ int alto_hip_cpd_als(alto_hip_tensor t, const alto_hip_als_opts *opts,
  * are built first.  Per tensor k: opts[k] (verbose must be 0), factors[k][m]
  * (host FP64 dims x rank: initial model in, final model out), lambdas[k]
  * (rank), fits[k] (max_iters) and iters[k]; each may be NULL.  Each tensor's
- * results are those of alto_hip_cpd_als on it alone.  timings (may be NULL):
- * timings[0] = wall seconds of the batch.  Returns the first failing status. */
+ * results are those of alto_hip_cpd_als on it alone.  timings (may be NULL,
+ * 2 doubles): wall seconds of the CP-ALS runs, then of the schedule builds
+ * that precede them.  Returns the first failing status. */
 int alto_hip_cpd_als_batch(int ntensors, const alto_hip_tensor *tensors,
                            const alto_hip_als_opts *opts, double *const *const *factors,
                            double *const *lambdas, double *const *fits, int *iters,

@@ -87,3 +87,21 @@ def test_als_f64_signed_init_matches_oracle(gpu):
     gf, gl, gfits, gfe, _ = gpu.cpd_alto(at, F, 8, 5, prec=gpu.PREC_F64, fit_mode=gpu.FIT_EXACT)
     np.testing.assert_allclose(gfe, rfe, rtol=0, atol=1e-8)
     np.testing.assert_allclose(gl, rl, rtol=1e-6)
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-10), (1, 1e-6), (2, 1e-6)])
+def test_als_graph_replay_matches_checked_loop(gpu, prec, tol):
+    """epsilon <= 0 runs every iteration without the per-iteration check:
+    iteration 0 directly, the rest as replays of one captured graph.  Every
+    replay's fit and the final model equal the checked loop's."""
+    dims, cidx, vals, F = _setup([200, 150, 90, 30], 40000, 86, 16, dup=0.1)
+    at = gpu.create_alto(dims, cidx, vals, nprtn=8)
+    sf, sl, sfits, sfe, _ = gpu.cpd_alto(at, F, 16, 7, eps=1e-30, prec=prec)
+    gf, gl, gfits, gfe, tim = gpu.cpd_alto(at, F, 16, 7, eps=0.0, prec=prec)
+    assert len(gfits) == len(sfits) == 7
+    np.testing.assert_allclose(gfits, sfits, rtol=0, atol=tol)
+    np.testing.assert_allclose(gfe, sfe, rtol=0, atol=tol)
+    np.testing.assert_allclose(gl, sl, rtol=10 * tol)
+    for a, b in zip(gf, sf):
+        np.testing.assert_allclose(a, b, rtol=10 * tol, atol=10 * tol)
+    assert tim[0] > 0 and tim[7] > 0

@@ -29,12 +29,13 @@ def _instances(n, seed):
 
 @pytest.mark.parametrize("prec,ftol", [(0, 1e-10), (1, 1e-6)])
 @pytest.mark.parametrize("nstreams", [1, 4])
-def test_batch_equals_single_runs(gpu, prec, ftol, nstreams):
+@pytest.mark.parametrize("eps", [1e-30, 0.0])
+def test_batch_equals_single_runs(gpu, prec, ftol, nstreams, eps):
     inst = _instances(9, 300)
     ats = [gpu.create_alto(d, c, v, nprtn=4) for d, c, v, _, _ in inst]
     iters = 6
     bf, bl, bfits, wall = gpu.cpd_alto_batch(ats, [x[4] for x in inst], [x[3] for x in inst], iters,
-                                             prec=prec, nstreams=nstreams)
+                                             eps=eps, prec=prec, nstreams=nstreams)
     assert wall > 0
     for k, (d, c, v, R, F) in enumerate(inst):
         sf, sl, sfits, _, _ = gpu.cpd_alto(ats[k], F, R, iters, prec=prec)
