@@ -179,11 +179,13 @@ static int als_run(TensorImpl *t, const alto_hip_als_opts *o, double *const *fac
     for (int m = 0; m < N; ++m) ALTO_TRY(ensure_schedule(t, m, prec == ALTO_PREC_F64, s));
     ALTO_HIP_TRY(hipStreamSynchronize(s));
 
-    if (o->epsilon <= 0.0 && !o->verbose && o->max_iters > 0) {
-        // Every iteration runs (|delta fit| < epsilon never holds), so no
-        // per-iteration host check: iteration 0 is enqueued directly and the
-        // later ones, identical but for the max-norm, replay one captured
-        // graph; the fits are read once at the end.
+    if (!o->verbose && o->max_iters > 0) {
+        // Iteration 0 is enqueued directly; the later ones, identical but
+        // for the max-norm, replay one captured graph.  With epsilon > 0 the
+        // fit is read back after every iteration and the loop stops as the
+        // reference's does; with epsilon <= 0 (|delta fit| < epsilon never
+        // holds) the fits are read once at the end.
+        const bool check = o->epsilon > 0.0;
         ALTO_TRY(w.cnt.alloc(sizeof(int)));
         k_zero_int<<<1, 64, 0, s>>>(w.cnt.as<int>());
         ALTO_LAUNCH_CHECK();
@@ -225,37 +227,56 @@ static int als_run(TensorImpl *t, const alto_hip_als_opts *o, double *const *fac
         gx.s = s;
         Events evs;
         hipEvent_t e0 = evs.make(), e1 = evs.make();
+        std::vector<double> fd(2 * (size_t)o->max_iters);
+        int done = o->max_iters;
+        double prev = 0.0;
         ALTO_HIP_TRY(hipEventRecord(e0, s));
-        ALTO_TRY(enqueue(true));
-        if (o->max_iters > 1) {
-            hipGraph_t g = nullptr;
-            ALTO_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            const int st = enqueue(false);
-            const hipError_t ce = hipStreamEndCapture(s, &g);
-            if (st != ALTO_OK || ce != hipSuccess) {
-                if (g) (void)hipGraphDestroy(g);
-                if (st != ALTO_OK) return st;
-                ALTO_HIP_TRY(ce);
+        for (int it = 0; it < o->max_iters; ++it) {
+            if (it == 0) {
+                ALTO_TRY(enqueue(true));
+            } else {
+                if (!gx.ge) {
+                    hipGraph_t g = nullptr;
+                    ALTO_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                    const int st = enqueue(false);
+                    const hipError_t ce = hipStreamEndCapture(s, &g);
+                    if (st != ALTO_OK || ce != hipSuccess) {
+                        if (g) (void)hipGraphDestroy(g);
+                        if (st != ALTO_OK) return st;
+                        ALTO_HIP_TRY(ce);
+                    }
+                    const hipError_t ie = hipGraphInstantiate(&gx.ge, g, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(g);
+                    ALTO_HIP_TRY(ie);
+                }
+                ALTO_HIP_TRY(hipGraphLaunch(gx.ge, s));
             }
-            const hipError_t ie = hipGraphInstantiate(&gx.ge, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            ALTO_HIP_TRY(ie);
-            for (int it = 1; it < o->max_iters; ++it) ALTO_HIP_TRY(hipGraphLaunch(gx.ge, s));
+            if (check) {
+                ALTO_HIP_TRY(hipMemcpyAsync(&fd[2 * it], fitd.as<double>() + 2 * it, sizeof(double) * 2,
+                                            hipMemcpyDeviceToHost, s));
+                ALTO_HIP_TRY(hipStreamSynchronize(s));
+                const double fit = fd[2 * it];
+                if (it > 0 && fabs(prev - fit) < o->epsilon) {
+                    done = it + 1;
+                    break;
+                }
+                prev = fit;
+            }
         }
         ALTO_HIP_TRY(hipEventRecord(e1, s));
-        std::vector<double> fd(2 * (size_t)o->max_iters);
-        ALTO_HIP_TRY(hipMemcpyAsync(fd.data(), fitd.p, sizeof(double) * fd.size(), hipMemcpyDeviceToHost, s));
+        if (!check)
+            ALTO_HIP_TRY(hipMemcpyAsync(fd.data(), fitd.p, sizeof(double) * fd.size(), hipMemcpyDeviceToHost, s));
         ALTO_HIP_TRY(hipStreamSynchronize(s));
-        for (int it = 0; it < o->max_iters; ++it) {
+        for (int it = 0; it < done; ++it) {
             if (fits) fits[it] = fd[2 * it];
             if (fits_exact) fits_exact[it] = fd[2 * it + 1];
         }
         const double tot = elapsed(e0, e1) * 1e-3;
         if (timings) {
-            double tv[8] = {tot, 0, 0, 0, 0, 0, 0, tot / o->max_iters};
+            double tv[8] = {tot, 0, 0, 0, 0, 0, 0, tot / done};
             memcpy(timings, tv, sizeof(tv));
         }
-        if (iters) *iters = o->max_iters;
+        if (iters) *iters = done;
         return download_model(t, R, prec, Uf, mt, lam, factors, lambda, s);
     }
 

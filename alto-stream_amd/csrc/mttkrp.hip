@@ -863,7 +863,7 @@ int run_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const 
     a.rank = rank;
     a.prec = prec;
     a.out64 = prec == ALTO_PREC_F64 || out_f64;
-    a.per_group = per_group ? per_group : 64;
+    a.per_group = per_group;  // 0: chosen below from the entry count
     a.flags = flags < 0 ? 3 : flags;
     if (algo == 2) a.flags |= 4;  // output-ordered: runs inside a range own their row
     // decode-once kernel: nmode <= 5 and 32-bit element offsets into every factor
@@ -903,6 +903,16 @@ int run_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const 
         }
     } else {
         return ALTO_ERR_ARG;
+    }
+    if (a.per_group == 0) {
+        // 64 entries per lane group, fewer when that would leave the device
+        // short of about 2^18 threads (small tensors are latency bound: a
+        // group walks its entries one after another)
+        const uint64_t vpl = prec == ALTO_PREC_F64 ? 4 : 8;  // elements per lane
+        uint64_t lanes = (uint64_t)rank / vpl;
+        if (lanes < 1) lanes = 1;
+        uint64_t pg = a.nnz * lanes / (1ULL << 18);
+        a.per_group = pg < 4 ? 4 : pg > 64 ? 64 : pg;
     }
     int st = t->pkw == 1 ? dispatch_prec<1>(a, s) : dispatch_prec<2>(a, s);
     ALTO_TRY(st);

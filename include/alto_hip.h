@@ -149,7 +149,11 @@ typedef struct {
  * final model); lambda: host rank doubles (out).  fits / fits_exact: host
  * arrays of max_iters (may be NULL).  timings (may be NULL): 8 doubles,
  * seconds: total, MTTKRP, pseudo-inverse, copy, normalise, Gram, fit, and
- * per-iteration total.  Returns the iteration count in *iters. */
+ * per-iteration total; the per-phase entries are measured only with
+ * verbose = 1 (the reference's printout), otherwise they are 0 and the
+ * iterations after the first replay one captured hipGraph (with
+ * epsilon <= 0 every iteration runs and the fits are read once at the end).
+ * Returns the iteration count in *iters. */
 int alto_hip_cpd_als(alto_hip_tensor t, const alto_hip_als_opts *opts,
                      double *const *factors, double *lambda, double *fits,
                      double *fits_exact, int *iters, double *timings,
