@@ -545,7 +545,7 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
                 const uint64_t I = bd[j];
                 const int nb = row_blocks(I);
                 ALTO_TRY(launch_fill_zero(buf.as<double>(), I * R, st));
-                ALTO_TRY(run_mttkrp(&at, j, prec, R, fptr, buf.p, 1, 1, 0, -1, st));
+                ALTO_TRY(run_mttkrp(&at, j, prec, R, fptr, buf.p, 1, 2, 0, -1, st));
                 int nc = 0;
                 for (int m = 0; m < N; ++m) {
                     if (m == j || m == s) continue;

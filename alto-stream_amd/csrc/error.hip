@@ -5,13 +5,17 @@
 // where <X, M> = sum_ir U_min[i, r] * MTTKRP_min[i, r] over the mode with the
 // fewest rows (the first such mode), ||X||^2 = sum of squared stored values,
 // and ||M||^2 = sum over i <= j of (2 - [i == j]) * prod_m (U_m^T U_m)[i, j].
-// The MTTKRP is the COO one of the reference's mttkrp_par (atomic rows);
-// the sums are fixed-order two-level reductions.
+// The MTTKRP runs over an ALTO tensor of the input (the reference uses its
+// COO mttkrp_par); the sums are fixed-order two-level reductions.
 #include <math.h>
 
 #include "dense.hpp"
+#include "tensor.hpp"
 
 namespace alto {
+
+int run_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const *factors,
+               void *out, int out_f64, int algo, uint64_t per_group, int flags, hipStream_t s);
 
 constexpr int kDotBlocks = 1024;
 
@@ -108,8 +112,18 @@ int cpd_error_dev(int N, const uint64_t *rows, uint64_t nnz, const uint64_t *con
     ALTO_TRY(gpart.alloc(sizeof(double) * kMaxBlocks * R * R));
     ALTO_TRY(res.alloc(sizeof(double) * 4));
     ALTO_TRY(launch_fill_zero(mt.as<double>(), I * R, s));
-    alto_hip_coo c{N, rows, nnz, cidx, vals, 1};
-    ALTO_TRY(alto_hip_mttkrp_coo(&c, mn, R, factors, mt.as<double>(), s));
+    {
+        // the MTTKRP through an ALTO tensor and the mode's output-ordered
+        // schedule: the few rows of the shortest mode would serialise the
+        // reference's per-nonzero row updates
+        alto_hip_coo c{N, rows, nnz, cidx, vals, 1};
+        TensorImpl t;
+        ALTO_TRY(build_alto(&t, &c, 0, 0, false, s));
+        const void *fp[kMaxModes];
+        for (int m = 0; m < N; ++m) fp[m] = factors[m];
+        ALTO_TRY(run_mttkrp(&t, mn, ALTO_PREC_F64, R, fp, mt.p, 1, 2, 0, -1, s));
+        ALTO_HIP_TRY(hipStreamSynchronize(s));
+    }
     k_dot_part<<<kDotBlocks, 256, 0, s>>>(factors[mn], mt.as<double>(), I * R, dpart.as<double>());
     ALTO_LAUNCH_CHECK();
     ALTO_TRY(launch_sumsq(vals, nnz, xx.as<double>(), sq.as<double>(), s));
