@@ -252,6 +252,18 @@ def mttkrp_alto_par(target, factors, rank, at, out, prec=PREC_F64, out_f64=True,
                                      ctypes.c_int(algo), _vp(stream)), "mttkrp_alto_par")
 
 
+def mttkrp_config(target, factors, rank, at, out, prec=PREC_F64, out_f64=True, algo=2, segment=0, flags=-1,
+                  stream=None):
+    """mttkrp_alto_par with explicit launch knobs (segment: entries per lane
+    group, 0 = from the entry count; flags -1 = defaults)."""
+    _check_out(out, int(at.dims[target]), rank, out_f64 or prec == PREC_F64)
+    check(lib().alto_hip_mttkrp_config(_vp(at.handle), ctypes.c_int(target), ctypes.c_int(prec),
+                                       ctypes.c_int(rank), _ptrs(factors), _vp(out.ptr),
+                                       ctypes.c_int(1 if out_f64 else 0), ctypes.c_int(algo),
+                                       ctypes.c_uint64(segment), ctypes.c_int(flags), _vp(stream)),
+          "mttkrp_config")
+
+
 def mttkrp_coo(dims, cidx_dev, vals_dev, target, rank, factors, out, stream=None):
     c, keep = _coo(dims, cidx_dev, vals_dev, True)
     check(lib().alto_hip_mttkrp_coo(ctypes.byref(c), ctypes.c_int(target), ctypes.c_int(rank), _ptrs(factors),

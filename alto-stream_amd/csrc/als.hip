@@ -25,9 +25,6 @@
 
 namespace alto {
 
-int run_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const *factors,
-               void *out, int out_f64, int algo, uint64_t per_group, int flags, hipStream_t s);
-
 // inner = sum over blocks/columns of u*M; ||M||^2 exact and diagonal-only.
 // Writes (fit, exact fit) to fitd[2k], fitd[2k+1] with k = *cnt (then
 // incremented) when cnt != nullptr, else k = slot.
@@ -174,9 +171,9 @@ static int als_run(TensorImpl *t, const alto_hip_als_opts *o, double *const *fac
     }
     ALTO_TRY(w.sumsq.alloc(sizeof(double) * kSumsqBlocks));
     ALTO_TRY(launch_sumsq(t->vals64.as<double>(), t->nnz, normX.as<double>(), w.sumsq.as<double>(), s));
-    // Build the per-mode output-ordered schedules outside the timed loop
-    // (tensor preparation, like create_alto).
-    for (int m = 0; m < N; ++m) ALTO_TRY(ensure_schedule(t, m, prec == ALTO_PREC_F64, s));
+    // Build the per-mode output-ordered schedules and tune their launch
+    // shapes outside the timed loop (tensor preparation, like create_alto).
+    for (int m = 0; m < N; ++m) ALTO_TRY(prepare_mttkrp(t, m, prec, R, fptr, 1, s));
     ALTO_HIP_TRY(hipStreamSynchronize(s));
 
     if (!o->verbose && o->max_iters > 0) {
@@ -398,12 +395,25 @@ extern "C" int alto_hip_cpd_als_batch(int ntensors, const alto_hip_tensor *tenso
     int dev = 0;
     ALTO_HIP_TRY(hipGetDevice(&dev));
     const auto t0 = std::chrono::steady_clock::now();
-    // Schedules first, on one stream: they allocate and free device memory,
-    // which would serialise the concurrent runs.
+    // Schedules and launch tuning first, on one stream: they allocate and
+    // free device memory, which would serialise the concurrent runs, and the
+    // tuner times kernels, which concurrent runs would disturb.  The tuner
+    // reads zero factors of each mode's shape (timing does not depend on the
+    // values).
     hipStream_t s0 = default_stream();
+    uint64_t fmax = 1;
     for (int k = 0; k < ntensors; ++k)
         for (int m = 0; m < tensors[k]->impl.nmode; ++m)
-            ALTO_TRY(ensure_schedule(&tensors[k]->impl, m, opts[k].prec == ALTO_PREC_F64, s0));
+            fmax = std::max<uint64_t>(fmax, tensors[k]->impl.dims[m] * (uint64_t)opts[k].rank);
+    DevBuf zf;
+    ALTO_TRY(zf.alloc(sizeof(double) * fmax));
+    ALTO_TRY(launch_fill_zero(zf.as<double>(), fmax, s0));
+    for (int k = 0; k < ntensors; ++k) {
+        const void *zp[kMaxModes];
+        for (int m = 0; m < kMaxModes; ++m) zp[m] = zf.p;
+        for (int m = 0; m < tensors[k]->impl.nmode; ++m)
+            ALTO_TRY(prepare_mttkrp(&tensors[k]->impl, m, opts[k].prec, opts[k].rank, zp, 1, s0));
+    }
     ALTO_HIP_TRY(hipStreamSynchronize(s0));
     const auto t1 = std::chrono::steady_clock::now();
     // Largest tensors first.

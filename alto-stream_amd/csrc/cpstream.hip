@@ -26,8 +26,6 @@
 
 namespace alto {
 
-int run_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const *factors,
-               void *out, int out_f64, int algo, uint64_t per_group, int flags, hipStream_t s);
 int cpd_error_dev(int N, const uint64_t *rows, uint64_t nnz, const uint64_t *const *cidx,
                   const double *vals, const double *const *factors, int R, hipStream_t s,
                   double *out);

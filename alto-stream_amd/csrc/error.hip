@@ -14,9 +14,6 @@
 
 namespace alto {
 
-int run_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const *factors,
-               void *out, int out_f64, int algo, uint64_t per_group, int flags, hipStream_t s);
-
 constexpr int kDotBlocks = 1024;
 
 struct IdxArgs {

@@ -16,7 +16,12 @@
 //   algo 2: nonzeros in the mode's output order (ALTO order inside each row;
 //           the oidx order of the op_sched variant) -> one flush per row per
 //           group, atomics only where a row crosses a group boundary.
+#include <stdlib.h>
+
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "tensor.hpp"
 
@@ -844,9 +849,120 @@ static int dispatch_prec(const MttkrpArgs &a, hipStream_t s) {
     return ALTO_ERR_ARG;
 }
 
+// Autotuner: for the default configuration of an output-ordered MTTKRP with
+// at least kTuneMinEntries entries, the first call of each bucket (modes,
+// rank, precision, output type, key words, log2 of the entry count, log2 of
+// the output rows) times the candidates -- segment length {p/2, p, 2p}
+// around the heuristic p, fiber or decode-once kernel -- best of three runs
+// each on a scratch output.  The heuristic choice is kept unless a candidate
+// is more than 3% faster.  ALTO_AUTOTUNE=0 in the environment turns it off.
+constexpr uint64_t kTuneMinEntries = 1ULL << 20;
+
+struct TuneKey {
+    int nmode, rank, prec, out64, pkw, lg, lr;
+    bool operator<(const TuneKey &o) const {
+        return std::tie(nmode, rank, prec, out64, pkw, lg, lr) <
+               std::tie(o.nmode, o.rank, o.prec, o.out64, o.pkw, o.lg, o.lr);
+    }
+};
+struct TuneVal {
+    uint64_t per_group;
+    bool fiber;
+};
+static std::mutex g_tune_mu;
+static std::map<TuneKey, TuneVal> g_tune;
+
+static bool autotune_enabled() {
+    static const bool on = [] {
+        const char *e = getenv("ALTO_AUTOTUNE");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
+static int autotune(TensorImpl *t, MttkrpArgs &a, hipStream_t s) {
+    int lg = 0, lr = 0;
+    while ((1ULL << (lg + 1)) <= a.nnz) ++lg;
+    while (lr < 63 && (1ULL << (lr + 1)) <= t->dims[a.target]) ++lr;
+    const TuneKey key{t->nmode, a.rank, a.prec, a.out64 ? 1 : 0, t->pkw, lg, lr};
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) {
+            a.per_group = it->second.per_group;
+            a.fiber = it->second.fiber;
+            return ALTO_OK;
+        }
+    }
+    const uint64_t rows = t->dims[a.target];
+    const size_t osz = a.out64 ? 8 : 4;
+    DevBuf scratch;
+    ALTO_TRY(scratch.alloc(osz * rows * a.rank));
+    hipEvent_t e0, e1;
+    ALTO_HIP_TRY(hipEventCreate(&e0));
+    ALTO_HIP_TRY(hipEventCreate(&e1));
+    MttkrpArgs c = a;
+    c.out = scratch.p;
+    const uint64_t p0 = a.per_group;
+    const uint64_t pgs[3] = {p0, p0 > 4 ? p0 / 2 : 2, p0 * 2 < 128 ? p0 * 2 : 128};
+    int st = ALTO_OK;
+    // best of three runs of one configuration
+    auto time_cfg = [&](uint64_t pg, bool fib) -> float {
+        c.per_group = pg;
+        c.fiber = fib;
+        float best_ms = 1e30f;
+        for (int rep = 0; rep < 3 && st == ALTO_OK; ++rep) {
+            float ms = 0.f;
+            if (hipEventRecord(e0, s) != hipSuccess) st = ALTO_ERR_HIP;
+            if (st == ALTO_OK) st = t->pkw == 1 ? dispatch_prec<1>(c, s) : dispatch_prec<2>(c, s);
+            if (st == ALTO_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+                st = ALTO_ERR_HIP;
+            if (st == ALTO_OK) (void)hipEventElapsedTime(&ms, e0, e1);
+            best_ms = ms < best_ms ? ms : best_ms;
+        }
+        return best_ms;
+    };
+    // the heuristic configuration stays unless another is more than 3% faster
+    TuneVal pick{a.per_group, a.fiber};
+    const float base_ms = time_cfg(a.per_group, a.fiber);
+    float best = base_ms;
+    for (int fib = 0; fib < 2 && st == ALTO_OK; ++fib) {
+        if (fib && a.prec == ALTO_PREC_BF16) continue;  // no fiber kernel for BF16 rows
+        for (int i = 0; i < 3 && st == ALTO_OK; ++i) {
+            if (pgs[i] == a.per_group && (fib != 0) == a.fiber) continue;
+            const float ms = time_cfg(pgs[i], fib != 0);
+            if (st == ALTO_OK && ms < best && ms < 0.97f * base_ms) {
+                best = ms;
+                pick = TuneVal{pgs[i], fib != 0};
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ALTO_TRY(st);
+    ALTO_LAUNCH_CHECK();
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        g_tune.emplace(key, pick);
+    }
+    a.per_group = pick.per_group;
+    a.fiber = pick.fiber;
+    return ALTO_OK;
+}
+
+// Builds the output-ordered schedule and runs the autotuner for a default
+// MTTKRP of `target` (algo 2) without touching any output.  Allocates nothing
+// once both are done (a free would synchronise the device).
+int prepare_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const *factors,
+                   int out_f64, hipStream_t s) {
+    return run_mttkrp(t, target, prec, rank, factors, nullptr, out_f64, 2, 0, -1, s, true);
+}
+
 int run_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const *factors,
-               void *out, int out_f64, int algo, uint64_t per_group, int flags, hipStream_t s) {
-    if (target < 0 || target >= t->nmode || rank < 1 || rank > 4096 || !factors || !out)
+               void *out, int out_f64, int algo, uint64_t per_group, int flags, hipStream_t s,
+               bool prepare_only) {
+    if (target < 0 || target >= t->nmode || rank < 1 || rank > 4096 || !factors ||
+        (!out && !prepare_only))
         return ALTO_ERR_ARG;
     if (prec != ALTO_PREC_F64 && prec != ALTO_PREC_F32 && prec != ALTO_PREC_BF16) return ALTO_ERR_ARG;
     for (int m = 0; m < t->nmode; ++m)
@@ -904,6 +1020,7 @@ int run_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const 
     } else {
         return ALTO_ERR_ARG;
     }
+    const bool defaults = a.per_group == 0 && flags < 0;
     if (a.per_group == 0) {
         // 64 entries per lane group, fewer when that would leave the device
         // short of about 2^18 threads (small tensors are latency bound: a
@@ -914,6 +1031,9 @@ int run_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const 
         uint64_t pg = a.nnz * lanes / (1ULL << 18);
         a.per_group = pg < 4 ? 4 : pg > 64 ? 64 : pg;
     }
+    if (defaults && algo == 2 && a.decode_once && a.nnz >= kTuneMinEntries && autotune_enabled())
+        ALTO_TRY(autotune(t, a, s));
+    if (prepare_only) return ALTO_OK;
     int st = t->pkw == 1 ? dispatch_prec<1>(a, s) : dispatch_prec<2>(a, s);
     ALTO_TRY(st);
     ALTO_LAUNCH_CHECK();

@@ -18,9 +18,6 @@
 
 namespace alto {
 
-int run_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const *factors,
-               void *out, int out_f64, int algo, uint64_t per_group, int flags, hipStream_t s);
-
 // D[i] = rsp[ridx[i]] (zeros where ridx[i] < 0), or rsp[0] for every row
 // when ridx == nullptr (the stream mode).
 __global__ __launch_bounds__(256) void k_expand_rows(const double *__restrict__ rsp, uint64_t nrsp,

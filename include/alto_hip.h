@@ -121,10 +121,14 @@ int alto_hip_mttkrp(alto_hip_tensor t, int target, int prec, int rank,
 int alto_hip_mttkrp_algo(alto_hip_tensor t, int target, int prec, int rank,
                          const void *const *factors, void *out, int out_f64,
                          int algo, void *stream);
-/* Same, with the kernel's tuning knobs (the autotuner's entry point):
- * segment = nonzeros per lane group (0 = default 256); flags bit 0 =
- * non-temporal key/value stream loads, bit 1 = reuse the gathered product
- * across equal consecutive keys (-1 = default: both). */
+/* Same, with the kernel's tuning knobs, which the defaults of the calls
+ * above pick (autotuned once per shape bucket for >= 2^20 schedule entries,
+ * unless ALTO_AUTOTUNE=0 is in the environment):
+ * segment = entries per lane group (0 = from the entry count, 4..64);
+ * flags bit 0 = non-temporal key/value stream loads, bit 1 = reuse the
+ * gathered product across equal consecutive keys, bit 3 = no decode-once
+ * kernel, bit 4 = no fiber kernel (-1 = defaults: bits 0 and 1).
+ * No reference counterpart (the reference fixes its launch shape). */
 int alto_hip_mttkrp_config(alto_hip_tensor t, int target, int prec, int rank,
                            const void *const *factors, void *out, int out_f64,
                            int algo, uint64_t segment, int flags, void *stream);

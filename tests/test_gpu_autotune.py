@@ -1,0 +1,73 @@
+"""GPU parity of the MTTKRP launch knobs and the autotuned default.
+
+Tensors with at least 2^20 schedule entries take the autotuned launch shape
+(segment length and kernel chosen by timing, cached per shape bucket).  Every
+choice the tuner can make, and the explicit knobs, must give the oracle's
+MTTKRP to the same bars as the default path: FP64 within 1e-12 and FP32
+within 2e-6 of the |X|-MTTKRP bound, per element.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import rand_factors, rand_tensor
+
+pytestmark = pytest.mark.gpu
+
+DIMS = [20000, 5000, 300]
+NNZ = 1_300_000  # above the tuner's 2^20-entry floor after merging duplicates
+
+
+@pytest.fixture(scope="module")
+def big():
+    dims, cidx, vals = rand_tensor(DIMS, NNZ, seed=91, dup_frac=0.02)
+    vals = vals - 0.5
+    return dims, cidx, vals
+
+
+def _expect(cidx, vals, F, mode):
+    exp = O.mttkrp_coo(cidx, vals, F, mode)
+    bnd = O.mttkrp_coo(cidx, np.abs(vals), [np.abs(f) for f in F], mode)
+    return exp, bnd
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 2e-6)])
+def test_autotuned_default_matches_oracle(gpu, big, prec, tol):
+    dims, cidx, vals = big
+    p = gpu.PREC_F64 if prec == "f64" else gpu.PREC_F32
+    R = 32
+    F = rand_factors(dims, R, seed=92, lo=-1, hi=1)
+    if prec == "f32":
+        F = [f.astype(np.float32).astype(np.float64) for f in F]
+    at = gpu.create_alto(dims, cidx, vals, nprtn=8)
+    devF = [gpu.to_prec(f, p) for f in F]
+    for mode in range(3):
+        print("prec", prec, "mode", mode, flush=True)
+        exp, bnd = _expect(cidx, vals, F, mode)
+        for call in range(2):  # the first call tunes, the second reads the cache
+            out = gpu.DeviceArray.zeros((dims[mode], R), np.float64)
+            gpu.mttkrp_alto_par(mode, devF, R, at, out, prec=p, out_f64=True)
+            got = out.numpy()
+            assert np.all(np.abs(got - exp) <= tol * bnd + 1e-300), (prec, mode, call)
+    at.close()
+
+
+@pytest.mark.parametrize("segment", [1, 4, 16, 64, 128])
+@pytest.mark.parametrize("flags", [-1, 3 | 16, 3 | 8, 0])
+def test_explicit_knobs_match_oracle(gpu, big, segment, flags):
+    dims, cidx, vals = big
+    R = 16
+    F = rand_factors(dims, R, seed=93, lo=-1, hi=1)
+    F32 = [f.astype(np.float32).astype(np.float64) for f in F]
+    at = gpu.create_alto(dims, cidx, vals, nprtn=8)
+    for p, FF, tol in ((gpu.PREC_F64, F, 1e-12), (gpu.PREC_F32, F32, 2e-6)):
+        devF = [gpu.to_prec(f, p) for f in FF]
+        for mode in (0, 2):
+            print("segment", segment, "flags", flags, "prec", p, "mode", mode, flush=True)
+            exp, bnd = _expect(cidx, vals, FF, mode)
+            out = gpu.DeviceArray.zeros((dims[mode], R), np.float64)
+            gpu.mttkrp_config(mode, devF, R, at, out, prec=p, out_f64=True, algo=2, segment=segment,
+                              flags=flags)
+            got = out.numpy()
+            assert np.all(np.abs(got - exp) <= tol * bnd + 1e-300), (segment, flags, p, mode)
+    at.close()
