@@ -359,6 +359,42 @@ def cpstream(dims, cidx, vals, stream_mode, rank, max_iters, eps, nonneg, init_f
                 admm_iters=admm[:n], grams=grams, slices=n, timings=tim, final_error=float(ferr[0]))
 
 
+def spcpstream(dims, cidx, vals, stream_mode, rank, max_iters, eps, init_factors, max_slices=0,
+               prec=PREC_F64, stream=None):
+    """spCP-stream (cpstream with use_spcpstream, src/streaming_cpd.cpp:26-366,
+    spcpstream_alto_iter 466-1085), host COO input.  Outputs as cpstream,
+    without ADMM counts."""
+    c, keep = _coo(dims, cidx, vals, False)
+    dims = np.ascontiguousarray(dims, dtype=np.uint64)
+    N = len(dims)
+    init = [np.ascontiguousarray(f, dtype=np.float64) for f in init_factors]
+    iarr = (_f64p * N)()
+    for i, f in enumerate(init):
+        iarr[i] = f.ctypes.data_as(_f64p)
+    nsl = int(dims[stream_mode])
+    outs = [np.zeros((int(dims[m]) if m != stream_mode else 1, rank)) for m in range(N)]
+    oarr = (_f64p * N)()
+    for i, f in enumerate(outs):
+        oarr[i] = f.ctypes.data_as(_f64p)
+    out_dims = np.zeros(N, dtype=np.uint64)
+    trows = np.zeros((nsl, rank))
+    outer = np.zeros(nsl, dtype=np.int32)
+    grams = np.zeros((N, rank, rank))
+    tim = np.zeros(4)
+    ferr = np.zeros(1)
+    n = lib().alto_hip_spcpstream(ctypes.byref(c), ctypes.c_int(stream_mode), ctypes.c_int(rank),
+                                  ctypes.c_int(max_iters), ctypes.c_double(eps), ctypes.c_int(max_slices),
+                                  ctypes.c_int(prec), iarr, oarr, out_dims.ctypes.data_as(_u64p),
+                                  trows.ctypes.data_as(_f64p), outer.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                  grams.ctypes.data_as(_f64p), ferr.ctypes.data_as(_f64p),
+                                  tim.ctypes.data_as(_f64p), _vp(stream))
+    if n < 0:
+        check(n, "spcpstream")
+    outs = [o[: int(out_dims[m])] for m, o in enumerate(outs)]
+    return dict(factors=outs, dims=out_dims, time_rows=trows[:n], outer_iters=outer[:n], grams=grams,
+                slices=n, timings=tim, final_error=float(ferr[0]))
+
+
 def rowsparse_mttkrp(dims, cidx, vals, mode, stream_mode, rank, rsp_vals, ridx, stream=None):
     """Row-sparse MTTKRP (rowsparse_mttkrp, src/rowsparse_mttkrp.cpp:97-187) on
     host arrays.  rsp_vals[m]: stored rows (k x rank) of mode m; ridx[m]:

@@ -22,13 +22,10 @@
 #include <vector>
 
 #include "dense.hpp"
+#include "stream.hpp"
 #include "tensor.hpp"
 
 namespace alto {
-
-int cpd_error_dev(int N, const uint64_t *rows, uint64_t nnz, const uint64_t *const *cidx,
-                  const double *vals, const double *const *factors, int R, hipStream_t s,
-                  double *out);
 
 static inline int pow2(int r) {
     int p = 1;
@@ -278,10 +275,76 @@ __global__ void k_zero_u64(uint64_t *p, uint64_t n) {
     if (i < n) p[i] = 0;
 }
 
-struct Slice {
-    uint64_t start, end;
-    uint64_t dims[kMaxModes];
-};
+int stream_prepare(const alto_hip_coo *coo, int s, StreamInput *out) {
+    const int N = coo->nmode;
+    const uint64_t nnz = coo->nnz;
+    std::vector<uint32_t> order(nnz);
+    std::iota(order.begin(), order.end(), 0u);
+    int lex[kMaxModes];
+    for (int t = 0; t < N; ++t) lex[t] = (s + t) % N;
+    for (uint64_t i = 0; i < nnz; ++i)
+        for (int m = 0; m < N; ++m)
+            if (coo->cidx[m][i] >= coo->dims[m]) {
+                set_error("a coordinate is not below its mode size");
+                return ALTO_ERR_ARG;
+            }
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        for (int t = 0; t < N; ++t) {
+            uint64_t x = coo->cidx[lex[t]][a], y = coo->cidx[lex[t]][b];
+            if (x != y) return x < y;
+        }
+        return false;
+    });
+    auto &cid = out->cid;
+    cid.assign(N, std::vector<uint64_t>(nnz));
+    out->vals.resize(nnz);
+    for (uint64_t i = 0; i < nnz; ++i) {
+        for (int m = 0; m < N; ++m) cid[m][i] = coo->cidx[m][order[i]];
+        out->vals[i] = coo->vals[order[i]];
+    }
+    order.clear();
+    order.shrink_to_fit();
+    for (int m = 0; m < N; ++m) {
+        if (m == s) continue;
+        std::vector<uint64_t> perm(coo->dims[m], UINT64_MAX);
+        uint64_t seen = 0;
+        for (uint64_t i = 0; i < nnz; ++i) {
+            uint64_t v = cid[m][i];
+            if (perm[v] == UINT64_MAX) perm[v] = seen++;
+            cid[m][i] = perm[v];
+        }
+    }
+    // slices (next_dynamic_batch with timeslice_limit 1)
+    out->slices.clear();
+    uint64_t ptr = 0, batch = 0, prev[kMaxModes] = {0};
+    while (ptr < nnz) {
+        uint64_t a = ptr;
+        while (a < nnz && cid[s][a] < batch) ++a;
+        uint64_t b = a;
+        while (b < nnz && cid[s][b] < batch + 1) ++b;
+        ++batch;
+        if (b == a) {
+            set_error("time slice %llu has no nonzeros", (unsigned long long)(batch - 1));
+            return ALTO_ERR_EMPTY_SLICE;
+        }
+        Slice sl;
+        sl.start = a;
+        sl.end = b;
+        for (int m = 0; m < N; ++m) {
+            if (m == s) {
+                sl.dims[m] = 1;
+                continue;
+            }
+            uint64_t mx = 0;
+            for (uint64_t i = a; i < b; ++i) mx = cid[m][i] > mx ? cid[m][i] : mx;
+            sl.dims[m] = std::max(mx + 1, prev[m]);
+            prev[m] = sl.dims[m];
+        }
+        out->slices.push_back(sl);
+        ptr = b;
+    }
+    return ALTO_OK;
+}
 
 static constexpr int kAdmmMaxIter = 25;
 
@@ -309,75 +372,12 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
     ALTO_TRY(ensure_init());
     hipStream_t st = to_stream(stream);
 
-    // ---- StreamingSparseTensor: sort by (s, s+1, ...) and relabel ----
-    std::vector<uint32_t> order(nnz);
-    std::iota(order.begin(), order.end(), 0u);
-    int lex[kMaxModes];
-    for (int t = 0; t < N; ++t) lex[t] = (s + t) % N;
-    for (uint64_t i = 0; i < nnz; ++i)
-        for (int m = 0; m < N; ++m)
-            if (coo->cidx[m][i] >= coo->dims[m]) {
-                set_error("a coordinate is not below its mode size");
-                return ALTO_ERR_ARG;
-            }
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
-        for (int t = 0; t < N; ++t) {
-            uint64_t x = coo->cidx[lex[t]][a], y = coo->cidx[lex[t]][b];
-            if (x != y) return x < y;
-        }
-        return false;
-    });
-    std::vector<std::vector<uint64_t>> cid(N, std::vector<uint64_t>(nnz));
-    std::vector<double> vals(nnz);
-    for (uint64_t i = 0; i < nnz; ++i) {
-        for (int m = 0; m < N; ++m) cid[m][i] = coo->cidx[m][order[i]];
-        vals[i] = coo->vals[order[i]];
-    }
-    order.clear();
-    order.shrink_to_fit();
-    uint64_t fdims[kMaxModes];
-    for (int m = 0; m < N; ++m) {
-        fdims[m] = coo->dims[m];
-        if (m == s) continue;
-        std::vector<uint64_t> perm(coo->dims[m], UINT64_MAX);
-        uint64_t seen = 0;
-        for (uint64_t i = 0; i < nnz; ++i) {
-            uint64_t v = cid[m][i];
-            if (perm[v] == UINT64_MAX) perm[v] = seen++;
-            cid[m][i] = perm[v];
-        }
-    }
-    // ---- slices (next_dynamic_batch with timeslice_limit 1) ----
-    std::vector<Slice> slices;
-    {
-        uint64_t ptr = 0, batch = 0, prev[kMaxModes] = {0};
-        while (ptr < nnz) {
-            uint64_t a = ptr;
-            while (a < nnz && cid[s][a] < batch) ++a;
-            uint64_t b = a;
-            while (b < nnz && cid[s][b] < batch + 1) ++b;
-            ++batch;
-            if (b == a) {
-                set_error("time slice %llu has no nonzeros", (unsigned long long)(batch - 1));
-                return ALTO_ERR_EMPTY_SLICE;
-            }
-            Slice sl;
-            sl.start = a;
-            sl.end = b;
-            for (int m = 0; m < N; ++m) {
-                if (m == s) {
-                    sl.dims[m] = 1;
-                    continue;
-                }
-                uint64_t mx = 0;
-                for (uint64_t i = a; i < b; ++i) mx = cid[m][i] > mx ? cid[m][i] : mx;
-                sl.dims[m] = std::max(mx + 1, prev[m]);
-                prev[m] = sl.dims[m];
-            }
-            slices.push_back(sl);
-            ptr = b;
-        }
-    }
+    // ---- StreamingSparseTensor: sort by (s, s+1, ...), relabel, slices ----
+    StreamInput sin;
+    ALTO_TRY(stream_prepare(coo, s, &sin));
+    const auto &cid = sin.cid;
+    const auto &vals = sin.vals;
+    const auto &slices = sin.slices;
     int nsl = (int)slices.size();
     if (max_slices > 0 && nsl > max_slices) nsl = max_slices;
 

@@ -1,0 +1,38 @@
+// Host-side stream preparation shared by CP-stream and spCP-stream.
+#pragma once
+#include <stdint.h>
+
+#include <vector>
+
+#include "tensor.hpp"
+
+namespace alto {
+
+// One time slice: nonzeros [start, end) of the prepared stream and the model
+// dims after it (non-stream modes: running max of index + 1; stream mode 1).
+struct Slice {
+    uint64_t start, end;
+    uint64_t dims[kMaxModes];
+};
+
+// StreamingSparseTensor (src/streaming_sptensor.cpp:69-264) with
+// next_dynamic_batch at timeslice limit 1: the nonzeros stably sorted by
+// (s, s+1, ...) mod N, every non-stream index relabelled by first appearance
+// in that order, one slice per time index.
+struct StreamInput {
+    std::vector<std::vector<uint64_t>> cid;  // per mode, relabelled (stream mode: time index)
+    std::vector<double> vals;
+    std::vector<Slice> slices;
+};
+
+// Host COO in; rejects coordinates >= dims (ALTO_ERR_ARG) and a time index
+// without nonzeros (ALTO_ERR_EMPTY_SLICE; the reference asserts).
+int stream_prepare(const alto_hip_coo *coo, int s, StreamInput *out);
+
+// cpd_error (src/streaming_cpd.cpp:368-464) over device COO and factors;
+// out[4]: err, ||X||^2, ||M||^2, <X, M>.
+int cpd_error_dev(int N, const uint64_t *rows, uint64_t nnz, const uint64_t *const *cidx,
+                  const double *vals, const double *const *factors, int R, hipStream_t s,
+                  double *out);
+
+}  // namespace alto

@@ -259,7 +259,7 @@ RowSparseMatrix *rowsparse_mttkrp(SparseTensor *X, RowSparseMatrix **rsp_mats, I
 
 // ---- streaming_cpd.cpp:26-366 ----
 int cpstream(SparseTensor *X, int rank, int max_iters, int streaming_mode, FType epsilon, IType seed,
-             bool nonneg, int max_slices, int prec) {
+             bool nonneg, int max_slices, int prec, bool use_spcpstream) {
     (void)seed;  // the reference seeds once with srand (main.cpp:188)
     const int N = X->nmodes;
     std::vector<std::vector<double>> init(N);
@@ -275,12 +275,18 @@ int cpstream(SparseTensor *X, int rank, int max_iters, int streaming_mode, FType
     std::vector<double> admm(X->dims[streaming_mode]);
     double tim[4] = {0, 0, 0, 0};
     double err = 0.0;
-    int n = alto_hip_cpstream(&c, streaming_mode, rank, max_iters, epsilon, nonneg ? 1 : 0, max_slices, prec,
-                              ip.data(), nullptr, nullptr, trows.data(), outer.data(), admm.data(), nullptr,
-                              &err, tim, nullptr);
-    if (n < 0) altoamd::fail("cpstream", n);
-    for (int k = 0; k < n; ++k)
-        fprintf(stderr, "slice %d: outer iterations %d, ADMM iterations %.0f\n", k, outer[k], admm[k]);
+    fprintf(stderr, "==== Executing %s (ALTO) ====\n", use_spcpstream ? "spCPSTREAM" : "CPSTREAM");
+    int n = use_spcpstream
+                ? alto_hip_spcpstream(&c, streaming_mode, rank, max_iters, epsilon, max_slices, prec, ip.data(),
+                                      nullptr, nullptr, trows.data(), outer.data(), nullptr, &err, tim, nullptr)
+                : alto_hip_cpstream(&c, streaming_mode, rank, max_iters, epsilon, nonneg ? 1 : 0, max_slices,
+                                    prec, ip.data(), nullptr, nullptr, trows.data(), outer.data(), admm.data(),
+                                    nullptr, &err, tim, nullptr);
+    if (n < 0) altoamd::fail(use_spcpstream ? "spcpstream" : "cpstream", n);
+    for (int k = 0; k < n; ++k) {
+        if (use_spcpstream) fprintf(stderr, "slice %d: outer iterations %d\n", k, outer[k]);
+        else fprintf(stderr, "slice %d: outer iterations %d, ADMM iterations %.0f\n", k, outer[k], admm[k]);
+    }
     fprintf(stderr, "Total CPSTREAM time (#it, total, iter)\n%d\t%f\t%f\n", n, tim[0], tim[0]);
     printf("Total CPSTREAM time: %f s (%d slices, %.2f slices/s)\n", tim[0], n, n / (tim[0] > 0 ? tim[0] : 1));
     printf("final fit error: %f\n", err * err);  // streaming_cpd.cpp:345

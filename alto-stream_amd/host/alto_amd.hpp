@@ -164,8 +164,10 @@ void cpd_alto(AltoTensor<LIT> *AT, KruskalModel *M, int max_iters, double epsilo
                    "cpd_alto");
 }
 
-// cpstream (streaming_cpd.cpp:26-366), CPSTREAM_ALTO, timeslice limit 1.
-// Rows are initialised with the reference's rand_val in mode order over the
-// final dims.  Returns the number of slices.
+// cpstream (streaming_cpd.cpp:26-366), timeslice limit 1: CPSTREAM_ALTO, or
+// SPCPSTREAM_ALTO when use_spcpstream (spcpstream_alto_iter, 466-1085; the
+// constraint is then unused, as in the reference).  Rows are initialised
+// with the reference's rand_val in mode order over the final dims.  Returns
+// the number of slices.
 int cpstream(SparseTensor *X, int rank, int max_iters, int streaming_mode, FType epsilon, IType seed,
-             bool nonneg, int max_slices = 0, int prec = ALTO_PREC_F64);
+             bool nonneg, int max_slices = 0, int prec = ALTO_PREC_F64, bool use_spcpstream = false);

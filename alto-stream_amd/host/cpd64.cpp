@@ -2,6 +2,7 @@
 // MI355X library.  Same flags and printed lines; the work runs on the GPU.
 //   -l als            CP-ALS (cpd_alto)                 main.cpp:335-372
 //   -l cpstream_alto  CP-stream (cpstream)              main.cpp:376-407
+//   -l spcpstream_alto  spCP-stream (cpstream, use_spcpstream)  main.cpp:163, 397
 //   -p                MTTKRP benchmark (BenchmarkAlto)  main.cpp:412-530
 //   -c                ALTO vs COO MTTKRP check           main.cpp:532-649
 // Extra long option: --prec f64|f32|bf16 (MTTKRP / factor storage precision).
@@ -46,7 +47,7 @@ static void Usage(const char *call) {
     fprintf(stderr, "\t-v or --version      Display version information\n");
     fprintf(stderr, "\t-i or --input        Input tensor file in text\n");
     fprintf(stderr, "\t-b or --bin          Input tensor file in binary\n");
-    fprintf(stderr, "\t-l or --model        als | cpstream_alto\n");
+    fprintf(stderr, "\t-l or --model        als | cpstream_alto | spcpstream_alto\n");
     fprintf(stderr, "\t-z or --constraint   nonneg | lasso (CP-stream)\n");
     fprintf(stderr, "\t-r or --rank         Rank\n");
     fprintf(stderr, "\t-m or --max-iter     Maximum outer iterations\n");
@@ -148,7 +149,7 @@ int main(int argc, char **argv) {
     int streaming_mode = -1;
     int nprtn = 1;
     int prec = ALTO_PREC_F64;
-    int model = -1;  // 0 als, 1 cpstream_alto
+    int model = -1;  // 0 als, 1 cpstream_alto, 2 spcpstream_alto
     bool nonneg = false;
     int c;
     while ((c = getopt_long(argc, argv, short_opt, long_opt, NULL)) != -1) {
@@ -162,6 +163,7 @@ int main(int argc, char **argv) {
             model_string = optarg;
             if (model_string == "ALS" || model_string == "als") model = 0;
             else if (model_string == "CPSTREAM_ALTO" || model_string == "cpstream_alto") model = 1;
+            else if (model_string == "SPCPSTREAM_ALTO" || model_string == "spcpstream_alto") model = 2;
             else fprintf(stderr, "Invalid -model: %s.\n", optarg);
             break;
         case 'z':
@@ -219,17 +221,17 @@ int main(int argc, char **argv) {
         printf("CPD (ALTO): %f s\n", now() - w);
         DestroyKruskalModel(M);
         destroy_alto(AT);
-    } else if (model == 1) {
+    } else if (model == 1 || model == 2) {
         if (streaming_mode < 0 || streaming_mode >= X->nmodes) {
             fprintf(stderr, "Need to provide -streaming_mode: %d\n", streaming_mode);
             return -1;
         }
         double w = now();
         cpstream(X, (int)rank, max_iters, streaming_mode, epsilon, 0, nonneg, 0,
-                 prec == ALTO_PREC_F64 ? ALTO_PREC_F64 : ALTO_PREC_F32);
-        printf("CPSTREAM (ALTO): %f s\n", now() - w);
+                 prec == ALTO_PREC_F64 ? ALTO_PREC_F64 : ALTO_PREC_F32, model == 2);
+        printf("%s (ALTO): %f s\n", model == 2 ? "spCPSTREAM" : "CPSTREAM", now() - w);
     } else {
-        fprintf(stderr, "No -l model given (als | cpstream_alto)\n");
+        fprintf(stderr, "No -l model given (als | cpstream_alto | spcpstream_alto)\n");
         return -1;
     }
     DestroySparseTensor(X);

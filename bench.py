@@ -397,6 +397,19 @@ def leg_c4(A, slices):
            "outer_iters_mean": round(float(np.mean(res["outer_iters"])), 2),
            "admm_iters_mean": round(float(np.mean(res["admm_iters"])), 2)}
     log("C4: %d slices in %.3f s (%.2f slices/s), wall %.1f s" % (slices, tim[0], slices / tim[0], wall))
+    # spCP-stream on the same stream (the reference's SPCPSTREAM_ALTO: row-sparse
+    # updates, no constraint), beside CP-stream without the constraint
+    sp = A.spcpstream(dims, hc, hv, 0, R, 10, 1e-6, init, prec=A.PREC_F32)
+    un = A.cpstream(dims, hc, hv, 0, R, 10, 1e-6, False, init, prec=A.PREC_F32)
+    ts, tu = float(sp["timings"][0]), float(un["timings"][0])
+    out["spcpstream"] = {"slices_per_s": round(slices / ts, 2), "slice_loop_s": round(ts, 3),
+                         "outer_iters_mean": round(float(np.mean(sp["outer_iters"])), 2),
+                         "final_error": sp["final_error"]}
+    out["cpstream_unconstrained"] = {"slices_per_s": round(slices / tu, 2), "slice_loop_s": round(tu, 3),
+                                     "outer_iters_mean": round(float(np.mean(un["outer_iters"])), 2),
+                                     "final_error": un["final_error"]}
+    log("C4: spCP-stream %.2f slices/s (%.2f outer iterations), CP-stream unconstrained %.2f slices/s (%.2f)" %
+        (slices / ts, np.mean(sp["outer_iters"]), slices / tu, np.mean(un["outer_iters"])))
     return out
 
 

@@ -213,6 +213,25 @@ int alto_hip_cpstream(const alto_hip_coo *coo, int stream_mode, int rank,
                       double *out_grams, double *out_final_error, double *timings,
                       void *stream);
 
+/* ---- spCP-stream: cpstream with use_spcpstream (model SPCPSTREAM_ALTO,
+ * src/streaming_cpd.cpp:26-366, main.cpp:163, 397) driving
+ * spcpstream_alto_iter (streaming_cpd.cpp:466-1085), timeslice limit 1.
+ * Per slice only the rows that occur in it are updated inside the outer
+ * loop (row-sparse MTTKRP over compact coordinates, update_salto_rowsparse,
+ * alto.hpp:301-434); the other rows take prevU * (Q Phi^-1) at the end.
+ * Constraints do not apply (the reference's spCP-stream solves
+ * unconstrained).  Arguments and outputs as alto_hip_cpstream, without the
+ * ADMM counts; out_grams holds the reference's grams array (non-stream
+ * modes: U^T U at the start of the last slice; stream mode: G_s).
+ * max_iters >= 1, rank <= 64.  Returns the number of slices processed, or a
+ * negative status. */
+int alto_hip_spcpstream(const alto_hip_coo *coo, int stream_mode, int rank,
+                        int max_iters, double epsilon, int max_slices, int prec,
+                        const double *const *init_factors, double *const *out_factors,
+                        uint64_t *out_dims, double *out_time_rows, int *out_outer_iters,
+                        double *out_grams, double *out_final_error, double *timings,
+                        void *stream);
+
 /* ---- row-sparse MTTKRP: rowsparse_mttkrp (include/rowsparse_mttkrp.hpp:18-41,
  * src/rowsparse_mttkrp.cpp:97-187) ----
  * Factor matrices are row-sparse, as the reference's RowSparseMatrix

@@ -948,3 +948,344 @@ int orc_cpstream(int nmode, const uint64_t *dims, uint64_t nnz,
     free(bv); free(keys);
     return slices;
 }
+
+/* ---------------- spCP-stream ---------------- */
+
+/* C = op(A) B for R x R row-major arrays (matmul, matrix.cpp:92-119). */
+static void rr_matmul(int R, const double *A, int ta, const double *B, double *C)
+{
+    for (int i = 0; i < R; ++i)
+        for (int j = 0; j < R; ++j) {
+            double t = 0.0;
+            for (int k = 0; k < R; ++k) t += (ta ? A[k * R + i] : A[i * R + k]) * B[k * R + j];
+            C[i * R + j] = t;
+        }
+}
+
+/* C = A^T B over `rows` rows of rows x R row-major arrays (matmul with
+ * transA, matrix.cpp:63-91; rsp_mataTb, rowsparse_matrix.cpp:56-66). */
+static void tall_atb(uint64_t rows, int R, const double *A, const double *B, double *C)
+{
+    for (int i = 0; i < R; ++i)
+        for (int j = 0; j < R; ++j) {
+            double t = 0.0;
+            for (uint64_t k = 0; k < rows; ++k) t += A[k * R + i] * B[k * R + j];
+            C[i * R + j] = t;
+        }
+}
+
+/* mat_form_gram (matrix.cpp:141-156): out = Hadamard of mats[m], m != mode. */
+static void hadamard_excl(int nmode, int R, double *const *mats, int mode, double *out)
+{
+    for (int q = 0; q < R * R; ++q) out[q] = 1.0;
+    for (int m = 0; m < nmode; ++m)
+        if (m != mode)
+            for (int q = 0; q < R * R; ++q) out[q] *= mats[m][q];
+}
+
+/* streaming_cpd.cpp:26-366 cpstream with model SPCPSTREAM_ALTO
+ * (use_spcpstream, main.cpp:163, 397), timeslice_limit 1, driving
+ * spcpstream_alto_iter (streaming_cpd.cpp:466-1085).  The stream handling,
+ * model growth and outputs are those of orc_cpstream; constraints are not
+ * used (spcpstream_alto_iter solves with pseudo_inverse only).  Per slice:
+ *  - nonzero_slices (streaming_cpd.cpp:1124-1161): the ascending distinct
+ *    indices of each mode in the slice (nz rows), the others (z rows) and
+ *    the compact map ridx;
+ *  - A_nz / A_nz_prev = the nz rows of U / prev U (convert_to_rspmat,
+ *    rowsparse_matrix.cpp:28-54); c = U^T U (upper triangle, mat_aTa,
+ *    matrix.cpp:252-267), h = prevU^T U; after the first slice
+ *    c_nz_prev = A_nz_prev^T A_nz_prev (mataTa_idx_based,
+ *    rowsparse_matrix.cpp:131-150) and c_z_prev = c_prev - c_nz_prev;
+ *  - update_salto_rowsparse (alto.hpp:301-434): the slice's ALTO tensor over
+ *    compact coordinates (dims = nz row counts);
+ *  - outer iterations (streaming_cpd.cpp:712-1000): s_t by MTTKRP over the
+ *    compact factors and a Cholesky solve with the Hadamard of the c's, the
+ *    temporal Gram update, then per non-stream mode m:
+ *      Q = Had_{k != m} h_k, Phi = Had_{k != m} c_k + 1e-12 I,
+ *      A_nz = (MTTKRP_m + A_nz_prev Q) Phi^-1,
+ *      c_nz = A_nz^T A_nz, h_nz = A_nz_prev^T A_nz, QPi = Q Phi^-1,
+ *      h_z = c_z_prev QPi, c_z = QPi^T h_z, c = c_nz + c_z, h = h_nz + h_z,
+ *      delta += sqrt(|tr c + tr c_prev - 2 tr h| / (tr c + 1e-12));
+ *  - z rows become prevU QPi, nz rows A_nz (streaming_cpd.cpp:1017-1045);
+ *    G_s *= 0.99; c_prev = c, c_nz_prev = c_nz.
+ * A failed Cholesky leaves the right-hand side unchanged (pseudo_inverse,
+ * matrix.cpp:226-250).  max_iters must be >= 1 (with 0 the reference reads
+ * an uninitialised Q Phi^-1).  out_grams: the reference's grams array (the
+ * non-stream modes' upper-triangle Grams from the start of the last slice,
+ * and G_s).  Returns the number of slices, or -1 on bad arguments. */
+int orc_spcpstream(int nmode, const uint64_t *dims, uint64_t nnz,
+                   const uint64_t *const *cidx_in, const double *vals_in,
+                   int s, int rank, int max_iters, double eps, int max_slices,
+                   double *const *init_factors, double *const *out_factors,
+                   uint64_t *out_dims, double *out_time_rows, int *out_outer_iters,
+                   double *out_grams)
+{
+    if (max_iters < 1) return -1;
+    int R = rank;
+    const size_t RR = (size_t)R * R;
+    int order[ORC_MAX_MODES];
+    for (int t = 0; t < nmode; ++t) order[t] = (s + t) % nmode;
+    uint64_t *idx = (uint64_t *)malloc(sizeof(uint64_t) * (nnz ? nnz : 1));
+    for (uint64_t i = 0; i < nnz; ++i) idx[i] = i;
+    struct lex_ctx lc = {cidx_in, order, nmode};
+    merge_sort_idx(nnz, idx, lex_less, &lc);
+    uint64_t *cidx[ORC_MAX_MODES];
+    double *vals = (double *)malloc(sizeof(double) * (nnz ? nnz : 1));
+    for (int m = 0; m < nmode; ++m) {
+        cidx[m] = (uint64_t *)malloc(sizeof(uint64_t) * (nnz ? nnz : 1));
+        for (uint64_t i = 0; i < nnz; ++i) cidx[m][i] = cidx_in[m][idx[i]];
+    }
+    for (uint64_t i = 0; i < nnz; ++i) vals[i] = vals_in[idx[i]];
+    free(idx);
+    for (int m = 0; m < nmode; ++m) {
+        if (m == s) continue;
+        uint64_t *perm = (uint64_t *)malloc(sizeof(uint64_t) * dims[m]);
+        for (uint64_t i = 0; i < dims[m]; ++i) perm[i] = UINT64_MAX;
+        uint64_t seen = 0;
+        for (uint64_t i = 0; i < nnz; ++i) {
+            uint64_t v = cidx[m][i];
+            if (perm[v] == UINT64_MAX) perm[v] = seen++;
+            cidx[m][i] = perm[v];
+        }
+        free(perm);
+    }
+
+    uint64_t cur[ORC_MAX_MODES] = {0}, prev_dim[ORC_MAX_MODES] = {0}, maxd[ORC_MAX_MODES];
+    for (int m = 0; m < nmode; ++m) maxd[m] = dims[m];
+    maxd[s] = 1;
+    double *U[ORC_MAX_MODES], *P[ORC_MAX_MODES], *G[ORC_MAX_MODES];
+    double *c[ORC_MAX_MODES], *h[ORC_MAX_MODES], *cnz[ORC_MAX_MODES], *hnz[ORC_MAX_MODES];
+    double *cz[ORC_MAX_MODES], *hz[ORC_MAX_MODES], *cprev[ORC_MAX_MODES];
+    double *czprev[ORC_MAX_MODES], *cnzprev[ORC_MAX_MODES], *Q[ORC_MAX_MODES], *QPi[ORC_MAX_MODES];
+    double *Anz[ORC_MAX_MODES], *Anzp[ORC_MAX_MODES];
+    int64_t *ridx[ORC_MAX_MODES];
+    uint64_t *nzr[ORC_MAX_MODES], nnzr[ORC_MAX_MODES], *cc[ORC_MAX_MODES], *bc[ORC_MAX_MODES];
+    for (int m = 0; m < nmode; ++m) {
+        U[m] = (double *)calloc(maxd[m] * R, sizeof(double));
+        P[m] = (double *)calloc(maxd[m] * R, sizeof(double));
+        Anz[m] = (double *)calloc(maxd[m] * R, sizeof(double));
+        Anzp[m] = (double *)calloc(maxd[m] * R, sizeof(double));
+        ridx[m] = (int64_t *)malloc(sizeof(int64_t) * maxd[m]);
+        nzr[m] = (uint64_t *)malloc(sizeof(uint64_t) * maxd[m]);
+        cc[m] = (uint64_t *)malloc(sizeof(uint64_t) * (nnz ? nnz : 1));
+        bc[m] = (uint64_t *)malloc(sizeof(uint64_t) * (nnz ? nnz : 1));
+        G[m] = (double *)calloc(RR, sizeof(double));
+        c[m] = (double *)calloc(RR, sizeof(double));     /* InitSparseCPGrams: zeros */
+        h[m] = (double *)calloc(RR, sizeof(double));
+        cnz[m] = (double *)calloc(RR, sizeof(double));
+        hnz[m] = (double *)calloc(RR, sizeof(double));
+        cz[m] = (double *)calloc(RR, sizeof(double));
+        hz[m] = (double *)calloc(RR, sizeof(double));
+        cprev[m] = (double *)calloc(RR, sizeof(double));
+        czprev[m] = (double *)calloc(RR, sizeof(double));
+        cnzprev[m] = (double *)calloc(RR, sizeof(double));
+        Q[m] = (double *)calloc(RR, sizeof(double));
+        QPi[m] = (double *)calloc(RR, sizeof(double));
+    }
+    double lambda[256];
+    double *Phi = (double *)malloc(sizeof(double) * RR);
+    double *old_gram = (double *)malloc(sizeof(double) * RR);
+    double *tmp = (double *)malloc(sizeof(double) * R);
+    double *bv = (double *)malloc(sizeof(double) * (nnz ? nnz : 1));
+    uint64_t *keys = (uint64_t *)malloc(sizeof(uint64_t) * 2 * (nnz ? nnz : 1));
+
+    uint64_t ptr = 0, batch = 0;
+    int slices = 0;
+    while (ptr < nnz && (max_slices <= 0 || slices < max_slices)) {
+        uint64_t start = ptr;
+        while (start < nnz && cidx[s][start] < batch) ++start;
+        uint64_t end = start;
+        while (end < nnz && cidx[s][end] < batch + 1) ++end;
+        ++batch;
+        uint64_t bn = end - start;
+        if (bn == 0) break;
+        uint64_t bd[ORC_MAX_MODES];
+        for (int m = 0; m < nmode; ++m) {
+            if (m == s) { bd[m] = 1; for (uint64_t i = 0; i < bn; ++i) bc[m][i] = 0; continue; }
+            uint64_t mx = 0;
+            for (uint64_t i = 0; i < bn; ++i) {
+                bc[m][i] = cidx[m][start + i];
+                if (bc[m][i] > mx) mx = bc[m][i];
+            }
+            bd[m] = mx + 1 > prev_dim[m] ? mx + 1 : prev_dim[m];
+            prev_dim[m] = bd[m];
+        }
+        for (uint64_t i = 0; i < bn; ++i) bv[i] = vals[start + i];
+        ptr = end;
+
+        /* model creation / growth (streaming_cpd.cpp:106-128) */
+        if (slices == 0) {
+            for (int m = 0; m < nmode; ++m) {
+                memcpy(U[m], init_factors[m], sizeof(double) * bd[m] * R);
+                memset(P[m], 0, sizeof(double) * maxd[m] * R);
+            }
+            memset(U[s], 0, sizeof(double) * R);
+            for (int m = 0; m < nmode; ++m) orc_gram_upper(bd[m], R, U[m], G[m]);
+            memset(G[s], 0, sizeof(double) * RR);
+        } else {
+            for (int m = 0; m < nmode; ++m)
+                if (bd[m] > cur[m]) {
+                    memcpy(U[m] + cur[m] * R, init_factors[m] + cur[m] * R,
+                           sizeof(double) * (bd[m] - cur[m]) * R);
+                    memset(P[m] + cur[m] * R, 0, sizeof(double) * (bd[m] - cur[m]) * R);
+                }
+            for (int m = 0; m < nmode; ++m)
+                if (m != s) orc_gram_upper(bd[m], R, U[m], G[m]);
+        }
+        for (int m = 0; m < nmode; ++m) cur[m] = bd[m];
+
+        /* spcpstream_alto_iter: first slice normalisation (585-598) */
+        if (slices == 0) {
+            for (int m = 0; m < nmode; ++m) {
+                if (m == s) continue;
+                orc_kruskal_norm(bd[m], R, U[m], lambda, 0);
+                orc_gram_upper(bd[m], R, U[m], G[m]);
+            }
+        }
+        /* nonzero_slices and the row-sparse matrices (607-641) */
+        for (int m = 0; m < nmode; ++m) {
+            for (uint64_t i = 0; i < bd[m]; ++i) ridx[m][i] = -1;
+            for (uint64_t e = 0; e < bn; ++e) ridx[m][bc[m][e]] = 0;
+            uint64_t k = 0;
+            for (uint64_t i = 0; i < bd[m]; ++i)
+                if (ridx[m][i] == 0) { nzr[m][k] = i; ridx[m][i] = (int64_t)k; ++k; }
+                else ridx[m][i] = -1;
+            nnzr[m] = k;
+            for (uint64_t r = 0; r < k; ++r) {
+                memcpy(Anzp[m] + r * R, P[m] + nzr[m][r] * R, sizeof(double) * R);
+                memcpy(Anz[m] + r * R, U[m] + nzr[m][r] * R, sizeof(double) * R);
+            }
+            orc_gram_upper(bd[m], R, U[m], c[m]);
+            tall_atb(bd[m], R, P[m], U[m], h[m]);
+        }
+        if (slices > 0) {  /* 644-656 */
+            for (int m = 0; m < nmode; ++m) {
+                if (m == s) continue;
+                tall_atb(nnzr[m], R, Anzp[m], Anzp[m], cnzprev[m]);
+                for (size_t q = 0; q < RR; ++q) czprev[m][q] = cprev[m][q] - cnzprev[m][q];
+            }
+        }
+        /* update_salto_rowsparse: compact coordinates (alto.hpp:301-434) */
+        uint64_t cdims[ORC_MAX_MODES];
+        for (int m = 0; m < nmode; ++m) {
+            cdims[m] = nnzr[m];
+            for (uint64_t e = 0; e < bn; ++e) cc[m][e] = (uint64_t)ridx[m][bc[m][e]];
+        }
+        int total = 0;
+        for (int m = 0; m < nmode; ++m) total += orc_num_bits(cdims[m]);
+        int kw = total > 64 ? 2 : 1;
+        uint64_t masks[ORC_MAX_MODES * 2];
+        orc_alto_masks(nmode, cdims, kw, masks);
+        orc_linearize(nmode, bn, (const uint64_t *const *)cc, masks, kw, keys);
+        double *sv = (double *)malloc(sizeof(double) * bn);
+        memcpy(sv, bv, sizeof(double) * bn);
+        orc_sort_alto(bn, kw, keys, sv, NULL);
+        int nprtn = orc_get_num_ptrn(bn, g_threads);
+        uint64_t *pp = (uint64_t *)malloc(sizeof(uint64_t) * (nprtn + 1));
+        uint64_t *iv = (uint64_t *)malloc(sizeof(uint64_t) * nprtn * nmode * 2);
+        orc_prtn_alto(nmode, cdims, bn, kw, keys, masks, nprtn, pp, iv);
+
+        double delta = 0.0, prev_delta = 0.0;
+        int outer = 0;
+        for (int i = 0; i < max_iters; ++i) {
+            delta = 0.0;
+            /* s_t (712-742) */
+            memset(Anz[s], 0, sizeof(double) * nnzr[s] * R);
+            orc_mttkrp_alto(nmode, cdims, bn, kw, keys, sv, masks, nprtn, pp, iv, s, Anz, R, s,
+                            Anz[s]);
+            hadamard_excl(nmode, R, c, s, Phi);
+            for (int q = 0; q < R; ++q) Phi[q * R + q] += 1e-12;
+            if (chol_upper(R, Phi) == 0)
+                for (uint64_t r = 0; r < nnzr[s]; ++r) chol_solve_row(R, Phi, Anz[s] + r * R);
+            /* temporal Gram (744-790) */
+            for (int a = 1; a < R; ++a)
+                for (int b = 0; b < a; ++b) G[s][b + a * R] = G[s][a + b * R];
+            memcpy(old_gram, G[s], sizeof(double) * RR);
+            for (uint64_t r = 0; r < bd[s]; ++r)
+                for (int n2 = 0; n2 < R; ++n2)
+                    for (int m2 = 0; m2 < R; ++m2)
+                        G[s][m2 + n2 * R] += Anz[s][r * R + m2] * Anz[s][r * R + n2];
+            memcpy(h[s], old_gram, sizeof(double) * RR);
+            memcpy(c[s], G[s], sizeof(double) * RR);
+            for (int m = 0; m < nmode; ++m) {  /* 792-970 */
+                if (m == s) continue;
+                hadamard_excl(nmode, R, h, m, Q[m]);
+                hadamard_excl(nmode, R, c, m, Phi);
+                for (int q = 0; q < R; ++q) Phi[q * R + q] += 1e-12;
+                memset(Anz[m], 0, sizeof(double) * nnzr[m] * R);
+                orc_mttkrp_alto(nmode, cdims, bn, kw, keys, sv, masks, nprtn, pp, iv, s, Anz, R, m,
+                                Anz[m]);
+                for (uint64_t r = 0; r < nnzr[m]; ++r) {  /* rsp_mat_mul + rsp_mat_add */
+                    for (int b = 0; b < R; ++b) {
+                        double t = 0.0;
+                        for (int a = 0; a < R; ++a) t += Anzp[m][r * R + a] * Q[m][a * R + b];
+                        tmp[b] = t;
+                    }
+                    for (int b = 0; b < R; ++b) Anz[m][r * R + b] += tmp[b];
+                }
+                int ok = chol_upper(R, Phi) == 0;
+                if (ok)
+                    for (uint64_t r = 0; r < nnzr[m]; ++r) chol_solve_row(R, Phi, Anz[m] + r * R);
+                tall_atb(nnzr[m], R, Anz[m], Anz[m], cnz[m]);
+                tall_atb(nnzr[m], R, Anzp[m], Anz[m], hnz[m]);
+                memcpy(QPi[m], Q[m], sizeof(double) * RR);
+                if (ok)
+                    for (int r = 0; r < R; ++r) chol_solve_row(R, Phi, QPi[m] + (size_t)r * R);
+                rr_matmul(R, czprev[m], 0, QPi[m], hz[m]);
+                rr_matmul(R, QPi[m], 1, hz[m], cz[m]);
+                double trc = 0.0, trh = 0.0, trcp = 0.0;
+                for (size_t q = 0; q < RR; ++q) {
+                    c[m][q] = cnz[m][q] + cz[m][q];
+                    h[m][q] = hnz[m][q] + hz[m][q];
+                }
+                for (int q = 0; q < R; ++q) {
+                    trc += c[m][q * R + q];
+                    trh += h[m][q * R + q];
+                    trcp += cprev[m][q * R + q];
+                }
+                delta += sqrt(fabs((trc + trcp - 2.0 * trh) / (trc + 1e-12)));
+            }
+            outer = i + 1;
+            if (i > 0 && fabs(prev_delta - delta) < eps) break;
+            prev_delta = delta;
+        }
+        /* z rows <- prevU QPi, nz rows <- A_nz (1017-1045) */
+        for (int m = 0; m < nmode; ++m) {
+            for (uint64_t i = 0; i < bd[m]; ++i) {
+                if (ridx[m][i] >= 0) continue;
+                for (int b = 0; b < R; ++b) {
+                    double t = 0.0;
+                    for (int a = 0; a < R; ++a) t += P[m][i * R + a] * QPi[m][a * R + b];
+                    tmp[b] = t;
+                }
+                memcpy(U[m] + i * R, tmp, sizeof(double) * R);
+            }
+            for (uint64_t r = 0; r < nnzr[m]; ++r)
+                memcpy(U[m] + nzr[m][r] * R, Anz[m] + r * R, sizeof(double) * R);
+        }
+        for (size_t q = 0; q < RR; ++q) G[s][q] *= 0.99;  /* 1051-1058 */
+        for (int m = 0; m < nmode; ++m) {
+            if (m == s) continue;
+            memcpy(cprev[m], c[m], sizeof(double) * RR);
+            memcpy(cnzprev[m], cnz[m], sizeof(double) * RR);
+        }
+        free(pp); free(iv); free(sv);
+        if (out_time_rows) memcpy(out_time_rows + (size_t)slices * R, U[s], sizeof(double) * R);
+        if (out_outer_iters) out_outer_iters[slices] = outer;
+        for (int m = 0; m < nmode; ++m) memcpy(P[m], U[m], sizeof(double) * bd[m] * R);
+        ++slices;
+    }
+    for (int m = 0; m < nmode; ++m) {
+        if (out_dims) out_dims[m] = cur[m];
+        if (out_factors && out_factors[m]) memcpy(out_factors[m], U[m], sizeof(double) * cur[m] * R);
+        if (out_grams) memcpy(out_grams + (size_t)m * RR, G[m], sizeof(double) * RR);
+    }
+    for (int m = 0; m < nmode; ++m) {
+        free(U[m]); free(P[m]); free(Anz[m]); free(Anzp[m]); free(ridx[m]); free(nzr[m]);
+        free(cc[m]); free(bc[m]); free(G[m]); free(c[m]); free(h[m]); free(cnz[m]); free(hnz[m]);
+        free(cz[m]); free(hz[m]); free(cprev[m]); free(czprev[m]); free(cnzprev[m]); free(Q[m]);
+        free(QPi[m]); free(cidx[m]);
+    }
+    free(vals); free(Phi); free(old_gram); free(tmp); free(bv); free(keys);
+    return slices;
+}

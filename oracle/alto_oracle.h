@@ -81,6 +81,12 @@ int orc_cpstream(int nmode, const uint64_t *dims, uint64_t nnz,
                  double *const *out_factors, uint64_t *out_dims,
                  double *out_time_rows, int *out_outer_iters,
                  double *out_admm_iters, double *out_grams);
+int orc_spcpstream(int nmode, const uint64_t *dims, uint64_t nnz,
+                   const uint64_t *const *cidx, const double *vals,
+                   int stream_mode, int rank, int max_iters, double eps, int max_slices,
+                   double *const *init_factors, double *const *out_factors,
+                   uint64_t *out_dims, double *out_time_rows, int *out_outer_iters,
+                   double *out_grams);
 
 #ifdef __cplusplus
 }

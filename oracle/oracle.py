@@ -289,6 +289,33 @@ def cpstream(dims, cidx, vals, stream_mode, rank, max_iters, eps, nonneg, init_f
                 admm_iters=admm[:n], grams=grams, slices=n)
 
 
+def spcpstream(dims, cidx, vals, stream_mode, rank, max_iters, eps, init_factors, max_slices=0):
+    """orc_spcpstream: cpstream with use_spcpstream (src/streaming_cpd.cpp:26-366)
+    driving spcpstream_alto_iter (streaming_cpd.cpp:466-1085).  Same outputs as
+    cpstream, without the ADMM counts."""
+    dims = np.ascontiguousarray(dims, dtype=np.uint64)
+    N = len(dims)
+    cidx = [np.ascontiguousarray(c, dtype=np.uint64) for c in cidx]
+    vals = np.ascontiguousarray(vals, dtype=np.float64)
+    init = [np.ascontiguousarray(f, dtype=np.float64) for f in init_factors]
+    nslices_max = int(dims[stream_mode])
+    outs = [np.zeros((int(dims[m]) if m != stream_mode else 1, rank)) for m in range(N)]
+    out_dims = np.zeros(N, dtype=np.uint64)
+    trows = np.zeros((nslices_max, rank))
+    outer = np.zeros(nslices_max, dtype=np.int32)
+    grams = np.zeros((N, rank, rank))
+    n = lib().orc_spcpstream(
+        ctypes.c_int(N), _p64(dims), ctypes.c_uint64(len(vals)), _ptr_array(cidx, "u"), _pf(vals),
+        ctypes.c_int(stream_mode), ctypes.c_int(rank), ctypes.c_int(max_iters), ctypes.c_double(eps),
+        ctypes.c_int(max_slices), _ptr_array(init, "f"), _ptr_array(outs, "f"), _p64(out_dims), _pf(trows),
+        outer.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _pf(grams))
+    if n < 0:
+        raise ValueError("orc_spcpstream: bad arguments")
+    outs = [o[: int(out_dims[m])] for m, o in enumerate(outs)]
+    return dict(factors=outs, dims=out_dims, time_rows=trows[:n], outer_iters=outer[:n], grams=grams,
+                slices=n)
+
+
 def cpd_error(cidx, vals, factors):
     """cpd_error (src/streaming_cpd.cpp:368-464) with kruskal_norm
     (src/kruskal_model.cpp:400-442), weights 1: the mode with the fewest

@@ -99,3 +99,25 @@ def test_cli_cpstream(gpu, tmp_path):
                         "-i", str(p)])
     assert rc == 0, err
     assert "Total CPSTREAM time" in out and "(4 slices" in out
+
+
+@pytest.mark.gpu
+def test_cli_spcpstream(gpu, tmp_path):
+    """-l spcpstream_alto runs spCP-stream (main.cpp:163, 397); the constraint
+    flag is accepted and unused, as in the reference."""
+    rng = np.random.default_rng(6)
+    T, nnz = 4, 3000
+    t = rng.integers(0, T, size=nnz)
+    t[:T] = np.arange(T)
+    i = rng.integers(0, 80, size=nnz)
+    j = rng.integers(0, 60, size=nnz)
+    v = rng.poisson(3, size=nnz) + 1
+    p = tmp_path / "s.tns"
+    with open(p, "w") as f:
+        for a, b, c, d in zip(t, i, j, v):
+            f.write("%d %d %d %d\n" % (a + 1, b + 1, c + 1, d))
+    rc, out, err = run(["-l", "spcpstream_alto", "-a", "0", "-z", "nonneg", "-r", "4", "-m", "3", "-e", "1e-6",
+                        "-i", str(p)])
+    assert rc == 0, err
+    assert "spCPSTREAM" in err and "Total CPSTREAM time" in out and "(4 slices" in out
+    assert "final fit error" in out

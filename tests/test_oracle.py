@@ -327,6 +327,118 @@ def test_cpstream_batching_and_invariants():
     np.testing.assert_allclose(G[0], G[0].T, rtol=1e-12)
 
 
+def _stream_case(seed, T=5, I=14, J=11, nnz=260, R=3, distinct=True):
+    """A 3-mode stream (mode 0 = time), every slice non-empty; distinct
+    coordinates unless asked otherwise."""
+    rng = np.random.default_rng(seed)
+    if distinct:
+        lin = rng.choice(T * I * J, size=nnz, replace=False)
+        t, i, j = np.unravel_index(lin, (T, I, J))
+    else:
+        t, i, j = rng.integers(0, T, nnz), rng.integers(0, I, nnz), rng.integers(0, J, nnz)
+    t = t.copy()
+    t[:T] = np.arange(T)
+    cidx = [t.astype(np.uint64), i.astype(np.uint64), j.astype(np.uint64)]
+    vals = rng.uniform(0.5, 2.0, nnz)
+    init = [rng.random((T, R)), rng.random((I, R)), rng.random((J, R))]
+    return [T, I, J], cidx, vals, init, R
+
+
+def _np_cpstream_unconstrained(dims, cidx, vals, R, iters, init):
+    """Dense numpy restatement of CP-stream without constraints, stream mode 0
+    (cpstream, streaming_cpd.cpp:26-366; cpstream_alto_iter 1198-1519): per
+    slice s_t = MTTKRP_t (Had_{m!=t} G_m + 1e-12 I)^-1, G_t += s_t^T s_t, then
+    per mode U_j = (MTTKRP_j + P_j (G_t,old Had_{m!=j,t} P_m^T U_m))
+    (Had_{m!=j} G_m + 1e-12 I)^-1; G_t *= 0.99 after the slice."""
+    c2, v2 = O.stream_relabel(cidx, vals, 0)
+    T = int(c2[0].max()) + 1
+    I, J = int(c2[1].max()) + 1, int(c2[2].max()) + 1
+    U = [None] * 3
+    P = [None] * 3
+    G = [None] * 3
+    prev = [1, 0, 0]
+    trows = []
+    eye = np.eye(R)
+    for t in range(T):
+        sel = c2[0] == t
+        bi, bj, bv = c2[1][sel].astype(np.int64), c2[2][sel].astype(np.int64), v2[sel]
+        bd = [1, max(int(bi.max()) + 1, prev[1]), max(int(bj.max()) + 1, prev[2])]
+        prev = bd
+        X = np.zeros((bd[1], bd[2]))
+        np.add.at(X, (bi, bj), bv)
+        if t == 0:
+            U = [np.zeros((1, R)), init[1][: bd[1]].copy(), init[2][: bd[2]].copy()]
+            P = [np.zeros((1, R)), np.zeros((bd[1], R)), np.zeros((bd[2], R))]
+            for m in (1, 2):
+                lam = np.sqrt(np.maximum((U[m] ** 2).sum(0), 1e-12))
+                U[m] /= lam
+            G = [np.zeros((R, R)), U[1].T @ U[1], U[2].T @ U[2]]
+        else:
+            for m in (1, 2):
+                old = U[m].shape[0]
+                if bd[m] > old:
+                    U[m] = np.vstack([U[m], init[m][old: bd[m]]])
+                    P[m] = np.vstack([P[m], np.zeros((bd[m] - old, R))])
+                G[m] = U[m].T @ U[m]
+        for _ in range(iters):
+            st = (U[1] * (X @ U[2])).sum(0)
+            st = np.linalg.solve(G[1] * G[2] + 1e-12 * eye, st)
+            U[0] = st[None, :]
+            Gold = G[0].copy()
+            G[0] = G[0] + np.outer(st, st)
+            for j in (1, 2):
+                k = 3 - j
+                mt = (X @ U[2]) * st if j == 1 else (X.T @ U[1]) * st
+                ata = Gold * (P[k].T @ U[k])
+                rhs = mt + P[j] @ ata
+                U[j] = np.linalg.solve((G[0] * G[k] + 1e-12 * eye).T, rhs.T).T
+                G[j] = U[j].T @ U[j]
+        G[0] *= 0.99
+        P = [u.copy() for u in U]
+        trows.append(st.copy())
+    return U, np.array(trows)
+
+
+def test_cpstream_unconstrained_oracle_matches_dense_restatement():
+    """The oracle's CP-stream without constraints (ALTO MTTKRP, Cholesky
+    solves, upper-triangle Grams) equals a dense numpy restatement of the
+    same updates for a fixed number of outer iterations (epsilon 0)."""
+    dims, cidx, vals, init, R = _stream_case(31)
+    iters = 3
+    res = O.cpstream(dims, cidx, vals, 0, R, iters, 0.0, False, init)
+    U, trows = _np_cpstream_unconstrained(dims, cidx, vals, R, iters, init)
+    assert res["slices"] == dims[0] and np.all(res["outer_iters"] == iters)
+    np.testing.assert_allclose(res["time_rows"], trows, rtol=1e-9, atol=1e-12)
+    for m in (1, 2):
+        np.testing.assert_allclose(res["factors"][m], U[m], rtol=1e-9, atol=1e-11)
+
+
+@pytest.mark.parametrize("seed,distinct", [(32, True), (33, False)])
+def test_spcpstream_oracle_equals_unconstrained_cpstream(seed, distinct):
+    """spCP-stream is CP-stream without constraints with the rows absent from
+    a slice updated implicitly: c = c_nz + c_z and h = h_nz + h_z track
+    U^T U and prevU^T U exactly, and the z rows end as prevU Q Phi^-1, the
+    value the full update gives them.  With epsilon 0 both run every outer
+    iteration, so factors, time rows and the temporal Gram agree to rounding."""
+    dims, cidx, vals, init, R = _stream_case(seed, T=6, I=20, J=15, nnz=300, R=4, distinct=distinct)
+    iters = 4
+    sp = O.spcpstream(dims, cidx, vals, 0, R, iters, 0.0, init)
+    cs = O.cpstream(dims, cidx, vals, 0, R, iters, 0.0, False, init)
+    assert sp["slices"] == cs["slices"] == dims[0]
+    assert np.all(sp["outer_iters"] == iters)
+    np.testing.assert_array_equal(sp["dims"], cs["dims"])
+    np.testing.assert_allclose(sp["time_rows"], cs["time_rows"], rtol=1e-8, atol=1e-11)
+    for m in (1, 2):
+        np.testing.assert_allclose(sp["factors"][m], cs["factors"][m], rtol=1e-8, atol=1e-10)
+    np.testing.assert_allclose(sp["grams"][0], cs["grams"][0], rtol=1e-8, atol=1e-10)
+
+
+def test_spcpstream_oracle_rejects_zero_iterations():
+    dims, cidx, vals, init, R = _stream_case(34)
+    with pytest.raises(ValueError):
+        O.spcpstream(dims, cidx, vals, 0, R, 0, 0.0, init)
+
+
 def _rowsparse_case(dims, nnz, R, seed, stream_mode=-1, absent=0.0):
     rng = np.random.default_rng(seed)
     cidx = [rng.integers(0, d, size=nnz).astype(np.uint64) for d in dims]
