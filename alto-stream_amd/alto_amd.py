@@ -443,12 +443,14 @@ def cpd_error(dims, cidx, vals, factors, stream=None):
     return err.value, tuple(parts)
 
 
-def gather_probe(table_bytes, gathers=1 << 26, seed=1, stream=None):
-    """ms per launch of the random 128-byte row gather probe over a table of
-    table_bytes (power of two)."""
+def gather_probe(rows, gathers=1 << 25, dist=0, zipf_s=1.1, layout=0, seed=1, stream=None):
+    """ms per launch of the 128-byte row gather probe over a table of `rows`
+    rows: dist 0 uniform, 1 Zipf(zipf_s); layout 0 = 8 lanes x 16 B, 1 = 4
+    lanes x 2 x 16 B per row."""
     ms = ctypes.c_float()
-    check(lib().alto_hip_gather_probe(ctypes.c_uint64(table_bytes), ctypes.c_uint64(gathers),
-                                      ctypes.c_uint64(seed), ctypes.byref(ms), _vp(stream)), "gather_probe")
+    check(lib().alto_hip_gather_probe(ctypes.c_uint64(rows), ctypes.c_uint64(gathers), ctypes.c_int(dist),
+                                      ctypes.c_double(zipf_s), ctypes.c_int(layout), ctypes.c_uint64(seed),
+                                      ctypes.byref(ms), _vp(stream)), "gather_probe")
     return ms.value
 
 

@@ -91,26 +91,54 @@ __global__ __launch_bounds__(256) void k_generate(GenArgs a, uint64_t **cidx, do
     vals[i] = v;
 }
 
-// Random-row gather rate: each group of 8 lanes reads `per` rows of 128
-// bytes (16 bytes per lane) at uniformly random row indices of a table of
-// `rows` rows (a power of two), as the MTTKRP gathers factor rows.
-__global__ __launch_bounds__(256) void k_gather_probe(const float4 *__restrict__ tab, uint64_t rows,
-                                                      uint64_t groups, int per, uint64_t seed,
-                                                      float *__restrict__ out) {
-    const uint64_t g = ((uint64_t)blockIdx.x * 256 + threadIdx.x) / 8;
-    const int l = threadIdx.x & 7;
-    if (g >= groups) return;
+// Row-gather rate: each lane group reads one 128-byte row per step at the
+// row the next entry of `idx` names (a coalesced 4-byte stream, as the
+// schedule's keys are).  LG = 8: 8 lanes x 16 B; LG = 4: 4 lanes x 2 x 16 B,
+// the fiber kernel's FP32 rank-32 row.
+template <int LG>
+__global__ __launch_bounds__(256) void k_gather_probe(const float4 *__restrict__ tab,
+                                                      const uint32_t *__restrict__ idx, uint64_t n,
+                                                      int per, float *__restrict__ out) {
+    constexpr int NL = 8 / LG;
+    const uint64_t g = ((uint64_t)blockIdx.x * 256 + threadIdx.x) / LG;
+    const int l = threadIdx.x % LG;
     float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll 4
     for (int k = 0; k < per; ++k) {
-        const uint64_t r = mix64(seed + g * per + k) & (rows - 1);
-        const float4 v = tab[r * 8 + l];
-        a.x += v.x;
-        a.y += v.y;
-        a.z += v.z;
-        a.w += v.w;
+        const uint64_t i = g * per + k;
+        if (i >= n) break;
+        const float4 *p = tab + (uint64_t)idx[i] * 8 + l * NL;
+#pragma unroll
+        for (int j = 0; j < NL; ++j) {
+            const float4 v = p[j];
+            a.x += v.x;
+            a.y += v.y;
+            a.z += v.z;
+            a.w += v.w;
+        }
     }
-    out[g * 8 + l] = a.x + a.y + a.z + a.w;
+    out[g * LG + l] = a.x + a.y + a.z + a.w;
+}
+
+// Row indices of the probe: uniform over `rows`, or Zipf by the cumulative
+// table cdf (rows entries) mapped through perm.
+__global__ __launch_bounds__(256) void k_probe_idx(uint32_t *__restrict__ idx, uint64_t n, uint64_t rows,
+                                                   const double *__restrict__ cdf,
+                                                   const uint32_t *__restrict__ perm, uint64_t seed) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t r = rnd(seed, 7, i);
+    if (!cdf) {
+        idx[i] = (uint32_t)__umul64hi(r, rows);
+        return;
+    }
+    const double u = u01(r);
+    uint64_t lo = 0, hi = rows - 1;  // first k with cdf[k] > u
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (cdf[mid] > u) hi = mid;
+        else lo = mid + 1;
+    }
+    idx[i] = perm[lo];
 }
 
 }  // namespace alto
@@ -195,32 +223,60 @@ extern "C" int alto_hip_generate(int nmode, const uint64_t *dims, uint64_t nnz, 
     return ALTO_OK;
 }
 
-extern "C" int alto_hip_gather_probe(uint64_t table_bytes, uint64_t gathers, uint64_t seed,
-                                     float *ms, void *stream) {
+extern "C" int alto_hip_gather_probe(uint64_t rows, uint64_t gathers, int dist, double zipf_s,
+                                     int layout, uint64_t seed, float *ms, void *stream) {
     using namespace alto;
-    ALTO_TRY(ensure_init());
-    if (!ms || table_bytes < 128 || gathers < 64 || (table_bytes & (table_bytes - 1)))
+    if (!ms || rows < 1 || rows >= (1ULL << 32) || gathers < 64 || (dist != 0 && dist != 1) ||
+        (layout != 0 && layout != 1) || (dist == 1 && !(zipf_s > 0.0)))
         return ALTO_ERR_ARG;
+    ALTO_TRY(ensure_init());
     hipStream_t s = to_stream(stream);
-    const uint64_t rows = table_bytes / 128;
     const int per = 64;
-    const uint64_t groups = gathers / per;
-    DevBuf tab, out;
-    ALTO_TRY(tab.alloc(table_bytes));
-    ALTO_TRY(out.alloc(sizeof(float) * 8 * groups));
-    ALTO_HIP_TRY(hipMemsetAsync(tab.p, 0, table_bytes, s));
-    const unsigned blocks = cdiv(groups * 8, 256);
+    const uint64_t groups = gathers / per, n = groups * per;
+    const int LG = layout == 0 ? 8 : 4;
+    const unsigned blocks = cdiv(groups * LG, 256);
+    DevBuf tab, idx, out, cdf, perm;
+    ALTO_TRY(tab.alloc(rows * 128));
+    ALTO_TRY(idx.alloc(sizeof(uint32_t) * n));
+    ALTO_TRY(out.alloc(sizeof(float) * (size_t)blocks * 256));
+    ALTO_HIP_TRY(hipMemsetAsync(tab.p, 0, rows * 128, s));
+    if (dist == 1) {
+        std::vector<double> c(rows);
+        double acc = 0.0;
+        for (uint64_t k = 0; k < rows; ++k) {
+            acc += pow((double)(k + 1), -zipf_s);
+            c[k] = acc;
+        }
+        for (auto &v : c) v /= acc;
+        c.back() = 2.0;
+        std::vector<uint32_t> pm(rows);
+        std::iota(pm.begin(), pm.end(), 0u);
+        for (uint64_t k = rows - 1; k > 0; --k) std::swap(pm[k], pm[rnd(seed, 301, k) % (k + 1)]);
+        ALTO_TRY(cdf.alloc(sizeof(double) * rows));
+        ALTO_TRY(perm.alloc(sizeof(uint32_t) * rows));
+        ALTO_HIP_TRY(hipMemcpyAsync(cdf.p, c.data(), sizeof(double) * rows, hipMemcpyHostToDevice, s));
+        ALTO_HIP_TRY(hipMemcpyAsync(perm.p, pm.data(), sizeof(uint32_t) * rows, hipMemcpyHostToDevice, s));
+        ALTO_HIP_TRY(hipStreamSynchronize(s));
+    }
+    k_probe_idx<<<cdiv(n, 256), 256, 0, s>>>(idx.as<uint32_t>(), n, rows, dist ? cdf.as<double>() : nullptr,
+                                             dist ? perm.as<uint32_t>() : nullptr, seed);
+    ALTO_LAUNCH_CHECK();
+    auto launch = [&]() {
+        if (LG == 8)
+            k_gather_probe<8><<<blocks, 256, 0, s>>>(tab.as<float4>(), idx.as<uint32_t>(), n, per, out.as<float>());
+        else
+            k_gather_probe<4><<<blocks, 256, 0, s>>>(tab.as<float4>(), idx.as<uint32_t>(), n, per, out.as<float>());
+    };
     hipEvent_t e0, e1;
     ALTO_HIP_TRY(hipEventCreate(&e0));
     ALTO_HIP_TRY(hipEventCreate(&e1));
-    k_gather_probe<<<blocks, 256, 0, s>>>(tab.as<float4>(), rows, groups, per, seed, out.as<float>());
+    launch();
     ALTO_HIP_TRY(hipEventRecord(e0, s));
     const int reps = 5;
-    for (int r = 0; r < reps; ++r)
-        k_gather_probe<<<blocks, 256, 0, s>>>(tab.as<float4>(), rows, groups, per, seed + r + 1,
-                                              out.as<float>());
+    for (int r = 0; r < reps; ++r) launch();
     ALTO_HIP_TRY(hipEventRecord(e1, s));
     ALTO_HIP_TRY(hipEventSynchronize(e1));
+    ALTO_LAUNCH_CHECK();
     float t = 0.f;
     ALTO_HIP_TRY(hipEventElapsedTime(&t, e0, e1));
     (void)hipEventDestroy(e0);

@@ -83,11 +83,17 @@ def main():
     A.init(0)
     dev = A.device_name()
     if args.probe:
-        for lg in range(21, 32, 2):
-            g = 1 << 26
-            ms = A.gather_probe(1 << lg, g)
-            print(json.dumps({"table_MiB": (1 << lg) >> 20, "ms": round(ms, 4), "G_rows_per_s": round(g / ms / 1e6, 2),
-                              "GB_per_s": round(g * 128 / ms / 1e6, 1)}), flush=True)
+        # 128-byte row gathers: uniform rows over growing tables (L2, MALL, HBM),
+        # and Zipf(1.1) rows as the workload's modes draw them, in both lane layouts
+        g = 1 << 25
+        for dist in (0, 1):
+            for rows in (16384, 65536, 200000, 500000, 1 << 20, 1 << 24):
+                for layout in (0, 1):
+                    ms = A.gather_probe(rows, g, dist=dist, zipf_s=1.1, layout=layout)
+                    print(json.dumps({"rows": rows, "table_MiB": round(rows * 128 / 2 ** 20, 1),
+                                      "dist": "zipf1.1" if dist else "uniform",
+                                      "layout": "8x16B" if layout == 0 else "4x2x16B", "ms": round(ms, 4),
+                                      "G_rows_per_s": round(g / ms / 1e6, 2)}), flush=True)
         return
     W = workload(args.workload)
     dims, nnz, R = W["dims"], W["nnz"], W["rank"]

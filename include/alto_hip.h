@@ -264,11 +264,14 @@ int alto_hip_generate(int nmode, const uint64_t *dims, uint64_t nnz, uint64_t se
                       uint64_t *const *cidx_dev, double *vals_dev, void *stream);
 
 /* ---- gather-rate probe (no reference counterpart: roofline calibration) ----
- * Average ms of one launch gathering `gathers` 128-byte rows at uniformly
- * random positions of a device table of table_bytes (a power of two), the
- * access pattern of the MTTKRP's factor-row gathers. */
-int alto_hip_gather_probe(uint64_t table_bytes, uint64_t gathers, uint64_t seed, float *ms,
-                          void *stream);
+ * Average ms of one launch gathering `gathers` 128-byte rows of a device
+ * table of `rows` rows, the access pattern of the MTTKRP's factor-row
+ * gathers: row indices from a coalesced 4-byte stream, uniform (dist 0) or
+ * Zipf(zipf_s) over a seeded permutation (dist 1); layout 0 = 8 lanes x
+ * 16 B per row, layout 1 = 4 lanes x 2 x 16 B (the fiber kernel's FP32
+ * rank-32 row). */
+int alto_hip_gather_probe(uint64_t rows, uint64_t gathers, int dist, double zipf_s, int layout,
+                          uint64_t seed, float *ms, void *stream);
 
 #ifdef __cplusplus
 }
