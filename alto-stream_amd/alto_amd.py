@@ -359,6 +359,29 @@ def cpstream(dims, cidx, vals, stream_mode, rank, max_iters, eps, nonneg, init_f
                 admm_iters=admm[:n], grams=grams, slices=n, timings=tim, final_error=float(ferr[0]))
 
 
+def stream_prepare(dims, cidx, vals, stream_mode, stream=None):
+    """StreamingSparseTensor on the device (streaming_sptensor.cpp:69-264):
+    host COO in; returns (cidx, vals, slice_ptr, slice_dims) of the sorted,
+    relabelled stream."""
+    c, keep = _coo(dims, cidx, vals, False)
+    N = len(dims)
+    n = len(vals)
+    outs = [np.zeros(n, dtype=np.uint64) for _ in range(N)]
+    oarr = (_u64p * N)()
+    for i, o in enumerate(outs):
+        oarr[i] = o.ctypes.data_as(_u64p)
+    ov = np.zeros(n)
+    ns = ctypes.c_uint64()
+    cap = max(n, 1)
+    ptr = np.zeros(cap + 1, dtype=np.uint64)
+    sd = np.zeros((cap, N), dtype=np.uint64)
+    check(lib().alto_hip_stream_prepare(ctypes.byref(c), ctypes.c_int(stream_mode), oarr, ov.ctypes.data_as(_f64p),
+                                        ctypes.c_uint64(cap), ctypes.byref(ns), ptr.ctypes.data_as(_u64p),
+                                        sd.ctypes.data_as(_u64p), _vp(stream)), "stream_prepare")
+    T = int(ns.value)
+    return outs, ov, ptr[: T + 1], sd[:T]
+
+
 def spcpstream(dims, cidx, vals, stream_mode, rank, max_iters, eps, init_factors, max_slices=0,
                prec=PREC_F64, stream=None):
     """spCP-stream (cpstream with use_spcpstream, src/streaming_cpd.cpp:26-366,

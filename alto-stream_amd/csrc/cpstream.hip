@@ -275,77 +275,6 @@ __global__ void k_zero_u64(uint64_t *p, uint64_t n) {
     if (i < n) p[i] = 0;
 }
 
-int stream_prepare(const alto_hip_coo *coo, int s, StreamInput *out) {
-    const int N = coo->nmode;
-    const uint64_t nnz = coo->nnz;
-    std::vector<uint32_t> order(nnz);
-    std::iota(order.begin(), order.end(), 0u);
-    int lex[kMaxModes];
-    for (int t = 0; t < N; ++t) lex[t] = (s + t) % N;
-    for (uint64_t i = 0; i < nnz; ++i)
-        for (int m = 0; m < N; ++m)
-            if (coo->cidx[m][i] >= coo->dims[m]) {
-                set_error("a coordinate is not below its mode size");
-                return ALTO_ERR_ARG;
-            }
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
-        for (int t = 0; t < N; ++t) {
-            uint64_t x = coo->cidx[lex[t]][a], y = coo->cidx[lex[t]][b];
-            if (x != y) return x < y;
-        }
-        return false;
-    });
-    auto &cid = out->cid;
-    cid.assign(N, std::vector<uint64_t>(nnz));
-    out->vals.resize(nnz);
-    for (uint64_t i = 0; i < nnz; ++i) {
-        for (int m = 0; m < N; ++m) cid[m][i] = coo->cidx[m][order[i]];
-        out->vals[i] = coo->vals[order[i]];
-    }
-    order.clear();
-    order.shrink_to_fit();
-    for (int m = 0; m < N; ++m) {
-        if (m == s) continue;
-        std::vector<uint64_t> perm(coo->dims[m], UINT64_MAX);
-        uint64_t seen = 0;
-        for (uint64_t i = 0; i < nnz; ++i) {
-            uint64_t v = cid[m][i];
-            if (perm[v] == UINT64_MAX) perm[v] = seen++;
-            cid[m][i] = perm[v];
-        }
-    }
-    // slices (next_dynamic_batch with timeslice_limit 1)
-    out->slices.clear();
-    uint64_t ptr = 0, batch = 0, prev[kMaxModes] = {0};
-    while (ptr < nnz) {
-        uint64_t a = ptr;
-        while (a < nnz && cid[s][a] < batch) ++a;
-        uint64_t b = a;
-        while (b < nnz && cid[s][b] < batch + 1) ++b;
-        ++batch;
-        if (b == a) {
-            set_error("time slice %llu has no nonzeros", (unsigned long long)(batch - 1));
-            return ALTO_ERR_EMPTY_SLICE;
-        }
-        Slice sl;
-        sl.start = a;
-        sl.end = b;
-        for (int m = 0; m < N; ++m) {
-            if (m == s) {
-                sl.dims[m] = 1;
-                continue;
-            }
-            uint64_t mx = 0;
-            for (uint64_t i = a; i < b; ++i) mx = cid[m][i] > mx ? cid[m][i] : mx;
-            sl.dims[m] = std::max(mx + 1, prev[m]);
-            prev[m] = sl.dims[m];
-        }
-        out->slices.push_back(sl);
-        ptr = b;
-    }
-    return ALTO_OK;
-}
-
 static constexpr int kAdmmMaxIter = 25;
 
 }  // namespace alto
@@ -359,8 +288,7 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
                                  double *out_admm_iters, double *out_grams,
                                  double *out_final_error, double *timings,
                                  void *stream) {
-    if (!coo || coo->on_device || !coo->dims || !coo->cidx || !coo->vals || !init_factors)
-        return ALTO_ERR_ARG;
+    if (!coo || !coo->dims || !coo->cidx || !coo->vals || !init_factors) return ALTO_ERR_ARG;
     const int N = coo->nmode;
     if (N < 2 || N > kMaxModes || s < 0 || s >= N || R < 1 || R > kMaxRank || max_iters < 0)
         return ALTO_ERR_ARG;
@@ -374,9 +302,7 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
 
     // ---- StreamingSparseTensor: sort by (s, s+1, ...), relabel, slices ----
     StreamInput sin;
-    ALTO_TRY(stream_prepare(coo, s, &sin));
-    const auto &cid = sin.cid;
-    const auto &vals = sin.vals;
+    ALTO_TRY(stream_prepare(coo, s, &sin, st));
     const auto &slices = sin.slices;
     int nsl = (int)slices.size();
     if (max_slices > 0 && nsl > max_slices) nsl = max_slices;
@@ -390,18 +316,11 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
     }
     for (int k = 0; k < nsl; ++k) maxslice = std::max(maxslice, slices[k].end - slices[k].start);
     uint64_t used = nsl ? slices[nsl - 1].end : 0;
-    DevBuf dci[kMaxModes], dvals, zidx, U[kMaxModes], P[kMaxModes], Uf[kMaxModes], duals[kMaxModes];
+    DevBuf zidx, U[kMaxModes], P[kMaxModes], Uf[kMaxModes], duals[kMaxModes];
     DevBuf G, oldG, ata, cross, gpart, vinv, flag, rho, buf, aux, part0, part, dpart, delta,
         admm_it, colp, lam;
-    for (int m = 0; m < N; ++m) {
-        if (m == s) continue;
-        ALTO_TRY(dci[m].alloc(sizeof(uint64_t) * std::max<uint64_t>(used, 1)));
-        if (used)
-            ALTO_HIP_TRY(hipMemcpyAsync(dci[m].p, cid[m].data(), sizeof(uint64_t) * used,
-                                        hipMemcpyHostToDevice, st));
-    }
-    ALTO_TRY(dvals.alloc(sizeof(double) * std::max<uint64_t>(used, 1)));
-    if (used) ALTO_HIP_TRY(hipMemcpyAsync(dvals.p, vals.data(), sizeof(double) * used, hipMemcpyHostToDevice, st));
+    DevBuf *dci = sin.cid;
+    DevBuf &dvals = sin.vals;
     ALTO_TRY(zidx.alloc(sizeof(uint64_t) * maxslice));
     k_zero_u64<<<cdiv(maxslice, 256), 256, 0, st>>>(zidx.as<uint64_t>(), maxslice);
     for (int m = 0; m < N; ++m) {
@@ -628,11 +547,7 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
     if (out_final_error) {
         // cpd_error(X, factored) (streaming_cpd.cpp:307-345): the processed
         // nonzeros in relabeled coordinates, the time factor = the s_t rows.
-        DevBuf tidx;
-        ALTO_TRY(tidx.alloc(sizeof(uint64_t) * std::max<uint64_t>(used, 1)));
-        if (used)
-            ALTO_HIP_TRY(hipMemcpyAsync(tidx.p, cid[s].data(), sizeof(uint64_t) * used,
-                                        hipMemcpyHostToDevice, st));
+        DevBuf &tidx = sin.cid[s];
         const uint64_t *cp[kMaxModes];
         const double *fp[kMaxModes];
         uint64_t rows[kMaxModes];

@@ -247,8 +247,7 @@ extern "C" int alto_hip_spcpstream(const alto_hip_coo *coo, int s, int R, int ma
                                    uint64_t *out_dims, double *out_time_rows, int *out_outer_iters,
                                    double *out_grams, double *out_final_error, double *timings,
                                    void *stream) {
-    if (!coo || coo->on_device || !coo->dims || !coo->cidx || !coo->vals || !init_factors)
-        return ALTO_ERR_ARG;
+    if (!coo || !coo->dims || !coo->cidx || !coo->vals || !init_factors) return ALTO_ERR_ARG;
     const int N = coo->nmode;
     if (N < 2 || N > kMaxModes || s < 0 || s >= N || R < 1 || R > 64 || max_iters < 1)
         return ALTO_ERR_ARG;
@@ -263,8 +262,7 @@ extern "C" int alto_hip_spcpstream(const alto_hip_coo *coo, int s, int R, int ma
     hipStream_t st = to_stream(stream);
 
     StreamInput sin;
-    ALTO_TRY(stream_prepare(coo, s, &sin));
-    const auto &cid = sin.cid;
+    ALTO_TRY(stream_prepare(coo, s, &sin, st));
     const auto &slices = sin.slices;
     int nsl = (int)slices.size();
     if (max_slices > 0 && nsl > max_slices) nsl = max_slices;
@@ -278,25 +276,19 @@ extern "C" int alto_hip_spcpstream(const alto_hip_coo *coo, int s, int R, int ma
     for (int k = 0; k < nsl; ++k) maxslice = std::max(maxslice, slices[k].end - slices[k].start);
     const uint64_t used = nsl ? slices[nsl - 1].end : 0;
     const size_t RR = (size_t)R * R;
-    DevBuf dci[kMaxModes], dvals, zidx, U[kMaxModes], P[kMaxModes], A[kMaxModes], Ap[kMaxModes],
+    DevBuf *dci = sin.cid;
+    DevBuf &dvals = sin.vals;
+    DevBuf zidx, U[kMaxModes], P[kMaxModes], A[kMaxModes], Ap[kMaxModes],
         Af[kMaxModes], flag[kMaxModes], pos[kMaxModes], rowind[kMaxModes], cco[kMaxModes];
     DevBuf G, c, h, cnz, hnz, cz, hz, cprev, czprev, cnzprev, Q, QPi, vinv, gpart, part, colp, lam,
         delta, iflag, scan_tmp, trows;
     for (int m = 0; m < N; ++m) {
         if (m == s) continue;
-        ALTO_TRY(dci[m].alloc(sizeof(uint64_t) * std::max<uint64_t>(used, 1)));
-        if (used)
-            ALTO_HIP_TRY(hipMemcpyAsync(dci[m].p, cid[m].data(), sizeof(uint64_t) * used,
-                                        hipMemcpyHostToDevice, st));
         ALTO_TRY(flag[m].alloc(sizeof(uint32_t) * maxd[m]));
         ALTO_TRY(pos[m].alloc(sizeof(uint32_t) * maxd[m]));
         ALTO_TRY(rowind[m].alloc(sizeof(uint64_t) * maxd[m]));
         ALTO_TRY(cco[m].alloc(sizeof(uint64_t) * maxslice));
     }
-    ALTO_TRY(dvals.alloc(sizeof(double) * std::max<uint64_t>(used, 1)));
-    if (used)
-        ALTO_HIP_TRY(hipMemcpyAsync(dvals.p, sin.vals.data(), sizeof(double) * used,
-                                    hipMemcpyHostToDevice, st));
     ALTO_TRY(zidx.alloc(sizeof(uint64_t) * maxslice));
     ALTO_HIP_TRY(hipMemsetAsync(zidx.p, 0, sizeof(uint64_t) * maxslice, st));
     for (int m = 0; m < N; ++m) {
@@ -521,11 +513,7 @@ extern "C" int alto_hip_spcpstream(const alto_hip_coo *coo, int s, int R, int ma
         for (int k = 0; k < nsl; ++k) out_outer_iters[k] = outer_v[k];
     if (out_final_error) {
         // cpd_error over the processed nonzeros (relabelled), time factor = s_t rows
-        DevBuf tidx;
-        ALTO_TRY(tidx.alloc(sizeof(uint64_t) * std::max<uint64_t>(used, 1)));
-        if (used)
-            ALTO_HIP_TRY(hipMemcpyAsync(tidx.p, cid[s].data(), sizeof(uint64_t) * used,
-                                        hipMemcpyHostToDevice, st));
+        DevBuf &tidx = sin.cid[s];
         const uint64_t *cp[kMaxModes];
         const double *fp[kMaxModes];
         uint64_t rows[kMaxModes];

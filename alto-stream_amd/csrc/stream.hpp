@@ -16,18 +16,19 @@ struct Slice {
 };
 
 // StreamingSparseTensor (src/streaming_sptensor.cpp:69-264) with
-// next_dynamic_batch at timeslice limit 1: the nonzeros stably sorted by
-// (s, s+1, ...) mod N, every non-stream index relabelled by first appearance
-// in that order, one slice per time index.
+// next_dynamic_batch at timeslice limit 1, built on the device: the nonzeros
+// stably sorted by (s, s+1, ...) mod N, every non-stream index relabelled by
+// first appearance in that order, one slice per time index.
 struct StreamInput {
-    std::vector<std::vector<uint64_t>> cid;  // per mode, relabelled (stream mode: time index)
-    std::vector<double> vals;
+    DevBuf cid[kMaxModes];  // device, per mode, relabelled (stream mode: time index)
+    DevBuf vals;            // device
     std::vector<Slice> slices;
 };
 
-// Host COO in; rejects coordinates >= dims (ALTO_ERR_ARG) and a time index
-// without nonzeros (ALTO_ERR_EMPTY_SLICE; the reference asserts).
-int stream_prepare(const alto_hip_coo *coo, int s, StreamInput *out);
+// Host or device COO in (nnz < 2^32); rejects coordinates >= dims
+// (ALTO_ERR_ARG) and a time index without nonzeros (ALTO_ERR_EMPTY_SLICE;
+// the reference asserts).
+int stream_prepare(const alto_hip_coo *coo, int s, StreamInput *out, hipStream_t st);
 
 // cpd_error (src/streaming_cpd.cpp:368-464) over device COO and factors;
 // out[4]: err, ||X||^2, ||M||^2, <X, M>.

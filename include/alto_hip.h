@@ -213,6 +213,23 @@ int alto_hip_cpstream(const alto_hip_coo *coo, int stream_mode, int rank,
                       double *out_grams, double *out_final_error, double *timings,
                       void *stream);
 
+/* ---- StreamingSparseTensor (src/streaming_sptensor.cpp:69-264 with tt_sort,
+ * src/sort.cpp:477-519; next_dynamic_batch at timeslice limit 1), on the
+ * device: the nonzeros stably sorted by (s, s+1, ...) mod N (ties keep input
+ * order), every non-stream index relabelled by first appearance in that
+ * order, one slice per time index.  out_cidx[m] / out_vals (nnz each; host
+ * or device as coo->on_device; entries may be NULL) receive the prepared
+ * stream.  *out_nslices = number of slices; when it is <= max_slices,
+ * out_slice_ptr (nslices + 1) receives the slice boundaries and
+ * out_slice_dims (nslices x nmode, row-major) the model dims after each
+ * slice (running max of label + 1; 1 for the stream mode).  nnz < 2^32;
+ * coordinates >= dims are ALTO_ERR_ARG; a time index without nonzeros below
+ * the largest is ALTO_ERR_EMPTY_SLICE (the reference asserts).  CP-stream and
+ * spCP-stream prepare their input with it. */
+int alto_hip_stream_prepare(const alto_hip_coo *coo, int stream_mode, uint64_t *const *out_cidx,
+                            double *out_vals, uint64_t max_slices, uint64_t *out_nslices,
+                            uint64_t *out_slice_ptr, uint64_t *out_slice_dims, void *stream);
+
 /* ---- spCP-stream: cpstream with use_spcpstream (model SPCPSTREAM_ALTO,
  * src/streaming_cpd.cpp:26-366, main.cpp:163, 397) driving
  * spcpstream_alto_iter (streaming_cpd.cpp:466-1085), timeslice limit 1.
