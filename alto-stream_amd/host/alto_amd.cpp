@@ -2,24 +2,89 @@
 // container and thin wrappers over the C ABI.
 #include "alto_amd.hpp"
 
+#include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
+#include <thread>
+#include <vector>
 
 using altoamd::check;
 
 // ---- sptensor.cpp:74-242 ----
+// Text import (read_tns_dims + read_tns_data, sptensor.cpp:74-158) in
+// parallel: the file is read once into memory and split at line boundaries
+// into one chunk per thread; a first pass counts each chunk's nonzero lines
+// and index maxima, a second parses them into place.  Line rules as the
+// reference's getline loop: a line counts when it has more than one
+// character (newline included) and does not start with '#'; the mode count
+// is the number of space/tab-separated tokens of the first such line minus
+// one; each line is parsed as a NUL-terminated string (strtoull per mode,
+// then strtod), so a short line reads zeros, as in the reference.
+namespace {
+
+struct Chunk {
+    size_t begin, end;  // [begin, end) of the buffer, starting at a line start
+    IType nnz = 0;
+    IType maxidx[ALTO_MAX_MODES] = {0};
+};
+
+int io_threads() {
+    unsigned hw = std::thread::hardware_concurrency();
+    if (const char *e = getenv("OMP_NUM_THREADS")) {
+        int v = atoi(e);
+        if (v > 0) hw = (unsigned)v;
+    }
+    if (hw < 1) hw = 1;
+    return (int)std::min(hw, 64u);
+}
+
+// Calls f(line, length incl. newline) for every line of [begin, end);
+// lines are NUL-terminated in place (the newline becomes '\0').
+template <typename F>
+void for_lines(char *buf, size_t begin, size_t end, F f) {
+    size_t a = begin;
+    while (a < end) {
+        size_t b = a;
+        while (b < end && buf[b] != '\n') ++b;
+        const size_t len = b < end ? b - a + 1 : b - a;  // getline's count
+        if (b < end) buf[b] = '\0';
+        f(buf + a, len);
+        a = b + 1;
+    }
+}
+
+}  // namespace
+
 static void read_text(FILE *fp, SparseTensor *X, IType zero_based) {
-    // pass 1: modes, dims (max index, +1 when zero based), nnz
-    char *line = nullptr;
-    size_t len = 0;
-    ssize_t nread;
+    if (fseek(fp, 0, SEEK_END) != 0) {
+        fprintf(stderr, "ALTO: cannot seek the tensor file\n");
+        exit(1);
+    }
+    const long fsz = ftell(fp);
+    rewind(fp);
+    std::vector<char> buf((size_t)(fsz > 0 ? fsz : 0) + 1, '\0');
+    if (fsz > 0 && fread(buf.data(), 1, (size_t)fsz, fp) != (size_t)fsz) {
+        fprintf(stderr, "ALTO: short read of the tensor file\n");
+        exit(1);
+    }
+    const size_t n = (size_t)(fsz > 0 ? fsz : 0);
+    // mode count from the first counted line (split on space and tab only)
     int nmodes = 0;
-    while ((nread = getline(&line, &len, fp)) != -1) {
-        if (nread > 1 && line[0] != '#') {
-            char *save = nullptr;
-            for (char *p = strtok_r(line, " \t\n", &save); p; p = strtok_r(nullptr, " \t\n", &save)) ++nmodes;
-            break;
+    {
+        size_t a = 0;
+        while (a < n) {
+            size_t b = a;
+            while (b < n && buf[b] != '\n') ++b;
+            const size_t len = b < n ? b - a + 1 : b - a;
+            if (len > 1 && buf[a] != '#') {
+                std::string line(buf.data() + a, len);
+                char *save = nullptr;
+                for (char *t = strtok_r(&line[0], " \t", &save); t; t = strtok_r(nullptr, " \t", &save)) ++nmodes;
+                break;
+            }
+            a = b + 1;
         }
     }
     --nmodes;
@@ -27,18 +92,47 @@ static void read_text(FILE *fp, SparseTensor *X, IType zero_based) {
         fprintf(stderr, "ALTO: bad .tns header (%d modes)\n", nmodes);
         exit(1);
     }
-    std::vector<IType> dims(nmodes, 0);
-    IType nnz = 0;
-    rewind(fp);
-    while ((nread = getline(&line, &len, fp)) != -1) {
-        if (nread > 1 && line[0] != '#') {
-            char *p = line;
-            for (int m = 0; m < nmodes; ++m) {
-                IType v = strtoull(p, &p, 10) + zero_based;
-                if (v > dims[m]) dims[m] = v;
-            }
-            ++nnz;
+    // chunks at line starts
+    const int T = (int)std::max<size_t>(1, std::min<size_t>((size_t)io_threads(), n / (1 << 16) + 1));
+    std::vector<Chunk> ch(T);
+    {
+        size_t prev = 0;
+        for (int c = 0; c < T; ++c) {
+            size_t b = c + 1 == T ? n : std::max(prev, (size_t)((double)n * (c + 1) / T));
+            while (b < n && b > 0 && buf[b - 1] != '\n') ++b;  // next line start
+            ch[c].begin = prev;
+            ch[c].end = b;
+            prev = b;
         }
+    }
+    char *base = buf.data();
+    auto parallel = [&](auto body) {
+        std::vector<std::thread> th;
+        for (int c = 1; c < T; ++c) th.emplace_back(body, c);
+        body(0);
+        for (auto &t : th) t.join();
+    };
+    // pass 1: counted lines and index maxima per chunk (read_tns_dims)
+    parallel([&](int c) {
+        Chunk &k = ch[c];
+        for_lines(base, k.begin, k.end, [&](char *line, size_t len) {
+            if (len > 1 && line[0] != '#') {
+                char *p = line;
+                for (int m = 0; m < nmodes; ++m) {
+                    IType v = strtoull(p, &p, 10) + zero_based;
+                    if (v > k.maxidx[m]) k.maxidx[m] = v;
+                }
+                ++k.nnz;
+            }
+        });
+    });
+    IType nnz = 0;
+    std::vector<IType> off(T);
+    std::vector<IType> dims(nmodes, 0);
+    for (int c = 0; c < T; ++c) {
+        off[c] = nnz;
+        nnz += ch[c].nnz;
+        for (int m = 0; m < nmodes; ++m) dims[m] = std::max(dims[m], ch[c].maxidx[m]);
     }
     X->nmodes = nmodes;
     X->nnz = nnz;
@@ -49,17 +143,26 @@ static void read_text(FILE *fp, SparseTensor *X, IType zero_based) {
         X->cidx[m] = (IType *)malloc(sizeof(IType) * (nnz ? nnz : 1));
     }
     X->vals = (FType *)malloc(sizeof(FType) * (nnz ? nnz : 1));
-    rewind(fp);
-    IType k = 0, one_based = zero_based ? 0 : 1;
-    while ((nread = getline(&line, &len, fp)) != -1) {
-        if (nread > 1 && line[0] != '#') {
-            char *p = line;
-            for (int m = 0; m < nmodes; ++m) X->cidx[m][k] = strtoull(p, &p, 10) - one_based;
-            X->vals[k] = strtod(p, &p);
-            ++k;
+    // pass 2: parse into place (read_tns_data); lines are NUL-terminated now
+    const IType one_based = zero_based ? 0 : 1;
+    parallel([&](int c) {
+        const Chunk &k = ch[c];
+        IType at = off[c];
+        size_t a = k.begin;
+        while (a < k.end) {
+            const size_t len0 = strlen(base + a);
+            const bool nl = a + len0 < k.end;  // the newline was replaced by '\0'
+            const size_t len = len0 + (nl ? 1 : 0);
+            char *line = base + a;
+            if (len > 1 && line[0] != '#') {
+                char *p = line;
+                for (int m = 0; m < nmodes; ++m) X->cidx[m][at] = (IType)strtoull(p, &p, 10) - one_based;
+                X->vals[at] = (FType)strtod(p, &p);
+                ++at;
+            }
+            a += len0 + 1;
         }
-    }
-    free(line);
+    });
 }
 
 void ImportSparseTensor(const char *file_path, FileFormat f, SparseTensor **X_, IType zero_based) {
