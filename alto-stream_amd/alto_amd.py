@@ -193,6 +193,14 @@ class AltoTensor:
         check(lib().alto_hip_tensor_keys(_vp(self.handle), k.ctypes.data_as(_u64p)), "keys")
         return k
 
+    def packed_keys(self):
+        """Sorted keys in the ALT_PEXT layout and the per-mode bit positions."""
+        k = np.zeros((self.nnz, self.kw), dtype=np.uint64)
+        pos = np.zeros(len(self.dims), dtype=np.int32)
+        check(lib().alto_hip_tensor_packed_keys(_vp(self.handle), k.ctypes.data_as(_u64p),
+                                                pos.ctypes.data_as(ctypes.POINTER(ctypes.c_int))), "packed_keys")
+        return k, pos
+
     def vals(self):
         v = np.zeros(self.nnz, dtype=np.float64)
         check(lib().alto_hip_tensor_vals(_vp(self.handle), v.ctypes.data_as(_f64p)), "vals")

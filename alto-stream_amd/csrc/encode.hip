@@ -573,6 +573,24 @@ int alto_hip_tensor_keys(alto_hip_tensor t, uint64_t *keys) {
                                nullptr);
 }
 
+int alto_hip_tensor_packed_keys(alto_hip_tensor t, uint64_t *keys, int *pos) {
+    if (!t) return ALTO_ERR_ARG;
+    const TensorImpl &ti = t->impl;
+    if (pos)
+        for (int m = 0; m < ti.nmode; ++m) pos[m] = ti.pack.shift[m];
+    if (!keys || ti.nnz == 0) return ALTO_OK;
+    if (ti.pkw == ti.kw)
+        return alto_hip_memcpy_d2h(keys, ti.pkeys.p, sizeof(uint64_t) * ti.kw * ti.nnz, nullptr);
+    // forced two-word keys over a one-word layout: zero high words
+    std::vector<uint64_t> lo(ti.nnz);
+    ALTO_TRY(alto_hip_memcpy_d2h(lo.data(), ti.pkeys.p, sizeof(uint64_t) * ti.nnz, nullptr));
+    for (uint64_t i = 0; i < ti.nnz; ++i) {
+        keys[2 * i] = lo[i];
+        keys[2 * i + 1] = 0;
+    }
+    return ALTO_OK;
+}
+
 int alto_hip_tensor_vals(alto_hip_tensor t, double *vals) {
     if (!t || !vals) return ALTO_ERR_ARG;
     return alto_hip_memcpy_d2h(vals, t->impl.vals64.p, sizeof(double) * t->impl.nnz, nullptr);

@@ -99,6 +99,12 @@ int alto_hip_tensor_dims(alto_hip_tensor t, uint64_t *dims);
 int alto_hip_tensor_masks(alto_hip_tensor t, uint64_t *masks);
 int alto_hip_tensor_keys(alto_hip_tensor t, uint64_t *keys);
 int alto_hip_tensor_vals(alto_hip_tensor t, double *vals);
+/* The sorted keys in the ALT_PEXT layout (alto.hpp:260-292 with mode_pos,
+ * alto.hpp:982-988): mode m's index in bits [pos[m], pos[m] + bits_m),
+ * pos = prefix sum of the modes' bit counts in mode order; key_words words
+ * per key (word 0 = low 64 bits).  The MTTKRP kernels read this layout.
+ * pos: nmode ints (may be NULL). */
+int alto_hip_tensor_packed_keys(alto_hip_tensor t, uint64_t *keys, int *pos);
 int alto_hip_tensor_perm(alto_hip_tensor t, uint32_t *perm);
 int alto_hip_tensor_partitions(alto_hip_tensor t, uint64_t *prtn_ptr,
                                uint64_t *intervals);

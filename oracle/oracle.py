@@ -123,6 +123,31 @@ def sort_alto(keys, vals):
     return keys, vals, perm
 
 
+def alt_pext(dims, keys, masks):
+    """ALT_PEXT re-encoding (alto.hpp:260-292, mode_pos alto.hpp:982-988):
+    each sorted key's mode fields pext'ed out and packed at pos[m] = prefix
+    sum of num_bits over the modes in mode order.  keys: (nnz, kw) words.
+    Returns (packed keys (nnz, kw), pos)."""
+    keys = np.asarray(keys, dtype=np.uint64).reshape(len(keys), -1)
+    masks = np.asarray(masks, dtype=np.uint64).reshape(len(dims), -1)
+    kw = keys.shape[1]
+    bits = [num_bits(d) for d in dims]
+    pos = np.concatenate([[0], np.cumsum(bits)[:-1]]).astype(int)
+    out = np.zeros_like(keys)
+    one = np.uint64(1)
+    for m in range(len(dims)):
+        j = 0  # next field bit: mask bits in ascending order (pext)
+        for w in range(kw):
+            mw = int(masks[m, w])
+            for b in range(64):
+                if (mw >> b) & 1:
+                    bit = (keys[:, w] >> np.uint64(b)) & one
+                    dst = int(pos[m]) + j
+                    out[:, dst // 64] |= bit << np.uint64(dst % 64)
+                    j += 1
+    return out, pos
+
+
 def prtn_alto(dims, keys, masks, nprtn):
     dims = np.ascontiguousarray(dims, dtype=np.uint64)
     keys = np.ascontiguousarray(keys, dtype=np.uint64)

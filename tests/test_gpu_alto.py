@@ -73,6 +73,19 @@ def test_encode_sort_partition_bit_exact(gpu, case, nprtn):
     np.testing.assert_array_equal(iv, ref.intervals)
 
 
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_alt_pext_layout_bit_exact(gpu, case):
+    """The packed keys the MTTKRP reads are the reference's ALT_PEXT
+    re-encoding (alto.hpp:260-292) of the sorted keys."""
+    name, dims, cidx, vals = case
+    ref = O.AltoTensor(dims, cidx, vals, nprtn=4)
+    at = gpu.create_alto(dims, cidx, vals, nprtn=4)
+    packed, pos = at.packed_keys()
+    exp, epos = O.alt_pext(dims, ref.keys, ref.masks)
+    np.testing.assert_array_equal(pos, epos)
+    np.testing.assert_array_equal(packed, exp)
+
+
 def test_forced_128bit_key_matches_oracle(gpu):
     dims, cidx, vals = rand_tensor([1000, 800, 600], 5000, seed=36, dup_frac=0.1)
     ref = O.AltoTensor(dims, cidx, vals, nprtn=4, kw=2)

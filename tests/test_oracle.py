@@ -90,6 +90,34 @@ def test_small_sorted_pairs_kat():
     assert list(at.vals) == [4.0, 3.0, 1.0, 6.0, 2.0, 5.0]
 
 
+def test_alt_pext_small_kat():
+    """ALT_PEXT re-encoding of small.tns (dims 2x3x2, bits 1/2/1, masks
+    0x1/0xC/0x2): by hand, key 1 -> (1,0,0) -> 1; 2 -> (0,0,1) -> 8;
+    4 -> (0,1,0) -> 2; 7 -> (1,1,1) -> 11; 8 -> (0,2,0) -> 4; 9 -> (1,2,0) -> 5
+    with fields at bit positions 0, 1, 3."""
+    dims, cidx, vals = O.read_tns(golden("small.tns"))
+    at = O.AltoTensor(dims, cidx, vals, nprtn=8)
+    packed, pos = O.alt_pext(dims, at.keys, at.masks)
+    assert list(pos) == [0, 1, 3]
+    assert [int(k) for k in packed[:, 0]] == [1, 8, 2, 11, 4, 5]
+
+
+def test_alt_pext_fields_are_the_sorted_coordinates():
+    """Each packed field is the coordinate of the sorted nonzero (128-bit
+    keys included)."""
+    rng = np.random.default_rng(42)
+    dims = [1 << 20, 1 << 20, 1 << 18, 1 << 16, 1 << 12]
+    cidx = [rng.integers(0, d, size=300).astype(np.uint64) for d in dims]
+    at = O.AltoTensor(dims, cidx, rng.random(300), nprtn=4)
+    packed, pos = O.alt_pext(dims, at.keys, at.masks)
+    for i in range(300):
+        v = int(packed[i, 0]) | (int(packed[i, 1]) << 64)
+        src = int(at.perm[i])
+        for m, d in enumerate(dims):
+            w = O.num_bits(d)
+            assert (v >> int(pos[m])) & ((1 << w) - 1) == int(cidx[m][src])
+
+
 def test_partition_closed_form_kat():
     dims, cidx, vals = O.read_tns(golden("small.tns"))
     at = O.AltoTensor(dims, cidx, vals, nprtn=8)
