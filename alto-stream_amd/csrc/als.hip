@@ -191,7 +191,7 @@ static int als_run(TensorImpl *t, const alto_hip_als_opts *o, double *const *fac
                 const uint64_t I = t->dims[n];
                 const int nb = row_blocks(I);
                 ALTO_TRY(launch_fill_zero(mt.as<double>(), I * R, s));
-                ALTO_TRY(run_mttkrp(t, n, prec, R, fptr, mt.p, 1, 2, 0, -1, s));
+                ALTO_TRY(run_mttkrp(t, n, prec, R, fptr, mt.p, 1, 2, 0, -1, s, false, true));
                 ALTO_TRY(launch_chol_inv(grams.as<double>(), N, n, R, 1e-3, true, vinv.as<double>(),
                                          flag.as<int>(), nullptr, s));
                 ALTO_TRY(launch_solve(mt.as<double>(), I, R, vinv.as<double>(), part.as<double>(), nb, s));
@@ -296,7 +296,7 @@ static int als_run(TensorImpl *t, const alto_hip_als_opts *o, double *const *fac
             ALTO_HIP_TRY(hipEventRecord(E[0], s));
             ALTO_TRY(launch_fill_zero(mt.as<double>(), I * R, s));
             ALTO_HIP_TRY(hipEventRecord(E[1], s));
-            ALTO_TRY(run_mttkrp(t, n, prec, R, fptr, mt.p, 1, 2, 0, -1, s));
+            ALTO_TRY(run_mttkrp(t, n, prec, R, fptr, mt.p, 1, 2, 0, -1, s, false, true));
             ALTO_HIP_TRY(hipEventRecord(E[2], s));
             ALTO_TRY(launch_chol_inv(grams.as<double>(), N, n, R, 1e-3, true, vinv.as<double>(),
                                      flag.as<int>(), nullptr, s));

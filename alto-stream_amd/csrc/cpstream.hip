@@ -270,6 +270,10 @@ __global__ void k_delta_add(const double *__restrict__ part, int nblk, double *_
     if (threadIdx.x == 0) *delta += sqrt(a) / (sqrt(b) + 1e-12);
 }
 
+__global__ void k_zero_flag(int *p) {
+    if (threadIdx.x == 0) p[0] = 0;
+}
+
 __global__ void k_zero_u64(uint64_t *p, uint64_t n) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
@@ -443,11 +447,13 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
         ALTO_TRY(launch_fill_zero(admm_it.as<double>(), 1, st));
         double prev_delta = 0.0;
         int outer = 0;
-        for (int i = 0; i < max_iters; ++i) {
+        // one outer iteration (streaming_cpd.cpp:1269-1470), no host syncs:
+        // replayed as a graph from the second iteration on
+        auto outer_iter = [&]() -> int {
             ALTO_TRY(launch_fill_zero(delta.as<double>(), 1, st));
             // s_t: stream-mode MTTKRP and solve (streaming_cpd.cpp:1272-1299)
             ALTO_TRY(launch_fill_zero(U[s].as<double>(), bd[s] * R, st));
-            ALTO_TRY(run_mttkrp(&at, s, prec, R, fptr, U[s].p, 1, 2, 0, -1, st));
+            ALTO_TRY(run_mttkrp(&at, s, prec, R, fptr, U[s].p, 1, 2, 0, -1, st, false, true));
             ALTO_TRY(launch_chol_inv(G.as<double>(), N, s, R, 1e-12, false, vinv.as<double>(),
                                      flag.as<int>() + 1, nullptr, st, 1, nullptr));
             ALTO_TRY(launch_solve(U[s].as<double>(), bd[s], R, vinv.as<double>(), colp.as<double>(),
@@ -462,7 +468,7 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
                 const uint64_t I = bd[j];
                 const int nb = row_blocks(I);
                 ALTO_TRY(launch_fill_zero(buf.as<double>(), I * R, st));
-                ALTO_TRY(run_mttkrp(&at, j, prec, R, fptr, buf.p, 1, 2, 0, -1, st));
+                ALTO_TRY(run_mttkrp(&at, j, prec, R, fptr, buf.p, 1, 2, 0, -1, st, false, true));
                 int nc = 0;
                 for (int m = 0; m < N; ++m) {
                     if (m == j || m == s) continue;
@@ -488,16 +494,30 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
                                                          aux.as<double>(), part0.as<double>());
                 ALTO_LAUNCH_CHECK();
                 if (nonneg) {
-                    ALTO_HIP_TRY(hipMemsetAsync(flag.p, 0, sizeof(int), st));
-                    for (int it = 0; it < kAdmmMaxIter; ++it) {
-                        k_admm_iter<<<nb, 256, lds_iter, st>>>(buf.as<double>(), aux.as<double>(),
-                                                               duals[j].as<double>(), vinv.as<double>(),
-                                                               rho.as<double>(), flag.as<int>(), I, R,
-                                                               ts.RP, part.as<double>());
-                        k_admm_check<<<1, 64, 0, st>>>(part.as<double>(), nb, part0.as<double>(), it,
-                                                       flag.as<int>(), admm_it.as<double>());
+                    // At most 25 iterations; once k_admm_check sets the flag the
+                    // remaining launches would do nothing, so they are issued in
+                    // short chunks and the flag is read between chunks (the
+                    // results are those of issuing all 25; the ADMM typically
+                    // converges in 2-3 iterations).
+                    k_zero_flag<<<1, 64, 0, st>>>(flag.as<int>());
+                    int it = 0;
+                    while (it < kAdmmMaxIter) {
+                        const int chunk = it == 0 ? 3 : 4;
+                        for (int c = 0; c < chunk && it < kAdmmMaxIter; ++c, ++it) {
+                            k_admm_iter<<<nb, 256, lds_iter, st>>>(buf.as<double>(), aux.as<double>(),
+                                                                   duals[j].as<double>(), vinv.as<double>(),
+                                                                   rho.as<double>(), flag.as<int>(), I, R,
+                                                                   ts.RP, part.as<double>());
+                            k_admm_check<<<1, 64, 0, st>>>(part.as<double>(), nb, part0.as<double>(), it,
+                                                           flag.as<int>(), admm_it.as<double>());
+                        }
+                        ALTO_LAUNCH_CHECK();
+                        if (it >= kAdmmMaxIter) break;
+                        int conv = 0;
+                        ALTO_HIP_TRY(hipMemcpyAsync(&conv, flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+                        ALTO_HIP_TRY(hipStreamSynchronize(st));
+                        if (conv) break;
                     }
-                    ALTO_LAUNCH_CHECK();
                     k_admm_final<<<cdiv(I * R, 256), 256, 0, st>>>(aux.as<double>(), duals[j].as<double>(),
                                                                    U[j].as<double>(), I * R);
                 }
@@ -508,6 +528,14 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
                 k_delta_add<<<1, 64, 0, st>>>(dpart.as<double>(), nb, delta.as<double>());
                 ALTO_LAUNCH_CHECK();
             }
+            return ALTO_OK;
+        };
+        // Without the constraint the iterations after the first replay one
+        // captured graph; with it they run directly (the ADMM reads its flag).
+        IterGraph ig(st);
+        for (int i = 0; i < max_iters; ++i) {
+            if (nonneg) ALTO_TRY(outer_iter());
+            else ALTO_TRY(ig.run(i, outer_iter));
             double hd[2];
             int hf[4];
             ALTO_HIP_TRY(hipMemcpyAsync(hd, delta.p, sizeof(double), hipMemcpyDeviceToHost, st));

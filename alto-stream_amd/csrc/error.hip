@@ -118,7 +118,7 @@ int cpd_error_dev(int N, const uint64_t *rows, uint64_t nnz, const uint64_t *con
         ALTO_TRY(build_alto(&t, &c, 0, 0, false, s));
         const void *fp[kMaxModes];
         for (int m = 0; m < N; ++m) fp[m] = factors[m];
-        ALTO_TRY(run_mttkrp(&t, mn, ALTO_PREC_F64, R, fp, mt.p, 1, 2, 0, -1, s));
+        ALTO_TRY(run_mttkrp(&t, mn, ALTO_PREC_F64, R, fp, mt.p, 1, 2, 0, -1, s, false, true));
         ALTO_HIP_TRY(hipStreamSynchronize(s));
     }
     k_dot_part<<<kDotBlocks, 256, 0, s>>>(factors[mn], mt.as<double>(), I * R, dpart.as<double>());

@@ -22,6 +22,7 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <type_traits>
 
 #include "tensor.hpp"
 
@@ -88,6 +89,15 @@ template <typename T> __device__ __forceinline__ T stream_load(const T *p, bool 
 // LDS across the workgroup, then with atomics for the workgroup's first and
 // last rows; bit 3 = no decode-once kernel; bit 4 = no fiber kernel (output
 // order only).
+// Sum of a row that this range owns (output-ordered schedule): added to the
+// output, or stored when the output held zeros on entry (flags bit 5), which
+// saves the dependent load of a read-modify-write.
+template <typename OT, typename CT>
+__device__ __forceinline__ void flush_owned(OT *p, CT v, bool zeroed) {
+    if (zeroed) *p = (OT)v;
+    else *p += (OT)v;
+}
+
 template <typename VT, typename FT, typename OT, int R, int PKW, int B>
 __global__ __launch_bounds__(256) void k_mttkrp_seg(const uint64_t *__restrict__ pkeys,
                                                     const VT *__restrict__ vals, uint64_t nnz,
@@ -116,7 +126,7 @@ __global__ __launch_bounds__(256) void k_mttkrp_seg(const uint64_t *__restrict__
     const int tsh = pk.shift[target], tw = pk.width[target];
     OT *outp = out + lig * V;
 
-    const bool nt = flags & 1, dedup = flags & 2;
+    const bool nt = flags & 1, dedup = flags & 2, zeroed = flags & 32;
     uint64_t cur = ~0ULL, first_row = ~0ULL;
     bool in_first = true;
     CT acc[V], facc[V];
@@ -196,7 +206,7 @@ __global__ __launch_bounds__(256) void k_mttkrp_seg(const uint64_t *__restrict__
                             in_first = false;
                         } else {
 #pragma unroll
-                            for (int v = 0; v < V; ++v) outp[cur * R + v] += (OT)acc[v];
+                            for (int v = 0; v < V; ++v) flush_owned(outp + cur * R + v, acc[v], zeroed);
                         }
                     }
                     cur = row;
@@ -248,7 +258,7 @@ __global__ __launch_bounds__(256) void k_mttkrp_seg(const uint64_t *__restrict__
             } else {
                 if (crow != ~0ULL) {
                     if (first_merged) atomic_add_out(o + crow * R, (OT)a);
-                    else o[crow * R] += (OT)a;
+                    else flush_owned(o + crow * R, a, zeroed);
                     first_merged = false;
                 }
                 crow = row;
@@ -292,7 +302,7 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned nb) {
     return x < r ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
 }
 
-template <typename VT, typename FT, typename OT, int R, int PKW, int NM1, int VPL>
+template <typename VT, typename FT, typename OT, int R, int PKW, int NM1, int VPL, bool ZEROED>
 __global__ __launch_bounds__(256) void k_mttkrp_dec(const uint64_t *__restrict__ pkeys,
                                                     const VT *__restrict__ vals, uint64_t nnz,
                                                     FPtrs<FT> F, OT *__restrict__ out,
@@ -316,6 +326,7 @@ __global__ __launch_bounds__(256) void k_mttkrp_dec(const uint64_t *__restrict__
     const int gib = threadIdx.x / L;
     const int lig = threadIdx.x % L;
     const bool nt = flags & 1, excl = flags & 4;
+    constexpr bool zeroed = ZEROED;  // output zero on entry: owned rows are stored
     uint64_t start = group * per_group;
     uint64_t end = start + per_group < nnz ? start + per_group : nnz;
     if (start > nnz) start = nnz;
@@ -416,7 +427,7 @@ __global__ __launch_bounds__(256) void k_mttkrp_dec(const uint64_t *__restrict__
                                 in_first = false;
                             } else {
 #pragma unroll
-                                for (int v = 0; v < VPL; ++v) outp[(uint64_t)cur * R + v] += (OT)acc[v];
+                                for (int v = 0; v < VPL; ++v) flush_owned(outp + (uint64_t)cur * R + v, acc[v], zeroed);
                             }
                         }
                         cur = row;
@@ -466,7 +477,7 @@ __global__ __launch_bounds__(256) void k_mttkrp_dec(const uint64_t *__restrict__
             } else {
                 if (crow != NONE) {
                     if (first_merged) atomic_add_out(o + (uint64_t)crow * R, (OT)a);
-                    else o[(uint64_t)crow * R] += (OT)a;
+                    else flush_owned(o + (uint64_t)crow * R, a, zeroed);
                     first_merged = false;
                 }
                 crow = row;
@@ -510,7 +521,7 @@ __device__ __forceinline__ uint32_t fib_depth(uint64_t alo, uint64_t ahi, uint64
     return d;
 }
 
-template <typename VT, typename FT, typename OT, int R, int PKW, int NM1, int VPL, int CB>
+template <typename VT, typename FT, typename OT, int R, int PKW, int NM1, int VPL, int CB, bool ZEROED>
 __global__ __launch_bounds__(256) void k_mttkrp_fib(const uint64_t *__restrict__ pkeys,
                                                     const VT *__restrict__ vals, uint64_t nnz,
                                                     FPtrs<FT> F, OT *__restrict__ out,
@@ -533,6 +544,7 @@ __global__ __launch_bounds__(256) void k_mttkrp_fib(const uint64_t *__restrict__
     const int gib = threadIdx.x / L;
     const int lig = threadIdx.x % L;
     const bool nt = flags & 1;
+    constexpr bool zeroed = ZEROED;  // output zero on entry: owned rows are stored
     uint64_t start = group * per_group;
     uint64_t end = start + per_group < nnz ? start + per_group : nnz;
     if (start > nnz) start = nnz;
@@ -653,7 +665,7 @@ __global__ __launch_bounds__(256) void k_mttkrp_fib(const uint64_t *__restrict__
                         in_first = false;
                     } else {
 #pragma unroll
-                        for (int v = 0; v < VPL; ++v) outp[(uint64_t)row * R + v] += (OT)t[0][v];
+                        for (int v = 0; v < VPL; ++v) flush_owned(outp + (uint64_t)row * R + v, t[0][v], zeroed);
                     }
 #pragma unroll
                     for (int v = 0; v < VPL; ++v) t[0][v] = 0;
@@ -678,7 +690,7 @@ __global__ __launch_bounds__(256) void k_mttkrp_fib(const uint64_t *__restrict__
             } else {
                 if (crow != NONE) {
                     if (first_merged) atomic_add_out(o + (uint64_t)crow * R, (OT)a);
-                    else o[(uint64_t)crow * R] += (OT)a;
+                    else flush_owned(o + (uint64_t)crow * R, a, zeroed);
                     first_merged = false;
                 }
                 crow = row;
@@ -760,24 +772,30 @@ struct MttkrpArgs {
     FibInfo fi;
 };
 
-template <typename VT, typename FT, typename OT, int R, int PKW, int VPL, int CB>
-static void launch_fib(const MttkrpArgs &a, const FPtrs<FT> &F, hipStream_t s) {
+template <typename VT, typename FT, typename OT, int R, int PKW, int VPL, int CB, bool Z>
+static void launch_fib_z(const MttkrpArgs &a, const FPtrs<FT> &F, hipStream_t s) {
     constexpr int L = R / VPL;
     const uint64_t groups = (a.nnz + a.per_group - 1) / a.per_group;
     const unsigned blocks = cdiv(groups * L, 256);
     const int nm1 = a.pk.nmode - 1;
     if (nm1 == 2)
-        k_mttkrp_fib<VT, FT, OT, R, PKW, 2, VPL, CB><<<blocks, 256, 0, s>>>(
+        k_mttkrp_fib<VT, FT, OT, R, PKW, 2, VPL, CB, Z><<<blocks, 256, 0, s>>>(
             a.pkeys, (const VT *)a.vals, a.nnz, F, (OT *)a.out, a.pk, a.target, a.fi, a.per_group,
             a.flags);
     else if (nm1 == 3)
-        k_mttkrp_fib<VT, FT, OT, R, PKW, 3, VPL, CB><<<blocks, 256, 0, s>>>(
+        k_mttkrp_fib<VT, FT, OT, R, PKW, 3, VPL, CB, Z><<<blocks, 256, 0, s>>>(
             a.pkeys, (const VT *)a.vals, a.nnz, F, (OT *)a.out, a.pk, a.target, a.fi, a.per_group,
             a.flags);
     else
-        k_mttkrp_fib<VT, FT, OT, R, PKW, 4, VPL, CB><<<blocks, 256, 0, s>>>(
+        k_mttkrp_fib<VT, FT, OT, R, PKW, 4, VPL, CB, Z><<<blocks, 256, 0, s>>>(
             a.pkeys, (const VT *)a.vals, a.nnz, F, (OT *)a.out, a.pk, a.target, a.fi, a.per_group,
             a.flags);
+}
+
+template <typename VT, typename FT, typename OT, int R, int PKW, int VPL, int CB>
+static void launch_fib(const MttkrpArgs &a, const FPtrs<FT> &F, hipStream_t s) {
+    if (a.flags & 32) launch_fib_z<VT, FT, OT, R, PKW, VPL, CB, true>(a, F, s);
+    else launch_fib_z<VT, FT, OT, R, PKW, VPL, CB, false>(a, F, s);
 }
 
 template <typename VT, typename FT, typename OT, int R, int PKW>
@@ -797,15 +815,20 @@ static void launch_seg(const MttkrpArgs &a, hipStream_t s) {
         unsigned dblocks = cdiv(dthreads, 256);
         const int nm1 = a.pk.nmode - 1;
         if (a.fiber) return launch_fib<VT, FT, OT, R, PKW, VPL, 1>(a, F, s);
-        if (nm1 == 2)
-            k_mttkrp_dec<VT, FT, OT, R, PKW, 2, VPL><<<dblocks, 256, 0, s>>>(
-                a.pkeys, (const VT *)a.vals, a.nnz, F, (OT *)a.out, a.pk, a.target, a.per_group, a.flags);
-        else if (nm1 == 3)
-            k_mttkrp_dec<VT, FT, OT, R, PKW, 3, VPL><<<dblocks, 256, 0, s>>>(
-                a.pkeys, (const VT *)a.vals, a.nnz, F, (OT *)a.out, a.pk, a.target, a.per_group, a.flags);
-        else
-            k_mttkrp_dec<VT, FT, OT, R, PKW, 4, VPL><<<dblocks, 256, 0, s>>>(
-                a.pkeys, (const VT *)a.vals, a.nnz, F, (OT *)a.out, a.pk, a.target, a.per_group, a.flags);
+        auto dec = [&](auto zc) {
+            constexpr bool Z = decltype(zc)::value;
+            if (nm1 == 2)
+                k_mttkrp_dec<VT, FT, OT, R, PKW, 2, VPL, Z><<<dblocks, 256, 0, s>>>(
+                    a.pkeys, (const VT *)a.vals, a.nnz, F, (OT *)a.out, a.pk, a.target, a.per_group, a.flags);
+            else if (nm1 == 3)
+                k_mttkrp_dec<VT, FT, OT, R, PKW, 3, VPL, Z><<<dblocks, 256, 0, s>>>(
+                    a.pkeys, (const VT *)a.vals, a.nnz, F, (OT *)a.out, a.pk, a.target, a.per_group, a.flags);
+            else
+                k_mttkrp_dec<VT, FT, OT, R, PKW, 4, VPL, Z><<<dblocks, 256, 0, s>>>(
+                    a.pkeys, (const VT *)a.vals, a.nnz, F, (OT *)a.out, a.pk, a.target, a.per_group, a.flags);
+        };
+        if (a.flags & 32) dec(std::true_type{});
+        else dec(std::false_type{});
     } else
         k_mttkrp_seg<VT, FT, OT, R, PKW, B><<<blocks, 256, 0, s>>>(
             a.pkeys, (const VT *)a.vals, a.nnz, F, (OT *)a.out, a.pk, a.target, a.per_group, a.flags);
@@ -955,12 +978,12 @@ static int autotune(TensorImpl *t, MttkrpArgs &a, hipStream_t s) {
 // once both are done (a free would synchronise the device).
 int prepare_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const *factors,
                    int out_f64, hipStream_t s) {
-    return run_mttkrp(t, target, prec, rank, factors, nullptr, out_f64, 2, 0, -1, s, true);
+    return run_mttkrp(t, target, prec, rank, factors, nullptr, out_f64, 2, 0, -1, s, true, true);
 }
 
 int run_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const *factors,
                void *out, int out_f64, int algo, uint64_t per_group, int flags, hipStream_t s,
-               bool prepare_only) {
+               bool prepare_only, bool out_zeroed) {
     if (target < 0 || target >= t->nmode || rank < 1 || rank > 4096 || !factors ||
         (!out && !prepare_only))
         return ALTO_ERR_ARG;
@@ -982,6 +1005,8 @@ int run_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const 
     a.per_group = per_group;  // 0: chosen below from the entry count
     a.flags = flags < 0 ? 3 : flags;
     if (algo == 2) a.flags |= 4;  // output-ordered: runs inside a range own their row
+    if (algo == 2 && out_zeroed) a.flags |= 32;
+    if (algo != 2) a.flags &= ~32;  // owned rows exist only in output-ordered ranges
     // decode-once kernel: nmode <= 5 and 32-bit element offsets into every factor
     a.decode_once = t->nmode >= 3 && t->nmode <= 5 && !(a.flags & 8);
     for (int m = 0; m < t->nmode; ++m)

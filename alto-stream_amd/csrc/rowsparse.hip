@@ -118,7 +118,7 @@ extern "C" int alto_hip_rowsparse_mttkrp(const alto_hip_coo *coo, int mode, int 
     DevBuf Y, present, pos, scan_tmp, rowind_d, vals_d;
     ALTO_TRY(Y.alloc(sizeof(double) * I * R));
     ALTO_HIP_TRY(hipMemsetAsync(Y.p, 0, sizeof(double) * I * R, s));
-    ALTO_TRY(run_mttkrp(&t, mode, ALTO_PREC_F64, R, fptr, Y.p, 1, 2, 0, -1, s));
+    ALTO_TRY(run_mttkrp(&t, mode, ALTO_PREC_F64, R, fptr, Y.p, 1, 2, 0, -1, s, false, true));
     ALTO_TRY(present.alloc(sizeof(uint32_t) * I));
     ALTO_TRY(pos.alloc(sizeof(uint32_t) * I));
     ALTO_HIP_TRY(hipMemsetAsync(present.p, 0, sizeof(uint32_t) * I, s));

@@ -436,11 +436,13 @@ extern "C" int alto_hip_spcpstream(const alto_hip_coo *coo, int s, int R, int ma
         // ---- outer iterations (streaming_cpd.cpp:712-1000) ----
         double prev_delta = 0.0;
         int outer = 0;
-        for (int i = 0; i < max_iters; ++i) {
+        // one outer iteration, no host syncs: replayed as a graph from the
+        // second iteration on
+        auto outer_iter = [&]() -> int {
             k_sp_zero1<<<1, 64, 0, st>>>(delta.as<double>());
             // s_t: MTTKRP of the stream mode, solve with Had_{m != s} c_m + 1e-12 I
             ALTO_TRY(launch_fill_zero(A[s].as<double>(), nnzr[s] * R, st));
-            ALTO_TRY(run_mttkrp(&at, s, prec, R, fptr, A[s].p, 1, 2, 0, -1, st));
+            ALTO_TRY(run_mttkrp(&at, s, prec, R, fptr, A[s].p, 1, 2, 0, -1, st, false, true));
             ALTO_TRY(launch_chol_inv(c.as<double>(), N, s, R, 1e-12, false, vinv.as<double>(),
                                      iflag.as<int>(), nullptr, st, 1, nullptr));
             ALTO_TRY(launch_solve(A[s].as<double>(), nnzr[s], R, vinv.as<double>(), part.as<double>(),
@@ -456,7 +458,7 @@ extern "C" int alto_hip_spcpstream(const alto_hip_coo *coo, int s, int R, int ma
                 ALTO_TRY(launch_chol_inv(c.as<double>(), N, m, R, 1e-12, false, vinv.as<double>(),
                                          iflag.as<int>(), nullptr, st, 1, nullptr));
                 ALTO_TRY(launch_fill_zero(A[m].as<double>(), n * R, st));
-                ALTO_TRY(run_mttkrp(&at, m, prec, R, fptr, A[m].p, 1, 2, 0, -1, st));
+                ALTO_TRY(run_mttkrp(&at, m, prec, R, fptr, A[m].p, 1, 2, 0, -1, st, false, true));
                 const unsigned nb = (unsigned)std::min<uint64_t>(cdiv(n, TR), 2048);
                 k_sp_rows<<<nb, 256, lds_rows, st>>>(A[m].as<double>(), Ap[m].as<double>(), mat(Q, m),
                                                      vinv.as<double>(), n, R, TR);
@@ -469,6 +471,11 @@ extern "C" int alto_hip_spcpstream(const alto_hip_coo *coo, int s, int R, int ma
                 ALTO_LAUNCH_CHECK();
                 ALTO_TRY(refresh_f32(m));
             }
+            return ALTO_OK;
+        };
+        IterGraph ig(st);
+        for (int i = 0; i < max_iters; ++i) {
+            ALTO_TRY(ig.run(i, outer_iter));
             double hd = 0.0;
             ALTO_HIP_TRY(hipMemcpyAsync(&hd, delta.p, sizeof(double), hipMemcpyDeviceToHost, st));
             ALTO_HIP_TRY(hipStreamSynchronize(st));

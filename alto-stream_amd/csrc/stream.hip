@@ -117,10 +117,17 @@ __global__ __launch_bounds__(256) void k_st_label(const uint32_t *__restrict__ h
     out[pos[j]] = Fs[run_first[run]];
 }
 
-__global__ __launch_bounds__(256) void k_st_time_hist(const uint64_t *__restrict__ t, uint64_t n,
-                                                      uint32_t *__restrict__ cnt) {
+// start[t] = first position of time index t in the sorted time indices (run
+// heads); time indices without nonzeros keep the fill value.
+__global__ __launch_bounds__(256) void k_st_time_starts(const uint64_t *__restrict__ t, uint64_t n,
+                                                        uint64_t *__restrict__ start) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) atomicAdd(cnt + t[i], 1u);
+    if (i < n && (i == 0 || t[i] != t[i - 1])) start[t[i]] = i;
+}
+
+__global__ __launch_bounds__(256) void k_st_fill_u64(uint64_t *__restrict__ p, uint64_t n, uint64_t v) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 __global__ __launch_bounds__(256) void k_st_at(const uint32_t *__restrict__ Fs,
@@ -240,22 +247,24 @@ int stream_prepare(const alto_hip_coo *coo, int s, StreamInput *out, hipStream_t
                   (unsigned long long)n);
         return ALTO_ERR_EMPTY_SLICE;
     }
-    std::vector<uint32_t> hc;
+    std::vector<uint64_t> hs(T);
     {
-        DevBuf cnt;
-        ALTO_TRY(cnt.alloc(sizeof(uint32_t) * T));
-        ALTO_HIP_TRY(hipMemsetAsync(cnt.p, 0, sizeof(uint32_t) * T, st));
-        k_st_time_hist<<<cdiv(n, 256), 256, 0, st>>>(out->cid[s].as<uint64_t>(), n, cnt.as<uint32_t>());
+        DevBuf starts;
+        ALTO_TRY(starts.alloc(sizeof(uint64_t) * T));
+        k_st_fill_u64<<<cdiv(T, 256), 256, 0, st>>>(starts.as<uint64_t>(), T, UINT64_MAX);
+        k_st_time_starts<<<cdiv(n, 256), 256, 0, st>>>(out->cid[s].as<uint64_t>(), n, starts.as<uint64_t>());
         ALTO_LAUNCH_CHECK();
-        hc.resize(T);
-        ALTO_HIP_TRY(hipMemcpyAsync(hc.data(), cnt.p, sizeof(uint32_t) * T, hipMemcpyDeviceToHost, st));
+        ALTO_HIP_TRY(hipMemcpyAsync(hs.data(), starts.p, sizeof(uint64_t) * T, hipMemcpyDeviceToHost, st));
         ALTO_HIP_TRY(hipStreamSynchronize(st));
     }
-    for (uint64_t t = 0; t < T; ++t)
-        if (hc[t] == 0) {
+    std::vector<uint32_t> hc(T);
+    for (uint64_t t = 0; t < T; ++t) {
+        if (hs[t] == UINT64_MAX) {
             set_error("time slice %llu has no nonzeros", (unsigned long long)t);
             return ALTO_ERR_EMPTY_SLICE;
         }
+        hc[t] = (uint32_t)((t + 1 < T ? hs[t + 1] : n) - hs[t]);
+    }
     std::vector<uint64_t> ends(T);
     out->slices.assign(T, Slice{});
     {

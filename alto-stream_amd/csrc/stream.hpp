@@ -30,6 +30,44 @@ struct StreamInput {
 // the reference asserts).
 int stream_prepare(const alto_hip_coo *coo, int s, StreamInput *out, hipStream_t st);
 
+// Runs the enqueue of one outer iteration directly the first time and, from
+// the second time on, as one captured graph replayed on the same stream (the
+// iterations of a slice are identical launches over the same buffers).  The
+// executable graph is destroyed only after the stream has drained.
+struct IterGraph {
+    hipStream_t s = nullptr;
+    hipGraphExec_t ge = nullptr;
+    explicit IterGraph(hipStream_t st) : s(st) {}
+    IterGraph(const IterGraph &) = delete;
+    IterGraph &operator=(const IterGraph &) = delete;
+    ~IterGraph() {
+        if (ge) {
+            (void)hipStreamSynchronize(s);
+            (void)hipGraphExecDestroy(ge);
+        }
+    }
+    template <typename F>
+    int run(int iter, F enqueue) {
+        if (iter == 0) return enqueue();
+        if (!ge) {
+            hipGraph_t g = nullptr;
+            ALTO_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            const int st = enqueue();
+            const hipError_t ce = hipStreamEndCapture(s, &g);
+            if (st != ALTO_OK || ce != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                if (st != ALTO_OK) return st;
+                ALTO_HIP_TRY(ce);
+            }
+            const hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            ALTO_HIP_TRY(ie);
+        }
+        ALTO_HIP_TRY(hipGraphLaunch(ge, s));
+        return ALTO_OK;
+    }
+};
+
 // cpd_error (src/streaming_cpd.cpp:368-464) over device COO and factors;
 // out[4]: err, ||X||^2, ||M||^2, <X, M>.
 int cpd_error_dev(int N, const uint64_t *rows, uint64_t nnz, const uint64_t *const *cidx,

@@ -74,10 +74,11 @@ int ensure_vals32(TensorImpl *t, hipStream_t s);
 int ensure_schedule(TensorImpl *t, int mode, bool need64, hipStream_t s);
 // out += MTTKRP of `target` (algo 1: ALTO order, 2: output-ordered schedule;
 // per_group 0 and flags -1: defaults, autotuned for large tensors).
-// prepare_only: build the schedule and tune, launch nothing.
+// prepare_only: build the schedule and tune, launch nothing.  out_zeroed:
+// the output holds zeros (rows owned by one range are stored, not added).
 int run_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const *factors,
                void *out, int out_f64, int algo, uint64_t per_group, int flags, hipStream_t s,
-               bool prepare_only = false);
+               bool prepare_only = false, bool out_zeroed = false);
 int prepare_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const *factors,
                    int out_f64, hipStream_t s);
 void fiber_levels(const TensorImpl *t, int mode, int *lvl);

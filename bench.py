@@ -72,6 +72,8 @@ def main():
     ap.add_argument("--c5-tensors", type=int, default=512, help="batched CP-ALS: number of tensors")
     ap.add_argument("--probe", action="store_true",
                     help="random 128-byte row gather rate vs table size (the MTTKRP's miss-service bound)")
+    ap.add_argument("--leg", choices=["c1", "c3", "c4", "c5"],
+                    help="run one other-workload leg alone and print its JSON (for profiling)")
     ap.add_argument("--counters", action="store_true",
                     help="untimed pass for rocprofv3 --pmc: each mode twice in FP32, BF16 and FP64")
     args = ap.parse_args()
@@ -82,6 +84,11 @@ def main():
     import alto_amd as A
     A.init(0)
     dev = A.device_name()
+    if args.leg:
+        legs = {"c1": lambda: leg_c1(A, not args.no_cpu), "c3": lambda: leg_c3(A),
+                "c4": lambda: leg_c4(A, args.c4_slices), "c5": lambda: leg_c5(A, args.c5_tensors)}
+        print(json.dumps(legs[args.leg]()), flush=True)
+        return
     if args.probe:
         # 128-byte row gathers: uniform rows over growing tables (L2, MALL, HBM),
         # and Zipf(1.1) rows as the workload's modes draw them, in both lane layouts

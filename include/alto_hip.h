@@ -133,7 +133,9 @@ int alto_hip_mttkrp_algo(alto_hip_tensor t, int target, int prec, int rank,
  * segment = entries per lane group (0 = from the entry count, 4..64);
  * flags bit 0 = non-temporal key/value stream loads, bit 1 = reuse the
  * gathered product across equal consecutive keys, bit 3 = no decode-once
- * kernel, bit 4 = no fiber kernel (-1 = defaults: bits 0 and 1).
+ * kernel, bit 4 = no fiber kernel, bit 5 = the output holds zeros on entry
+ * (algo 2: rows owned by one range are stored instead of added; the caller
+ * guarantees the zeros) (-1 = defaults: bits 0 and 1).
  * No reference counterpart (the reference fixes its launch shape). */
 int alto_hip_mttkrp_config(alto_hip_tensor t, int target, int prec, int rank,
                            const void *const *factors, void *out, int out_f64,

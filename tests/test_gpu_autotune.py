@@ -53,7 +53,7 @@ def test_autotuned_default_matches_oracle(gpu, big, prec, tol):
 
 
 @pytest.mark.parametrize("segment", [1, 4, 16, 64, 128])
-@pytest.mark.parametrize("flags", [-1, 3 | 16, 3 | 8, 0])
+@pytest.mark.parametrize("flags", [-1, 3 | 16, 3 | 8, 0, 3 | 32, 3 | 16 | 32])
 def test_explicit_knobs_match_oracle(gpu, big, segment, flags):
     dims, cidx, vals = big
     R = 16
