@@ -302,6 +302,82 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned nb) {
     return x < r ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
 }
 
+// Combines one workgroup's range partials: slot e = 0 .. 2G-1 holds group
+// e >> 1's first (e even) or last (e odd) row partial, in output order, with
+// row 0xFFFFFFFF for an empty slot.  Consecutive equal rows are summed; the
+// workgroup's first and last rows may continue in the neighbouring
+// workgroups (atomic adds), the rows between are owned by this workgroup.
+// Two levels, so no thread walks all 2G slots: P = 256 / R threads per
+// column each reduce 2G / P consecutive slots and flush the runs that start
+// and end inside their chunk (owned: neither the workgroup's first nor its
+// last row); one thread per column then merges the chunks' first and last
+// runs in order.  Requires a 256-thread workgroup.
+template <typename OT, typename CT, int R, int G>
+__device__ __forceinline__ void block_combine(const uint32_t (*s_row)[G], const CT (*s_part)[G][R], OT *out,
+                                              bool zeroed) {
+    constexpr uint32_t NONE = 0xFFFFFFFFu;
+    constexpr int P = 256 / R, S = 2 * G, C = S / P;
+    static_assert(256 % R == 0 && S % P == 0, "combine tiling");
+    __shared__ uint32_t m_row[2][R][P];
+    __shared__ CT m_sum[2][R][P];
+    const int col = threadIdx.x / P, j = threadIdx.x % P;
+    OT *o = out + col;
+    uint32_t frow = NONE, crow = NONE;
+    CT fsum = 0, csum = 0;
+    int runs = 0;
+#pragma unroll
+    for (int k = 0; k < C; ++k) {
+        const int e = j * C + k;
+        const uint32_t row = s_row[e & 1][e >> 1];
+        if (row == NONE) continue;
+        const CT x = s_part[e & 1][e >> 1][col];
+        if (row == crow) {
+            csum += x;
+            continue;
+        }
+        if (runs == 1) {
+            frow = crow;
+            fsum = csum;
+        } else if (runs > 1) {
+            flush_owned(o + (uint64_t)crow * R, csum, zeroed);
+        }
+        crow = row;
+        csum = x;
+        ++runs;
+    }
+    if (runs == 1) {  // a single run: it is the chunk's first
+        frow = crow;
+        fsum = csum;
+        crow = NONE;
+    }
+    m_row[0][col][j] = frow;
+    m_sum[0][col][j] = fsum;
+    m_row[1][col][j] = crow;
+    m_sum[1][col][j] = csum;
+    __syncthreads();
+    if (j != 0) return;
+    uint32_t mrow = NONE;
+    CT a = 0;
+    bool first = true;
+    for (int k = 0; k < 2 * P; ++k) {
+        const uint32_t row = m_row[k & 1][col][k >> 1];
+        if (row == NONE) continue;
+        const CT x = m_sum[k & 1][col][k >> 1];
+        if (row == mrow) {
+            a += x;
+        } else {
+            if (mrow != NONE) {
+                if (first) atomic_add_out(o + (uint64_t)mrow * R, (OT)a);
+                else flush_owned(o + (uint64_t)mrow * R, a, zeroed);
+                first = false;
+            }
+            mrow = row;
+            a = x;
+        }
+    }
+    if (mrow != NONE) atomic_add_out(o + (uint64_t)mrow * R, (OT)a);
+}
+
 template <typename VT, typename FT, typename OT, int R, int PKW, int NM1, int VPL, bool ZEROED>
 __global__ __launch_bounds__(256) void k_mttkrp_dec(const uint64_t *__restrict__ pkeys,
                                                     const VT *__restrict__ vals, uint64_t nnz,
@@ -462,30 +538,7 @@ __global__ __launch_bounds__(256) void k_mttkrp_dec(const uint64_t *__restrict__
         s_row[1][gib] = cur;
     }
     __syncthreads();
-    if (threadIdx.x < R) {
-        const int r = threadIdx.x;
-        OT *o = out + r;
-        uint32_t crow = NONE;
-        CT a = 0;
-        bool first_merged = true;
-        for (int e = 0; e < 2 * G; ++e) {
-            uint32_t row = s_row[e & 1][e >> 1];
-            if (row == NONE) continue;
-            CT x = s_part[e & 1][e >> 1][r];
-            if (row == crow) {
-                a += x;
-            } else {
-                if (crow != NONE) {
-                    if (first_merged) atomic_add_out(o + (uint64_t)crow * R, (OT)a);
-                    else flush_owned(o + (uint64_t)crow * R, a, zeroed);
-                    first_merged = false;
-                }
-                crow = row;
-                a = x;
-            }
-        }
-        if (crow != NONE) atomic_add_out(o + (uint64_t)crow * R, (OT)a);
-    }
+    block_combine<OT, CT, R, G>(s_row, s_part, out, zeroed);
 }
 
 // Fiber variant of k_mttkrp_dec for output-ordered schedules whose rows are
@@ -675,30 +728,7 @@ __global__ __launch_bounds__(256) void k_mttkrp_fib(const uint64_t *__restrict__
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
-    if (threadIdx.x < R) {
-        const int r = threadIdx.x;
-        OT *o = out + r;
-        uint32_t crow = NONE;
-        CT a = 0;
-        bool first_merged = true;
-        for (int e = 0; e < 2 * G; ++e) {
-            uint32_t row = s_row[e & 1][e >> 1];
-            if (row == NONE) continue;
-            CT x = s_part[e & 1][e >> 1][r];
-            if (row == crow) {
-                a += x;
-            } else {
-                if (crow != NONE) {
-                    if (first_merged) atomic_add_out(o + (uint64_t)crow * R, (OT)a);
-                    else flush_owned(o + (uint64_t)crow * R, a, zeroed);
-                    first_merged = false;
-                }
-                crow = row;
-                a = x;
-            }
-        }
-        if (crow != NONE) atomic_add_out(o + (uint64_t)crow * R, (OT)a);
-    }
+    block_combine<OT, CT, R, G>(s_row, s_part, out, zeroed);
 }
 
 // elements per lane of the decode kernel: 8 FP32 / BF16, 4 FP64 (<= R)
