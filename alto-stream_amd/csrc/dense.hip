@@ -266,75 +266,73 @@ __global__ __launch_bounds__(64) void k_chol_inv_w(const double *__restrict__ gr
     if (t == 0 && flag) *flag = 0;
 }
 
-__device__ __forceinline__ double wave_bcast(double v, int lane) {
-    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-    return __hiloint2double(hi, lo);
-}
-
-// Register-resident variant for R in {8, 16, 32, 64}: lane i holds row i of
-// the matrix (A[R]) and column i of L^{-1} (x[R]); other lanes' values come
-// from v_readlane with compile-time lanes, so nothing waits on LDS.  Same
-// arithmetic as k_chol_inv_w: Cholesky updates A[a][b] -= A[a][j] * A[b][j]
-// (j ascending), column k of L^{-1} by forward substitution in order, and
-// vinv[a][b] = sum_{k >= max(a, b)} X[k][a] X[k][b] in ascending k.
+// Inverse of Phi + shift I (Phi = Hadamard of the Grams of the modes other
+// than `mode`) for R in {8, 16, 32, 64}: in-place Gauss-Jordan elimination
+// without pivoting on the symmetric matrix built from Phi's row-major upper
+// triangle (the triangle the reference's Cholesky reads), in LDS with 256
+// threads, R steps of three barriers.  For a symmetric positive definite
+// matrix the pivots are the squares of the Cholesky diagonal, so the
+// factorisation fails (pivot not > 0) exactly where the Cholesky would; the
+// fallbacks are those of k_chol_inv_w.  The result is symmetrised.
 template <int R>
-__global__ __launch_bounds__(64) void k_chol_inv_r(const double *__restrict__ grams, int nmode,
-                                                   int mode, double diag_add,
-                                                   double *__restrict__ vinv,
-                                                   int *__restrict__ flag,
-                                                   double *__restrict__ phi_out, int fail_mode,
-                                                   double *__restrict__ rho_out) {
-    const int t = threadIdx.x;
-    const bool own = t < R;
+__global__ __launch_bounds__(256) void k_inv_gj(const double *__restrict__ grams, int nmode, int mode,
+                                                double diag_add, double *__restrict__ vinv,
+                                                int *__restrict__ flag, double *__restrict__ phi_out,
+                                                int fail_mode, double *__restrict__ rho_out) {
     constexpr int RR = R * R;
-    double A[R];
-#pragma unroll
-    for (int c = 0; c < R; ++c) {
-        double v = 1.0;
-        if (own)
-            for (int m = 0; m < nmode; ++m)
-                if (m != mode) v *= grams[(size_t)m * RR + t * R + c];
-        A[c] = v;
-        if (own && phi_out) phi_out[t * R + c] = v;
-    }
-    double dgl = 0.0;  // this lane's diagonal entry
-#pragma unroll
-    for (int c = 0; c < R; ++c)
-        if (c == t) dgl = A[c];
-    double shift = diag_add;
-    if (rho_out) {  // ADMM penalty: rho = tr(Phi) / R (admm.cpp:254), summed in row order
-        double tr = 0.0;
-#pragma unroll
-        for (int i = 0; i < R; ++i) tr += wave_bcast(dgl, i);
-        shift = tr / (double)R;
-        if (t == 0) *rho_out = shift;
-    }
-#pragma unroll
-    for (int c = 0; c < R; ++c)
-        if (c == t) A[c] += shift;
-    bool fail = false;
-#pragma unroll
-    for (int j = 0; j < R; ++j) {
-        if (!fail) {
-            const double d = wave_bcast(A[j], j);
-            if (!(d > 0.0)) {
-                fail = true;
-            } else {
-                const double l = sqrt(d);
-                if (t == j) A[j] = l;
-                if (t > j) A[j] /= l;
-#pragma unroll
-                for (int b = j + 1; b < R; ++b) {
-                    const double abj = wave_bcast(A[j], b);
-                    if (b <= t) A[b] -= A[j] * abj;
-                }
+    __shared__ double A[R][R + 1];
+    __shared__ double col[R];
+    __shared__ double sh[2];  // shift, pivot
+    __shared__ int bad;
+    const int t = threadIdx.x;
+    for (int e = t; e < RR; e += 256) {
+        const int i = e / R, j = e % R;
+        const int a = i < j ? i : j, b = i < j ? j : i;
+        double v = 1.0, w = 1.0;
+        for (int m = 0; m < nmode; ++m)
+            if (m != mode) {
+                v *= grams[(size_t)m * RR + a * R + b];
+                w *= grams[(size_t)m * RR + e];
             }
+        A[i][j] = v;
+        if (phi_out) phi_out[e] = w;
+    }
+    __syncthreads();
+    if (t == 0) {
+        double shift = diag_add;
+        if (rho_out) {  // ADMM penalty: rho = tr(Phi) / R (admm.cpp:254), summed in row order
+            double tr = 0.0;
+            for (int k = 0; k < R; ++k) tr += A[k][k];
+            shift = tr / (double)R;
+            *rho_out = shift;
+        }
+        sh[0] = shift;
+        bad = 0;
+    }
+    __syncthreads();
+    if (t < R) A[t][t] += sh[0];
+    for (int k = 0; k < R; ++k) {
+        __syncthreads();
+        if (t == 0) {
+            const double p = A[k][k];
+            if (!(p > 0.0)) bad = 1;
+            sh[1] = p;
+        }
+        if (t < R) col[t] = A[t][k];
+        __syncthreads();
+        if (bad) break;
+        const double p = sh[1];
+        if (t < R) A[k][t] = (t == k ? 1.0 : A[k][t]) / p;
+        __syncthreads();
+        for (int e = t; e < RR; e += 256) {
+            const int i = e / R, j = e % R;
+            if (i != k) A[i][j] = (j == k ? 0.0 : A[i][j]) - col[i] * A[k][j];
         }
     }
-    if (fail) {
+    __syncthreads();
+    if (bad) {
         if (fail_mode == 1) {
-            for (int e = t; e < RR; e += 64) vinv[e] = (e / R == e % R) ? 1.0 : 0.0;
+            for (int e = t; e < RR; e += 256) vinv[e] = (e / R == e % R) ? 1.0 : 0.0;
         } else {
             // reference fallback (cpd.hpp:438-476): minimum-norm solve of the
             // unregularised diagonal
@@ -346,7 +344,7 @@ __global__ __launch_bounds__(64) void k_chol_inv_r(const double *__restrict__ gr
                 mx = fabs(v) > mx ? fabs(v) : mx;
             }
             const double tol = mx * R * DBL_EPSILON;
-            for (int e = t; e < RR; e += 64) {
+            for (int e = t; e < RR; e += 256) {
                 const int i = e / R, jj = e % R;
                 double v = 0.0;
                 if (i == jj) {
@@ -361,29 +359,9 @@ __global__ __launch_bounds__(64) void k_chol_inv_r(const double *__restrict__ gr
         if (t == 0 && flag) *flag = 1;
         return;
     }
-    // column t of L^{-1}: x[i] = (delta_it - sum_{p=t}^{i-1} L[i][p] x[p]) / L[i][i]
-    double x[R];
-#pragma unroll
-    for (int i = 0; i < R; ++i) {
-        double sacc = (i == t) ? 1.0 : 0.0;
-#pragma unroll
-        for (int p = 0; p < i; ++p) {
-            const double lip = wave_bcast(A[p], i);
-            if (p >= t) sacc -= lip * x[p];
-        }
-        const double lii = wave_bcast(A[i], i);
-        x[i] = (i >= t) ? sacc / lii : 0.0;
-    }
-    // row t of vinv
-#pragma unroll
-    for (int b = 0; b < R; ++b) {
-        double sacc = 0.0;
-#pragma unroll
-        for (int k = b; k < R; ++k) {
-            const double xkb = wave_bcast(x[k], b);
-            if (k >= t) sacc += x[k] * xkb;
-        }
-        if (own) vinv[t * R + b] = sacc;
+    for (int e = t; e < RR; e += 256) {
+        const int i = e / R, j = e % R;
+        vinv[e] = 0.5 * (A[i][j] + A[j][i]);
     }
     if (t == 0 && flag) *flag = 0;
 }
@@ -402,17 +380,13 @@ int launch_chol_inv(const double *grams, int nmode, int mode, int R, double diag
     });
     if (R == 8 || R == 16 || R == 32 || R == 64) {
         if (R == 8)
-            k_chol_inv_r<8><<<1, 64, 0, s>>>(grams, nmode, mode, diag_add, vinv, flag, phi_out,
-                                             fail_mode, rho_out);
+            k_inv_gj<8><<<1, 256, 0, s>>>(grams, nmode, mode, diag_add, vinv, flag, phi_out, fail_mode, rho_out);
         else if (R == 16)
-            k_chol_inv_r<16><<<1, 64, 0, s>>>(grams, nmode, mode, diag_add, vinv, flag, phi_out,
-                                              fail_mode, rho_out);
+            k_inv_gj<16><<<1, 256, 0, s>>>(grams, nmode, mode, diag_add, vinv, flag, phi_out, fail_mode, rho_out);
         else if (R == 32)
-            k_chol_inv_r<32><<<1, 64, 0, s>>>(grams, nmode, mode, diag_add, vinv, flag, phi_out,
-                                              fail_mode, rho_out);
+            k_inv_gj<32><<<1, 256, 0, s>>>(grams, nmode, mode, diag_add, vinv, flag, phi_out, fail_mode, rho_out);
         else
-            k_chol_inv_r<64><<<1, 64, 0, s>>>(grams, nmode, mode, diag_add, vinv, flag, phi_out,
-                                              fail_mode, rho_out);
+            k_inv_gj<64><<<1, 256, 0, s>>>(grams, nmode, mode, diag_add, vinv, flag, phi_out, fail_mode, rho_out);
     } else if (R <= 64) {
         const size_t lds = sizeof(double) * 2 * (size_t)R * R;
         k_chol_inv_w<<<1, 64, lds, s>>>(grams, nmode, mode, R, diag_add, vinv, flag, phi_out,
