@@ -304,6 +304,17 @@ def cpd_alto(at, factors, rank, max_iters, eps=1e-30, prec=PREC_F64, fit_mode=FI
     return fs, lam, fits[:n], fits_exact[:n], tim
 
 
+def pseudo_inverse(grams, mode, U, stream=None):
+    """pseudo_inverse (cpd.hpp:377-480) on the GPU: returns (U V^-1, fallback)."""
+    g = np.ascontiguousarray(np.stack([np.asarray(x, dtype=np.float64) for x in grams]))
+    U = np.ascontiguousarray(U, dtype=np.float64).copy()
+    fb = ctypes.c_int()
+    check(lib().alto_hip_pseudo_inverse(ctypes.c_int(g.shape[0]), ctypes.c_int(U.shape[1]), g.ctypes.data_as(_f64p),
+                                        ctypes.c_int(mode), ctypes.c_uint64(U.shape[0]), U.ctypes.data_as(_f64p),
+                                        ctypes.byref(fb), _vp(stream)), "pseudo_inverse")
+    return U, fb.value
+
+
 def cpd_alto_batch(tensors, factors, ranks, max_iters, eps=1e-30, prec=PREC_F64, fit_mode=FIT_REFERENCE_DIAG,
                    nstreams=8):
     """CP-ALS on many independent ALTO tensors concurrently

@@ -468,3 +468,36 @@ extern "C" int alto_hip_cpd_als_batch(int ntensors, const alto_hip_tensor *tenso
     }
     return ALTO_OK;
 }
+
+// pseudo_inverse(grams, M, mode) (cpd.hpp:377-480): U <- U (Had_{m != mode}
+// G_m + 1e-3 I)^{-1} row by row; when the factorisation fails, the
+// minimum-norm solve of the unregularised diagonal.  Host arrays.
+extern "C" int alto_hip_pseudo_inverse(int nmode, int rank, const double *grams, int mode, uint64_t rows,
+                                       double *U, int *fallback, void *stream) {
+    if (nmode < 1 || nmode > kMaxModes || rank < 1 || rank > kMaxRank || mode < 0 || mode >= nmode || !grams ||
+        (!U && rows))
+        return ALTO_ERR_ARG;
+    ALTO_TRY(ensure_init());
+    hipStream_t s = to_stream(stream);
+    const size_t RR = (size_t)rank * rank;
+    DevBuf g, u, vinv, flag, part;
+    ALTO_TRY(g.alloc(sizeof(double) * nmode * RR));
+    ALTO_TRY(u.alloc(sizeof(double) * (rows ? rows : 1) * rank));
+    ALTO_TRY(vinv.alloc(sizeof(double) * RR));
+    ALTO_TRY(flag.alloc(sizeof(int)));
+    const int nb = row_blocks(rows ? rows : 1);
+    ALTO_TRY(part.alloc(sizeof(double) * 3 * rank * nb));
+    ALTO_HIP_TRY(hipMemcpyAsync(g.p, grams, sizeof(double) * nmode * RR, hipMemcpyHostToDevice, s));
+    if (rows) ALTO_HIP_TRY(hipMemcpyAsync(u.p, U, sizeof(double) * rows * rank, hipMemcpyHostToDevice, s));
+    ALTO_TRY(launch_chol_inv(g.as<double>(), nmode, mode, rank, 1e-3, true, vinv.as<double>(), flag.as<int>(),
+                             nullptr, s));
+    if (rows) {
+        ALTO_TRY(launch_solve(u.as<double>(), rows, rank, vinv.as<double>(), part.as<double>(), nb, s));
+        ALTO_HIP_TRY(hipMemcpyAsync(U, u.p, sizeof(double) * rows * rank, hipMemcpyDeviceToHost, s));
+    }
+    int hf = 0;
+    ALTO_HIP_TRY(hipMemcpyAsync(&hf, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    ALTO_HIP_TRY(hipStreamSynchronize(s));
+    if (fallback) *fallback = hf;
+    return ALTO_OK;
+}

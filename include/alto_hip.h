@@ -187,6 +187,15 @@ int alto_hip_cpd_als_batch(int ntensors, const alto_hip_tensor *tensors,
                            double *const *lambdas, double *const *fits, int *iters,
                            int nstreams, double *timings);
 
+/* ---- pseudo_inverse (include/cpd.hpp:377-480): U <- U V^{-1} row by row with
+ * V = Hadamard of grams[m], m != mode (nmode consecutive rank x rank
+ * row-major matrices; the row-major upper triangle is read, as DPOTRF('L')
+ * does) plus 1e-3 on the diagonal; when V is not positive definite, the
+ * minimum-norm solve of the unregularised diagonal and *fallback = 1.
+ * Host arrays: grams, U (rows x rank, in place). */
+int alto_hip_pseudo_inverse(int nmode, int rank, const double *grams, int mode, uint64_t rows,
+                            double *U, int *fallback, void *stream);
+
 /* ---- model error: cpd_error (src/streaming_cpd.cpp:368-464) with
  * kruskal_norm (src/kruskal_model.cpp:400-442) ----
  * err = sqrt(||X||^2 + ||M||^2 - 2 <X, M>) / ||X|| for the Kruskal model of
