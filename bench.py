@@ -75,7 +75,7 @@ def main():
     ap.add_argument("--leg", choices=["c1", "c3", "c4", "c5"],
                     help="run one other-workload leg alone and print its JSON (for profiling)")
     ap.add_argument("--counters", action="store_true",
-                    help="untimed pass for rocprofv3 --pmc: each mode twice in FP32, BF16 and FP64")
+                    help="untimed pass for rocprofv3 --pmc: warm-up, a calibration copy, then each mode once in FP32, BF16 and FP64")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -122,13 +122,26 @@ def main():
     rng = np.random.default_rng(W["seed"])
     F64 = [rng.random((d, R)) for d in dims]
     if args.counters:
+        # Warm-up call of every (precision, mode): schedules and launch tuning.
+        # Then the calibration copy, k_f64_to_f32 over 16,000,000 elements
+        # (128,000,000 B read, 64,000,000 B written), which also marks where the
+        # measured calls start: one per precision (FP32, BF16, FP64) and mode.
+        sets = []
         for p in (A.PREC_F32, A.PREC_BF16, A.PREC_F64):
             dF = [A.to_prec(f, p) for f in F64]
             os_ = [A.DeviceArray.zeros((d, R), np.float64 if p == A.PREC_F64 else np.float32) for d in dims]
-            for _ in range(2):
-                for n in range(N):
-                    A.mttkrp_alto_par(n, dF, R, at, os_[n], prec=p, out_f64=False, algo=args.algo)
-            A.synchronize()
+            for n in range(N):
+                A.mttkrp_alto_par(n, dF, R, at, os_[n], prec=p, out_f64=False, algo=args.algo)
+            sets.append((p, dF, os_))
+        A.synchronize()
+        src = A.DeviceArray.zeros(16_000_000, np.float64)
+        dst = A.DeviceArray.zeros(16_000_000, np.float32)
+        A.convert(dst, A.PREC_F32, src, A.PREC_F64, 16_000_000)
+        A.synchronize()
+        for p, dF, os_ in sets:
+            for n in range(N):
+                A.mttkrp_alto_par(n, dF, R, at, os_[n], prec=p, out_f64=False, algo=args.algo)
+        A.synchronize()
         print(json.dumps({"counters_pass": True}), flush=True)
         return
     devF = [A.to_prec(f, prec) for f in F64]
