@@ -10,16 +10,44 @@ constexpr int kDenseThreads = 256;
 constexpr int kMaxBlocks = 2048;  // row blocks per reduction (partials buffer size)
 constexpr int kMaxRank = 128;
 
-// One wave per output entry: lanes sum (op 0) or max (op 1) part[b * stride +
-// off] over blocks b = lane, lane + 64, ..., then a fixed shuffle tree; the
-// result is valid in lane 0.  Deterministic for a given nblk.
+// One wave per output entry: lane l sums (op 0) or maxes (op 1)
+// part[b * stride + off] over blocks b = l, l + 64, ... into four
+// accumulators taken in turn (four loads in flight), combined as
+// (a0 + a1) + (a2 + a3), then a fixed shuffle tree; the result is valid in
+// lane 0.  Deterministic for a given nblk.
 __device__ __forceinline__ double wave_reduce_entry(const double *__restrict__ part, int nblk,
                                                     size_t stride, size_t off, int op) {
     const int lane = threadIdx.x & 63;
-    double a = op == 0 ? 0.0 : -INFINITY;
-    for (int b = lane; b < nblk; b += 64) {
-        double v = part[(size_t)b * stride + off];
-        a = op == 0 ? a + v : (v > a ? v : a);
+    const double init = op == 0 ? 0.0 : -INFINITY;
+    double a0 = init, a1 = init, a2 = init, a3 = init;
+    int b = lane;
+    for (; b + 192 < nblk; b += 256) {
+        const double v0 = part[(size_t)b * stride + off];
+        const double v1 = part[(size_t)(b + 64) * stride + off];
+        const double v2 = part[(size_t)(b + 128) * stride + off];
+        const double v3 = part[(size_t)(b + 192) * stride + off];
+        if (op == 0) {
+            a0 += v0;
+            a1 += v1;
+            a2 += v2;
+            a3 += v3;
+        } else {
+            a0 = v0 > a0 ? v0 : a0;
+            a1 = v1 > a1 ? v1 : a1;
+            a2 = v2 > a2 ? v2 : a2;
+            a3 = v3 > a3 ? v3 : a3;
+        }
+    }
+    for (; b < nblk; b += 64) {
+        const double v = part[(size_t)b * stride + off];
+        a0 = op == 0 ? a0 + v : (v > a0 ? v : a0);
+    }
+    double a;
+    if (op == 0) {
+        a = (a0 + a1) + (a2 + a3);
+    } else {
+        const double m01 = a1 > a0 ? a1 : a0, m23 = a3 > a2 ? a3 : a2;
+        a = m23 > m01 ? m23 : m01;
     }
     for (int o = 32; o > 0; o >>= 1) {
         double v = __shfl_down(a, o, 64);
