@@ -267,23 +267,24 @@ __global__ __launch_bounds__(64) void k_chol_inv_w(const double *__restrict__ gr
 }
 
 // Inverse of Phi + shift I (Phi = Hadamard of the Grams of the modes other
-// than `mode`) for R in {8, 16, 32, 64}: in-place Gauss-Jordan elimination
-// without pivoting on the symmetric matrix built from Phi's row-major upper
-// triangle (the triangle the reference's Cholesky reads), in LDS with 256
-// threads, R steps of three barriers.  For a symmetric positive definite
-// matrix the pivots are the squares of the Cholesky diagonal, so the
-// factorisation fails (pivot not > 0) exactly where the Cholesky would; the
-// fallbacks are those of k_chol_inv_w.  The result is symmetrised.
+// than `mode`) for R in {8, 16, 32, 64}: Gauss-Jordan elimination without
+// pivoting on the symmetric matrix built from Phi's row-major upper triangle
+// (the triangle the reference's Cholesky reads), 256 threads, two LDS copies
+// used in turn so that each of the R steps reads one and writes the other
+// behind a single barrier.  For a symmetric positive definite matrix the
+// pivots are the squares of the Cholesky diagonal, so the elimination fails
+// (pivot not > 0) exactly where the Cholesky would; every thread reads the
+// same pivot, so the decision is uniform.  Fallbacks as k_chol_inv_w; the
+// result is symmetrised.  Dynamic LDS: 2 R (R + 1) doubles.
 template <int R>
 __global__ __launch_bounds__(256) void k_inv_gj(const double *__restrict__ grams, int nmode, int mode,
                                                 double diag_add, double *__restrict__ vinv,
                                                 int *__restrict__ flag, double *__restrict__ phi_out,
                                                 int fail_mode, double *__restrict__ rho_out) {
-    constexpr int RR = R * R;
-    __shared__ double A[R][R + 1];
-    __shared__ double col[R];
-    __shared__ double sh[2];  // shift, pivot
-    __shared__ int bad;
+    constexpr int RR = R * R, LD = R + 1;
+    extern __shared__ double lds[];
+    double *cur = lds, *nxt = lds + R * LD;
+    __shared__ double shift_s;
     const int t = threadIdx.x;
     for (int e = t; e < RR; e += 256) {
         const int i = e / R, j = e % R;
@@ -292,9 +293,9 @@ __global__ __launch_bounds__(256) void k_inv_gj(const double *__restrict__ grams
         for (int m = 0; m < nmode; ++m)
             if (m != mode) {
                 v *= grams[(size_t)m * RR + a * R + b];
-                w *= grams[(size_t)m * RR + e];
+                if (phi_out) w *= grams[(size_t)m * RR + e];
             }
-        A[i][j] = v;
+        cur[i * LD + j] = v;
         if (phi_out) phi_out[e] = w;
     }
     __syncthreads();
@@ -302,34 +303,33 @@ __global__ __launch_bounds__(256) void k_inv_gj(const double *__restrict__ grams
         double shift = diag_add;
         if (rho_out) {  // ADMM penalty: rho = tr(Phi) / R (admm.cpp:254), summed in row order
             double tr = 0.0;
-            for (int k = 0; k < R; ++k) tr += A[k][k];
+            for (int k = 0; k < R; ++k) tr += cur[k * LD + k];
             shift = tr / (double)R;
             *rho_out = shift;
         }
-        sh[0] = shift;
-        bad = 0;
+        shift_s = shift;
     }
     __syncthreads();
-    if (t < R) A[t][t] += sh[0];
+    if (t < R) cur[t * LD + t] += shift_s;
+    __syncthreads();
+    bool bad = false;
     for (int k = 0; k < R; ++k) {
-        __syncthreads();
-        if (t == 0) {
-            const double p = A[k][k];
-            if (!(p > 0.0)) bad = 1;
-            sh[1] = p;
+        const double p = cur[k * LD + k];
+        if (!(p > 0.0)) {
+            bad = true;
+            break;
         }
-        if (t < R) col[t] = A[t][k];
-        __syncthreads();
-        if (bad) break;
-        const double p = sh[1];
-        if (t < R) A[k][t] = (t == k ? 1.0 : A[k][t]) / p;
-        __syncthreads();
+        const double ip = 1.0 / p;
         for (int e = t; e < RR; e += 256) {
             const int i = e / R, j = e % R;
-            if (i != k) A[i][j] = (j == k ? 0.0 : A[i][j]) - col[i] * A[k][j];
+            const double rk = (j == k ? 1.0 : cur[k * LD + j]) * ip;  // new row k
+            nxt[i * LD + j] = i == k ? rk : (j == k ? 0.0 : cur[i * LD + j]) - cur[i * LD + k] * rk;
         }
+        __syncthreads();
+        double *tmp = cur;
+        cur = nxt;
+        nxt = tmp;
     }
-    __syncthreads();
     if (bad) {
         if (fail_mode == 1) {
             for (int e = t; e < RR; e += 256) vinv[e] = (e / R == e % R) ? 1.0 : 0.0;
@@ -361,7 +361,7 @@ __global__ __launch_bounds__(256) void k_inv_gj(const double *__restrict__ grams
     }
     for (int e = t; e < RR; e += 256) {
         const int i = e / R, j = e % R;
-        vinv[e] = 0.5 * (A[i][j] + A[j][i]);
+        vinv[e] = 0.5 * (cur[i * LD + j] + cur[j * LD + i]);
     }
     if (t == 0 && flag) *flag = 0;
 }
@@ -377,16 +377,19 @@ int launch_chol_inv(const double *grams, int nmode, int mode, int R, double diag
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 256);
         (void)hipFuncSetAttribute((const void *)k_chol_inv_w,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 256);
+        (void)hipFuncSetAttribute((const void *)k_inv_gj<64>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 256);
     });
     if (R == 8 || R == 16 || R == 32 || R == 64) {
+        const size_t lds = sizeof(double) * 2 * (size_t)R * (R + 1);
         if (R == 8)
-            k_inv_gj<8><<<1, 256, 0, s>>>(grams, nmode, mode, diag_add, vinv, flag, phi_out, fail_mode, rho_out);
+            k_inv_gj<8><<<1, 256, lds, s>>>(grams, nmode, mode, diag_add, vinv, flag, phi_out, fail_mode, rho_out);
         else if (R == 16)
-            k_inv_gj<16><<<1, 256, 0, s>>>(grams, nmode, mode, diag_add, vinv, flag, phi_out, fail_mode, rho_out);
+            k_inv_gj<16><<<1, 256, lds, s>>>(grams, nmode, mode, diag_add, vinv, flag, phi_out, fail_mode, rho_out);
         else if (R == 32)
-            k_inv_gj<32><<<1, 256, 0, s>>>(grams, nmode, mode, diag_add, vinv, flag, phi_out, fail_mode, rho_out);
+            k_inv_gj<32><<<1, 256, lds, s>>>(grams, nmode, mode, diag_add, vinv, flag, phi_out, fail_mode, rho_out);
         else
-            k_inv_gj<64><<<1, 256, 0, s>>>(grams, nmode, mode, diag_add, vinv, flag, phi_out, fail_mode, rho_out);
+            k_inv_gj<64><<<1, 256, lds, s>>>(grams, nmode, mode, diag_add, vinv, flag, phi_out, fail_mode, rho_out);
     } else if (R <= 64) {
         const size_t lds = sizeof(double) * 2 * (size_t)R * R;
         k_chol_inv_w<<<1, 64, lds, s>>>(grams, nmode, mode, R, diag_add, vinv, flag, phi_out,
