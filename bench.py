@@ -33,14 +33,14 @@ def log(*a):
 
 
 def workload(name):
-    if name == "c2":
+    if name == "zipf4":
         return dict(dims=[500000, 200000, 50000, 1000], nnz=100_000_000, seed=0xA1700001, rank=32,
                     mode_dist=[1, 1, 1, 1], zipf=[1.1] * 4, vdist=1, p1=0.0, p2=1.0,
                     desc="4-mode 500000x200000x50000x1000, 1e8 nnz, Zipf(1.1) indices, lognormal(0,1) values, rank 32")
-    if name == "c2s":  # reduced shape for quick checks (not a bench line)
+    if name == "zipf4_small":  # reduced shape for quick checks (not a bench line)
         return dict(dims=[50000, 20000, 5000, 100], nnz=2_000_000, seed=0xA1700001, rank=32,
                     mode_dist=[1, 1, 1, 1], zipf=[1.1] * 4, vdist=1, p1=0.0, p2=1.0,
-                    desc="reduced c2 shape")
+                    desc="reduced shape of the 4-mode Zipf workload")
     raise SystemExit("unknown workload " + name)
 
 
@@ -59,7 +59,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--workload", default="zipf4")
     ap.add_argument("--prec", default="f32", choices=["f32", "bf16", "f64"])
     ap.add_argument("--algo", type=int, default=0)
     ap.add_argument("--als-iters", type=int, default=10)
@@ -68,11 +68,11 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=25_000_000)
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the other workloads (plain CP-ALS, 5-mode BF16, CP-stream sample)")
-    ap.add_argument("--c4-slices", type=int, default=24, help="CP-stream sample: number of time slices")
-    ap.add_argument("--c5-tensors", type=int, default=512, help="batched CP-ALS: number of tensors")
+    ap.add_argument("--stream-slices", type=int, default=24, help="CP-stream sample: number of time slices")
+    ap.add_argument("--batch-tensors", type=int, default=512, help="batched CP-ALS: number of tensors")
     ap.add_argument("--probe", action="store_true",
                     help="random 128-byte row gather rate vs table size (the MTTKRP's miss-service bound)")
-    ap.add_argument("--leg", choices=["c1", "c3", "c4", "c5"],
+    ap.add_argument("--leg", choices=["plain_als", "bf16_5mode", "cp_stream", "batched_als"],
                     help="run one other-workload leg alone and print its JSON (for profiling)")
     ap.add_argument("--counters", action="store_true",
                     help="untimed pass for rocprofv3 --pmc: warm-up, a calibration copy, then each mode once in FP32, BF16 and FP64")
@@ -85,8 +85,9 @@ def main():
     A.init(0)
     dev = A.device_name()
     if args.leg:
-        legs = {"c1": lambda: leg_c1(A, not args.no_cpu), "c3": lambda: leg_c3(A),
-                "c4": lambda: leg_c4(A, args.c4_slices), "c5": lambda: leg_c5(A, args.c5_tensors)}
+        legs = {"plain_als": lambda: leg_plain_als(A, not args.no_cpu), "bf16_5mode": lambda: leg_bf16_5mode(A),
+                "cp_stream": lambda: leg_cp_stream(A, args.stream_slices),
+                "batched_als": lambda: leg_batched_als(A, args.batch_tensors)}
         print(json.dumps(legs[args.leg]()), flush=True)
         return
     if args.probe:
@@ -221,10 +222,10 @@ def main():
         result["cpu_baseline"] = cpu_baseline(A, at, dims, R, F64, args.cpu_sample)
     at.close()
     if not args.no_extra:
-        result["other_workloads"] = {"c1_plain_cp_als": leg_c1(A, not args.no_cpu),
-                                     "c3_5mode_bf16": leg_c3(A),
-                                     "c4_cp_stream": leg_c4(A, args.c4_slices),
-                                     "c5_batched_cp_als": leg_c5(A, args.c5_tensors)}
+        result["other_workloads"] = {"plain_cp_als_1000x800x600": leg_plain_als(A, not args.no_cpu),
+                                     "bf16_5mode_5e7": leg_bf16_5mode(A),
+                                     "cp_stream_200000x100000": leg_cp_stream(A, args.stream_slices),
+                                     "batched_cp_als_512": leg_batched_als(A, args.batch_tensors)}
     print(json.dumps(result), flush=True)
 
 
@@ -232,9 +233,9 @@ def measured_traffic(workload, prec):
     """HBM bytes per MTTKRP launch (FETCH_SIZE + WRITE_SIZE, scaled by a copy
     kernel of known size) from the committed rocprofv3 summary of this
     workload, or None when there is none."""
-    if workload != "c2":
+    if workload != "zipf4":
         return None
-    path = os.path.join(ROOT, "profiles", "mttkrp_c2_rank32.md")
+    path = os.path.join(ROOT, "profiles", "mttkrp_zipf4mode_rank32.md")
     key = "traffic_bytes_per_launch_%s:" % {"f32": "fp32", "bf16": "bf16", "f64": "fp64"}[prec]
     try:
         for line in open(path):
@@ -286,12 +287,12 @@ def cpu_baseline(A, at, dims, R, F64, sample):
 
 
 
-def leg_c1(A, with_cpu):
+def leg_plain_als(A, with_cpu):
     """Plain CP-ALS as the reference runs it: 3-mode 1000x800x600, 1e6
     distinct uniform coordinates, U(0,1) values, rank 16, 20 iterations;
     plus 100 mode-0 MTTKRPs (the reference's -p benchmark)."""
     dims, nnz, R = [1000, 800, 600], 1_000_000, 16
-    log("C1: plain CP-ALS, 1000x800x600, 1e6 nnz, rank 16")
+    log("plain CP-ALS, 1000x800x600, 1e6 nnz, rank 16")
     rng = np.random.default_rng(0xA1700000)
     total = dims[0] * dims[1] * dims[2]
     lin = np.unique(rng.integers(0, total, size=int(nnz * 1.05)))
@@ -330,19 +331,19 @@ def leg_c1(A, with_cpu):
         out["cpu_port_cp_als_s_per_iter"] = round(dt, 5)
         out["cpu_cores"] = threads
         out["cpu_fit_after_5"] = float(rfits[-1])
-    log("C1: CP-ALS f64 %.5f s/iter, f32 %.5f s/iter, MTTKRP mode 0 %.4f ms" %
+    log("plain CP-ALS: f64 %.5f s/iter, f32 %.5f s/iter, MTTKRP mode 0 %.4f ms" %
         (out["cp_als_f64_s_per_iter"], out["cp_als_f32_s_per_iter"], out["mttkrp_mode0_f64_ms"]))
     return out
 
 
-def leg_c3(A):
+def leg_bf16_5mode(A):
     """5-mode 2^20 x 2^20 x 2^18 x 2^16 x 2^12, 5e7 nnz (70% uniform, 30% in a
     256^5 sub-block), N(0,1) values, rank 64, 128-bit keys: all-mode MTTKRP
     with BF16-stored factors (FP32 accumulate) and FP64, and 10 CP-ALS
     iterations BF16 vs FP64 with the fit difference."""
     dims, nnz, R = [1 << 20, 1 << 20, 1 << 18, 1 << 16, 1 << 12], 50_000_000, 64
     seed = 0xA1700003
-    log("C3: 5-mode, 5e7 nnz, rank 64, BF16 / FP64")
+    log("5-mode 2^20x2^20x2^18x2^16x2^12, 5e7 nnz, rank 64, BF16 / FP64")
     t0 = time.time()
     cidx, vals = A.generate(dims, nnz, seed, [0] * 5, None, 2, 0.0, 1.0, block_frac=0.3, block=256)
     at = A.create_alto(dims, cidx, vals, nprtn=256, on_device=True)
@@ -385,20 +386,20 @@ def leg_c3(A):
     out["cp_als_speedup_bf16_vs_f64"] = round(out["cp_als_f64_s_per_iter"] / out["cp_als_bf16_s_per_iter"], 3)
     out["fit_abs_diff_bf16_vs_f64"] = abs(fits["bf16"] - fits["f64"])
     at.close()
-    log("C3: MTTKRP bf16 %.3f ms, f64 %.3f ms all-mode; CP-ALS bf16 %.4f s/iter, f64 %.4f s/iter, |dfit| %.2e" %
+    log("5-mode: MTTKRP bf16 %.3f ms, f64 %.3f ms all-mode; CP-ALS bf16 %.4f s/iter, f64 %.4f s/iter, |dfit| %.2e" %
         (out["mttkrp_bf16_ms_all_mode"], out["mttkrp_f64_ms_all_mode"], out["cp_als_bf16_s_per_iter"],
          out["cp_als_f64_s_per_iter"], out["fit_abs_diff_bf16_vs_f64"]))
     return out
 
 
-def leg_c4(A, slices):
+def leg_cp_stream(A, slices):
     """CP-stream with non-negative ADMM on the first `slices` time slices of
     the 1000-slice workload: slices of 200000 x 100000, per-slice nnz
     lognormal(ln 2e5, 1) clipped to [2e4, 2e6], Zipf(1.05) indices,
     Poisson(3)+1 values, rank 16, at most 10 outer iterations per slice;
     FP32 MTTKRP, FP64 ADMM/solve."""
     seed = 0xA1700004
-    log("C4: CP-stream, %d slices" % slices)
+    log("CP-stream 200000x100000, %d slices" % slices)
     rng = np.random.default_rng(seed)
     sizes = np.clip(np.round(rng.lognormal(np.log(2e5), 1.0, 1000)), 2e4, 2e6).astype(np.int64)[:slices]
     nnz = int(sizes.sum())
@@ -422,7 +423,7 @@ def leg_c4(A, slices):
            "wall_s_incl_upload_and_stream_prep": round(wall, 3),
            "outer_iters_mean": round(float(np.mean(res["outer_iters"])), 2),
            "admm_iters_mean": round(float(np.mean(res["admm_iters"])), 2)}
-    log("C4: %d slices in %.3f s (%.2f slices/s), wall %.1f s" % (slices, tim[0], slices / tim[0], wall))
+    log("CP-stream: %d slices in %.3f s (%.2f slices/s), wall %.1f s" % (slices, tim[0], slices / tim[0], wall))
     # spCP-stream on the same stream (the reference's SPCPSTREAM_ALTO: row-sparse
     # updates, no constraint), beside CP-stream without the constraint
     sp = A.spcpstream(dims, hc, hv, 0, R, 10, 1e-6, init, prec=A.PREC_F32)
@@ -434,12 +435,12 @@ def leg_c4(A, slices):
     out["cpstream_unconstrained"] = {"slices_per_s": round(slices / tu, 2), "slice_loop_s": round(tu, 3),
                                      "outer_iters_mean": round(float(np.mean(un["outer_iters"])), 2),
                                      "final_error": un["final_error"]}
-    log("C4: spCP-stream %.2f slices/s (%.2f outer iterations), CP-stream unconstrained %.2f slices/s (%.2f)" %
+    log("CP-stream: spCP-stream %.2f slices/s (%.2f outer iterations), CP-stream unconstrained %.2f slices/s (%.2f)" %
         (slices / ts, np.mean(sp["outer_iters"]), slices / tu, np.mean(un["outer_iters"])))
     return out
 
 
-def leg_c5(A, ntensors):
+def leg_batched_als(A, ntensors):
     """Batched CP-ALS of uneven instances: 3-mode tensors, each mode size
     uniform in [500, 50000], nnz log-uniform in [1e4, 5e6], uniform
     coordinates, U(0,1) values; ranks 8/16/32 round robin; 25 iterations
@@ -448,7 +449,7 @@ def leg_c5(A, ntensors):
     streams and, for comparison, on 1 stream (the tensors one after
     another)."""
     seed = 0xA1700005
-    log("C5: batched CP-ALS, %d tensors" % ntensors)
+    log("batched CP-ALS, %d tensors" % ntensors)
     rng = np.random.default_rng(seed)
     ats, Fs, ranks, total = [], [], [], 0
     for k in range(ntensors):
@@ -474,7 +475,7 @@ def leg_c5(A, ntensors):
     out["unit_nnz_iterations"] = "1e9 nonzero-iterations/s (nnz x 25 ALS iterations / wall)"
     for a in ats:
         a.close()
-    log("C5: %d tensors (%d nnz): %.3f s on 16 streams, %.3f s on 8, %.3f s on 1" %
+    log("batched CP-ALS: %d tensors (%d nnz): %.3f s on 16 streams, %.3f s on 8, %.3f s on 1" %
         (ntensors, total, out["wall_s_16_streams"], out["wall_s_8_streams"], out["wall_s_1_streams"]))
     return out
 
