@@ -464,6 +464,63 @@ __global__ __launch_bounds__(256) void k_solve(double *__restrict__ M, uint64_t 
     }
 }
 
+// k_solve for R in {8, 16, 32, 64}: thread (rl, c) keeps column c of vinv in
+// registers, so the inner product reads only the (broadcast) tile row from
+// LDS.  Same tiles, arithmetic order and partials as k_solve.
+template <int R>
+__global__ __launch_bounds__(256) void k_solve_r(double *__restrict__ M, uint64_t rows,
+                                                 const double *__restrict__ vinv,
+                                                 double *__restrict__ partials) {
+    constexpr int lanes = 256 / R, TR = lanes * 4;
+    __shared__ double T[TR * R > 768 ? TR * R : 768];
+    const int t = threadIdx.x;
+    const int c = t % R, rl = t / R;
+    double w[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) w[k] = vinv[k * R + c];
+    double ss = 0.0, mx = -INFINITY, ip = 0.0;
+    const uint64_t ntiles = (rows + TR - 1) / TR;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const uint64_t r0 = tile * TR;
+        const int nr = (int)((rows - r0) < (uint64_t)TR ? (rows - r0) : TR);
+        __syncthreads();
+        for (int e = t; e < nr * R; e += 256) T[e] = M[r0 * R + e];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int rr = rl + q * lanes;
+            if (rr < nr) {
+                const double *x = T + rr * R;
+                double s = 0.0;
+#pragma unroll
+                for (int k = 0; k < R; ++k) s += x[k] * w[k];
+                M[(r0 + rr) * R + c] = s;
+                ss += s * s;
+                mx = s > mx ? s : mx;
+                ip += s * x[c];
+            }
+        }
+    }
+    __syncthreads();
+    double *red = T;  // 3 * 256 doubles
+    red[t] = ss;
+    red[256 + t] = mx;
+    red[512 + t] = ip;
+    __syncthreads();
+    if (rl == 0) {
+        for (int l = 1; l < lanes; ++l) {
+            ss += red[l * R + c];
+            const double m2 = red[256 + l * R + c];
+            mx = m2 > mx ? m2 : mx;
+            ip += red[512 + l * R + c];
+        }
+        double *p = partials + (size_t)blockIdx.x * 3 * R;
+        p[c] = ss;
+        p[R + c] = mx;
+        p[2 * R + c] = ip;
+    }
+}
+
 int launch_solve(double *M, uint64_t rows, int R, const double *vinv, double *partials, int nblk,
                  hipStream_t s) {
     if (R > kMaxRank) return ALTO_ERR_UNSUPPORTED;
@@ -477,7 +534,11 @@ int launch_solve(double *M, uint64_t rows, int R, const double *vinv, double *pa
         (void)hipFuncSetAttribute((const void *)k_solve, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024 - 256);
     });
-    k_solve<<<nblk, 256, lds, s>>>(M, rows, R, RP, vinv, partials);
+    if (R == 8) k_solve_r<8><<<nblk, 256, 0, s>>>(M, rows, vinv, partials);
+    else if (R == 16) k_solve_r<16><<<nblk, 256, 0, s>>>(M, rows, vinv, partials);
+    else if (R == 32) k_solve_r<32><<<nblk, 256, 0, s>>>(M, rows, vinv, partials);
+    else if (R == 64) k_solve_r<64><<<nblk, 256, 0, s>>>(M, rows, vinv, partials);
+    else k_solve<<<nblk, 256, lds, s>>>(M, rows, R, RP, vinv, partials);
     ALTO_LAUNCH_CHECK();
     return ALTO_OK;
 }
@@ -547,21 +608,31 @@ __global__ __launch_bounds__(256) void k_scale_gram(const double *__restrict__ u
             T[e] = v;
         }
         __syncthreads();
+        // thread t owns entries t * NQ .. t * NQ + NQ - 1; when they share
+        // one row a (R a multiple of NQ), T[r][a] is read once per row
+        const int e0 = t * NQ;
+        if (R % NQ == 0 && e0 < RR) {
+            const int a = e0 / R, b0 = e0 % R;
+            for (int r = 0; r < nr; ++r) {
+                const double xa = T[r * R + a];
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            int e = t + q * 256;
-            if (e < RR) {
-                int a = e / R, b = e % R;
-                double s = g[q];
-                for (int r = 0; r < nr; ++r) s += T[r * R + a] * T[r * R + b];
-                g[q] = s;
+                for (int q = 0; q < NQ; ++q) g[q] += xa * T[r * R + b0 + q];
+            }
+        } else if (R % NQ != 0) {
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const int e = e0 + q;
+                if (e < RR) {
+                    const int a = e / R, b = e % R;
+                    for (int r = 0; r < nr; ++r) g[q] += T[r * R + a] * T[r * R + b];
+                }
             }
         }
     }
     double *p = gpart + (size_t)blockIdx.x * RR;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-        int e = t + q * 256;
+        const int e = t * NQ + q;
         if (e < RR) p[e] = g[q];
     }
 }
@@ -619,21 +690,31 @@ __global__ __launch_bounds__(256) void k_cross_gram(const double *__restrict__ P
             TU[e] = U[r0 * R + e];
         }
         __syncthreads();
+        // thread t owns entries t * NQ .. t * NQ + NQ - 1; when they share
+        // one row a (R a multiple of NQ), TP[r][a] is read once per row
+        const int e0 = t * NQ;
+        if (R % NQ == 0 && e0 < RR) {
+            const int a = e0 / R, b0 = e0 % R;
+            for (int r = 0; r < nr; ++r) {
+                const double pa = TP[r * R + a];
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            int e = t + q * 256;
-            if (e < RR) {
-                int a = e / R, b = e % R;
-                double sacc = g[q];
-                for (int r = 0; r < nr; ++r) sacc += TP[r * R + a] * TU[r * R + b];
-                g[q] = sacc;
+                for (int q = 0; q < NQ; ++q) g[q] += pa * TU[r * R + b0 + q];
+            }
+        } else if (R % NQ != 0) {
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const int e = e0 + q;
+                if (e < RR) {
+                    const int a = e / R, b = e % R;
+                    for (int r = 0; r < nr; ++r) g[q] += TP[r * R + a] * TU[r * R + b];
+                }
             }
         }
     }
     double *p = gpart + (size_t)blockIdx.x * RR;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-        int e = t + q * 256;
+        const int e = t * NQ + q;
         if (e < RR) p[e] = g[q];
     }
 }

@@ -207,7 +207,7 @@ def main():
 
     if not args.no_als:
         als = {}
-        for name, p in (("f32", A.PREC_F32), ("f64", A.PREC_F64)):
+        for name, p in (("f32", A.PREC_F32), ("bf16", A.PREC_BF16), ("f64", A.PREC_F64)):
             t0 = time.time()
             _, lam, fits, fe, tim = A.cpd_alto(at, F64, R, args.als_iters, eps=1e-30, prec=p)
             als[name] = dict(s_per_iter=round(float(tim[7]), 5),
@@ -216,6 +216,9 @@ def main():
         als["speedup_f32_vs_f64"] = round(als["f64"]["s_per_iter"] / als["f32"]["s_per_iter"], 3)
         als["dfit"] = abs(als["f32"]["fit"] - als["f64"]["fit"])
         als["dfit_rel"] = als["dfit"] / max(abs(als["f64"]["fit"]), 1e-300)
+        als["speedup_bf16_vs_f64"] = round(als["f64"]["s_per_iter"] / als["bf16"]["s_per_iter"], 3)
+        als["dfit_bf16"] = abs(als["bf16"]["fit"] - als["f64"]["fit"])
+        als["dfit_rel_bf16"] = als["dfit_bf16"] / max(abs(als["f64"]["fit"]), 1e-300)
         result["cp_als"] = als
 
     if not args.no_cpu:
