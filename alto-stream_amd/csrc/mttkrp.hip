@@ -905,7 +905,7 @@ static int dispatch_prec(const MttkrpArgs &a, hipStream_t s) {
 // Autotuner: for the default configuration of an output-ordered MTTKRP with
 // at least kTuneMinEntries entries, the first call of each bucket (modes,
 // rank, precision, output type, key words, log2 of the entry count, log2 of
-// the output rows) times the candidates -- segment length {p/2, p, 2p}
+// the output rows) times the candidates -- segment length {p/4, p/2, p, 2p}
 // around the heuristic p, fiber or decode-once kernel -- best of three runs
 // each on a scratch output.  The heuristic choice is kept unless a candidate
 // is more than 3% faster.  ALTO_AUTOTUNE=0 in the environment turns it off.
@@ -957,7 +957,15 @@ static int autotune(TensorImpl *t, MttkrpArgs &a, hipStream_t s) {
     MttkrpArgs c = a;
     c.out = scratch.p;
     const uint64_t p0 = a.per_group;
-    const uint64_t pgs[3] = {p0, p0 > 4 ? p0 / 2 : 2, p0 * 2 < 128 ? p0 * 2 : 128};
+    // segment lengths p/4, p/2, p, 2p (at least 2, at most 128; repeats dropped)
+    uint64_t pgs[4];
+    int npg = 0;
+    for (uint64_t cand : {p0 / 4, p0 / 2, p0, p0 * 2}) {
+        cand = cand < 2 ? 2 : cand > 128 ? 128 : cand;
+        bool seen = false;
+        for (int i = 0; i < npg; ++i) seen = seen || pgs[i] == cand;
+        if (!seen) pgs[npg++] = cand;
+    }
     int st = ALTO_OK;
     // best of three runs of one configuration
     auto time_cfg = [&](uint64_t pg, bool fib) -> float {
@@ -980,8 +988,10 @@ static int autotune(TensorImpl *t, MttkrpArgs &a, hipStream_t s) {
     const float base_ms = time_cfg(a.per_group, a.fiber);
     float best = base_ms;
     for (int fib = 0; fib < 2 && st == ALTO_OK; ++fib) {
-        if (fib && a.prec == ALTO_PREC_BF16) continue;  // no fiber kernel for BF16 rows
-        for (int i = 0; i < 3 && st == ALTO_OK; ++i) {
+        // BF16 rows: the fiber walk measured about 1.5x slower than decode-once
+        // on a 4-mode Zipf tensor (1e8 entries, rank 32), so it is not timed
+        if (fib && a.prec == ALTO_PREC_BF16) continue;
+        for (int i = 0; i < npg && st == ALTO_OK; ++i) {
             if (pgs[i] == a.per_group && (fib != 0) == a.fiber) continue;
             const float ms = time_cfg(pgs[i], fib != 0);
             if (st == ALTO_OK && ms < best && ms < 0.97f * base_ms) {
@@ -1058,9 +1068,9 @@ int run_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const 
         if (a.nnz == 0) return ALTO_OK;
         a.pkeys = sc.pkeys.as<uint64_t>();
         a.vals = prec == ALTO_PREC_F64 ? sc.vals64.p : sc.vals32.p;
-        // fiber kernel: FP64 and FP32 factors (BF16 rows are cheaper to gather
-        // per nonzero than the fiber bookkeeping)
-        a.fiber = a.decode_once && !(a.flags & 16) && prec != ALTO_PREC_BF16;
+        // fiber kernel: FP64 and FP32 factors by default, BF16 only on request
+        // (flags bit 6) or by the tuner's timing
+        a.fiber = a.decode_once && !(a.flags & 16) && (prec != ALTO_PREC_BF16 || (a.flags & 64));
         auto bits = [&](int m, uint64_t *lo, uint64_t *hi) {
             const int sh = t->pack.shift[m], w = t->pack.width[m];
             unsigned __int128 mk = w ? ((((unsigned __int128)1) << w) - 1) << sh : 0;

@@ -135,7 +135,8 @@ int alto_hip_mttkrp_algo(alto_hip_tensor t, int target, int prec, int rank,
  * gathered product across equal consecutive keys, bit 3 = no decode-once
  * kernel, bit 4 = no fiber kernel, bit 5 = the output holds zeros on entry
  * (algo 2: rows owned by one range are stored instead of added; the caller
- * guarantees the zeros) (-1 = defaults: bits 0 and 1).
+ * guarantees the zeros), bit 6 = fiber kernel also for BF16 factors
+ * (-1 = defaults: bits 0 and 1).
  * No reference counterpart (the reference fixes its launch shape). */
 int alto_hip_mttkrp_config(alto_hip_tensor t, int target, int prec, int rank,
                            const void *const *factors, void *out, int out_f64,
