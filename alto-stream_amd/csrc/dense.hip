@@ -637,9 +637,136 @@ __global__ __launch_bounds__(256) void k_scale_gram(const double *__restrict__ u
     }
 }
 
+// The same on the FP64 matrix cores, R a multiple of 16.  A wave takes four
+// rows at a time: lane l holds U[4c + l / 16][16 I + l % 16] for each
+// 16-column block I, which is both the A operand (U^T, 16 x 4) and the B
+// operand (U, 4 x 16) of v_mfma_f64_16x16x4f64, so block (I, J) of U^T U
+// accumulates as mfma(v[I], v[J]) with no LDS traffic.  Only the blocks
+// I <= J are formed; the partial is mirrored from them, so it is exactly
+// symmetric.  Rows are dealt out four at a time to the grid's waves; the
+// next four rows are loaded while the current ones are multiplied.  The four
+// waves' sums are added in LDS in wave order (deterministic).  MODE 0 also
+// scales u by 1 / lambda and stores the factor in PT (FP64, FP32 or BF16
+// bits), taking the Gram of the stored values; MODE 1 reads U in PT.  u and
+// Uout may be the same array (FP64 in place): each element is read and
+// written by one lane, the read first.
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+template <int MODE, int R, typename PT>
+__global__ __launch_bounds__(256) void k_scale_gram_mfma(const double *u, const double *__restrict__ lambda,
+                                                         const PT *__restrict__ Uin, PT *Uout, uint64_t rows,
+                                                         double *__restrict__ gpart) {
+    static_assert(R % 16 == 0 && R <= 64, "16-column blocks");
+    constexpr int NB = R / 16, NP = NB * (NB + 1) / 2, RR = R * R;
+    using RawT = typename std::conditional<MODE == 0, double, PT>::type;
+    __shared__ double G[RR];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int kr = lane >> 4, c16 = lane & 15;
+    v4d acc[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) acc[p] = v4d{0.0, 0.0, 0.0, 0.0};
+    double lam[NB];
+#pragma unroll
+    for (int I = 0; I < NB; ++I) lam[I] = (MODE == 0 && lambda) ? lambda[16 * I + c16] : 1.0;
+    const uint64_t nchunks = (rows + 3) / 4, stride = (uint64_t)gridDim.x * 4;
+    auto load = [&](uint64_t c, RawT *raw) {
+        const uint64_t row = c * 4 + kr;
+#pragma unroll
+        for (int I = 0; I < NB; ++I) {
+            const uint64_t gi = row * R + 16 * I + c16;
+            if constexpr (MODE == 0) raw[I] = row < rows ? u[gi] : 0.0;
+            else raw[I] = row < rows ? Uin[gi] : (PT)0;
+        }
+    };
+    uint64_t c = (uint64_t)blockIdx.x * 4 + wave;
+    RawT nxt[NB];
+    if (c < nchunks) load(c, nxt);
+    for (; c < nchunks; c += stride) {
+        RawT raw[NB];
+#pragma unroll
+        for (int I = 0; I < NB; ++I) raw[I] = nxt[I];
+        if (c + stride < nchunks) load(c + stride, nxt);
+        const uint64_t row = c * 4 + kr;
+        double v[NB];
+#pragma unroll
+        for (int I = 0; I < NB; ++I) {
+            if constexpr (MODE == 0) {
+                double x = raw[I];
+                if (lambda) x = x / lam[I];
+                const uint64_t gi = row * R + 16 * I + c16;
+                if constexpr (sizeof(PT) == 8) {
+                    if (row < rows) Uout[gi] = x;
+                    v[I] = x;
+                } else if constexpr (sizeof(PT) == 4) {
+                    const float f = (float)x;
+                    if (row < rows) Uout[gi] = f;
+                    v[I] = (double)f;
+                } else {
+                    const uint16_t b = f32_to_bf16((float)x);
+                    if (row < rows) Uout[gi] = b;
+                    v[I] = (double)bf16_to_f32(b);
+                }
+                if (row >= rows) v[I] = 0.0;
+            } else {
+                if constexpr (sizeof(PT) == 2) v[I] = (double)bf16_to_f32(raw[I]);
+                else v[I] = (double)raw[I];
+            }
+        }
+        int p = 0;
+#pragma unroll
+        for (int I = 0; I < NB; ++I)
+#pragma unroll
+            for (int J = I; J < NB; ++J, ++p) acc[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[I], v[J], acc[p], 0, 0, 0);
+    }
+    // D item q of lane l is element (l / 16 + 4 q, l % 16) of the block
+    for (int w = 0; w < 4; ++w) {
+        if (wave == w) {
+            int p = 0;
+#pragma unroll
+            for (int I = 0; I < NB; ++I)
+#pragma unroll
+                for (int J = I; J < NB; ++J, ++p)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int i = kr + 4 * q, j = c16;
+                        if (I == J && i > j) continue;
+                        const int a = 16 * I + i, b = 16 * J + j;
+                        const double val = acc[p][q];
+                        if (w == 0) {
+                            G[a * R + b] = val;
+                            G[b * R + a] = val;
+                        } else {
+                            G[a * R + b] += val;
+                            if (a != b) G[b * R + a] += val;
+                        }
+                    }
+        }
+        __syncthreads();
+    }
+    double *out = gpart + (size_t)blockIdx.x * RR;
+    for (int e = threadIdx.x; e < RR; e += 256) out[e] = G[e];
+}
+
+template <int MODE, int R>
+static void launch_sg_mfma(const double *u, const double *lambda, const void *Uin, void *Uout, int prec,
+                           uint64_t rows, double *gpart, int nblk, hipStream_t s) {
+    if (prec == ALTO_PREC_F64)
+        k_scale_gram_mfma<MODE, R, double><<<nblk, 256, 0, s>>>(u, lambda, (const double *)Uin, (double *)Uout,
+                                                                 rows, gpart);
+    else if (prec == ALTO_PREC_F32)
+        k_scale_gram_mfma<MODE, R, float><<<nblk, 256, 0, s>>>(u, lambda, (const float *)Uin, (float *)Uout,
+                                                                rows, gpart);
+    else
+        k_scale_gram_mfma<MODE, R, uint16_t><<<nblk, 256, 0, s>>>(u, lambda, (const uint16_t *)Uin,
+                                                                   (uint16_t *)Uout, rows, gpart);
+}
+
 template <int MODE>
 static void launch_sg(const double *u, const double *lambda, const void *Uin, void *Uout, int prec,
                       uint64_t rows, int R, double *gpart, int nblk, hipStream_t s) {
+    if (R == 16) return launch_sg_mfma<MODE, 16>(u, lambda, Uin, Uout, prec, rows, gpart, nblk, s);
+    if (R == 32) return launch_sg_mfma<MODE, 32>(u, lambda, Uin, Uout, prec, rows, gpart, nblk, s);
+    if (R == 64) return launch_sg_mfma<MODE, 64>(u, lambda, Uin, Uout, prec, rows, gpart, nblk, s);
     size_t lds = sizeof(double) * 32 * R;
     int nq = (R * R + 255) / 256;
     if (nq <= 1)

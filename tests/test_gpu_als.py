@@ -32,6 +32,23 @@ def test_als_f64_matches_oracle(gpu, fit_mode):
         np.testing.assert_allclose(a, b, rtol=1e-6, atol=1e-9)
 
 
+@pytest.mark.parametrize("rank,rows", [(32, [300, 200, 100]), (64, [500, 300, 130]), (24, [300, 200, 100])])
+def test_als_f64_ranks_match_oracle(gpu, rank, rows):
+    """Ranks 32 and 64 take the matrix-core Gram, 24 the LDS one; the row
+    counts are not multiples of four (partial row groups)."""
+    dims, cidx, vals, F = _setup(rows, 40000, 87, rank, dup=0.05)
+    print("rank", rank, "dims", dims, flush=True)
+    ref = O.AltoTensor(dims, cidx, vals, nprtn=8)
+    rf, rl, rfits, rfe = ref.cpd_als(F, rank, 4, fit_mode=O.FIT_EXACT)
+    at = gpu.create_alto(dims, cidx, vals, nprtn=8)
+    gf, gl, gfits, gfe, _ = gpu.cpd_alto(at, F, rank, 4, prec=gpu.PREC_F64, fit_mode=gpu.FIT_EXACT)
+    np.testing.assert_allclose(gfits, rfits, rtol=0, atol=1e-8)
+    np.testing.assert_allclose(gfe, rfe, rtol=0, atol=1e-8)
+    np.testing.assert_allclose(gl, rl, rtol=1e-6)
+    for a, b in zip(gf, rf):
+        np.testing.assert_allclose(a, b, rtol=1e-6, atol=1e-9)
+
+
 def test_als_f64_fixture_med4(gpu):
     dims, cidx, vals = O.read_tns(golden("med4.tns"))
     F = rand_factors(dims, 8, seed=81)
@@ -42,14 +59,15 @@ def test_als_f64_fixture_med4(gpu):
     np.testing.assert_allclose(gfits, rfits, rtol=0, atol=1e-8)
 
 
+@pytest.mark.parametrize("rank", [16, 32, 64])
 @pytest.mark.parametrize("prec,tol", [(1, 1e-5), (2, 2e-3)])
-def test_als_reduced_precision_fit(gpu, prec, tol):
-    dims, cidx, vals, F = _setup([300, 200, 150, 40], 60000, 82, 32, dup=0.1)
+def test_als_reduced_precision_fit(gpu, prec, tol, rank):
+    dims, cidx, vals, F = _setup([300, 200, 150, 40], 60000, 82, rank, dup=0.1)
     ref = O.AltoTensor(dims, cidx, vals, nprtn=8)
-    _, _, rfits, rfe = ref.cpd_als(F, 32, 5)
+    _, _, rfits, rfe = ref.cpd_als(F, rank, 5)
     at = gpu.create_alto(dims, cidx, vals, nprtn=8)
-    _, _, gfits, gfe, _ = gpu.cpd_alto(at, F, 32, 5, prec=prec)
-    print("prec", prec, "max |dfit|", np.abs(gfits - rfits).max(), flush=True)
+    _, _, gfits, gfe, _ = gpu.cpd_alto(at, F, rank, 5, prec=prec)
+    print("prec", prec, "rank", rank, "max |dfit|", np.abs(gfits - rfits).max(), flush=True)
     assert np.abs(gfits - rfits).max() <= tol
     assert np.abs(gfe - rfe).max() <= tol
 
