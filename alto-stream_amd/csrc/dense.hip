@@ -266,69 +266,130 @@ __global__ __launch_bounds__(64) void k_chol_inv_w(const double *__restrict__ gr
     if (t == 0 && flag) *flag = 0;
 }
 
+// v_readlane of a double: lane `l` (wave-uniform) of x.
+__device__ __forceinline__ double readlane_f64(double x, int l) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // Inverse of Phi + shift I (Phi = Hadamard of the Grams of the modes other
 // than `mode`) for R in {8, 16, 32, 64}: Gauss-Jordan elimination without
 // pivoting on the symmetric matrix built from Phi's row-major upper triangle
-// (the triangle the reference's Cholesky reads), 256 threads, two LDS copies
-// used in turn so that each of the R steps reads one and writes the other
-// behind a single barrier.  For a symmetric positive definite matrix the
-// pivots are the squares of the Cholesky diagonal, so the elimination fails
-// (pivot not > 0) exactly where the Cholesky would; every thread reads the
-// same pivot, so the decision is uniform.  Fallbacks as k_chol_inv_w; the
-// result is symmetrised.  Dynamic LDS: 2 R (R + 1) doubles.
+// (the triangle the reference's Cholesky reads).  For a symmetric positive
+// definite matrix the pivots are the squares of the Cholesky diagonal, so the
+// elimination fails (pivot not > 0) exactly where the Cholesky would; every
+// thread reads the same pivot, so the decision is uniform.  The matrix stays
+// in registers: thread t < min(R^2, 256) owns column t % R for the rows
+// i = t / R (mod C), C = min(R^2, 256) / R.  Step k needs row k (for the
+// thread's column) and column k (for the thread's rows).  Row k's owners
+// publish it to LDS, double-buffered by the parity of k, so each step has one
+// barrier; column k comes from the lane of the same wave that owns column k
+// for the same rows (v_readlane, no LDS).  Fallbacks as k_chol_inv_w; the
+// result is symmetrised.
 template <int R>
 __global__ __launch_bounds__(256) void k_inv_gj(const double *__restrict__ grams, int nmode, int mode,
                                                 double diag_add, double *__restrict__ vinv,
                                                 int *__restrict__ flag, double *__restrict__ phi_out,
                                                 int fail_mode, double *__restrict__ rho_out) {
     constexpr int RR = R * R, LD = R + 1;
-    extern __shared__ double lds[];
-    double *cur = lds, *nxt = lds + R * LD;
+    constexpr int NT = RR < 256 ? RR : 256, C = NT / R, NR = R / C, G = 64 / R;
+    __shared__ double full[R * LD];
+    __shared__ double rowb[2][R];
     __shared__ double shift_s;
     const int t = threadIdx.x;
-    for (int e = t; e < RR; e += 256) {
-        const int i = e / R, j = e % R;
-        const int a = i < j ? i : j, b = i < j ? j : i;
-        double v = 1.0, w = 1.0;
-        for (int m = 0; m < nmode; ++m)
-            if (m != mode) {
-                v *= grams[(size_t)m * RR + a * R + b];
-                if (phi_out) w *= grams[(size_t)m * RR + e];
+    const bool own = t < NT;
+    const int j = t % R, cls = t / R;  // rows cls, cls + C, ...
+    const int grp = (t & 63) / R;
+    // Phi from the upper triangles, mode by mode; each thread's entries load
+    // together (one round trip per mode)
+    constexpr int PE = (RR + 255) / 256;
+    double v[PE], w[PE];
+#pragma unroll
+    for (int q = 0; q < PE; ++q) v[q] = w[q] = 1.0;
+    for (int m = 0; m < nmode; ++m) {
+        if (m == mode) continue;
+        const double *g = grams + (size_t)m * RR;
+#pragma unroll
+        for (int q = 0; q < PE; ++q) {
+            const int e = t + 256 * q;
+            if (e < RR) {
+                const int i = e / R, jj = e % R;
+                const int lo = i < jj ? i : jj, hi = i < jj ? jj : i;
+                v[q] *= g[lo * R + hi];
             }
-        cur[i * LD + j] = v;
-        if (phi_out) phi_out[e] = w;
+        }
+    }
+    if (phi_out) {
+        for (int m = 0; m < nmode; ++m) {
+            if (m == mode) continue;
+            const double *g = grams + (size_t)m * RR;
+#pragma unroll
+            for (int q = 0; q < PE; ++q)
+                if (t + 256 * q < RR) w[q] *= g[t + 256 * q];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < PE; ++q) {
+        const int e = t + 256 * q;
+        if (e < RR) {
+            full[(e / R) * LD + e % R] = v[q];
+            if (phi_out) phi_out[e] = w[q];
+        }
     }
     __syncthreads();
     if (t == 0) {
         double shift = diag_add;
         if (rho_out) {  // ADMM penalty: rho = tr(Phi) / R (admm.cpp:254), summed in row order
             double tr = 0.0;
-            for (int k = 0; k < R; ++k) tr += cur[k * LD + k];
+            for (int k = 0; k < R; ++k) tr += full[k * LD + k];
             shift = tr / (double)R;
             *rho_out = shift;
         }
         shift_s = shift;
     }
     __syncthreads();
-    if (t < R) cur[t * LD + t] += shift_s;
-    __syncthreads();
+    double a[NR];
+#pragma unroll
+    for (int n = 0; n < NR; ++n) {
+        const int i = cls + C * n;
+        a[n] = own ? full[i * LD + j] + (i == j ? shift_s : 0.0) : 0.0;
+    }
     bool bad = false;
     for (int k = 0; k < R; ++k) {
-        const double p = cur[k * LD + k];
+        const int b = k & 1;
+        if (own) {
+            if (k % C == cls) {
+                double v = 0.0;
+#pragma unroll
+                for (int n = 0; n < NR; ++n) v = n == k / C ? a[n] : v;
+                rowb[b][j] = v;
+            }
+        }
+        __syncthreads();
+        const double p = rowb[b][k];
         if (!(p > 0.0)) {
             bad = true;
             break;
         }
-        const double ip = 1.0 / p;
-        for (int e = t; e < RR; e += 256) {
-            const int i = e / R, j = e % R;
-            const double rk = (j == k ? 1.0 : cur[k * LD + j]) * ip;  // new row k
-            nxt[i * LD + j] = i == k ? rk : (j == k ? 0.0 : cur[i * LD + j]) - cur[i * LD + k] * rk;
+        if (own) {
+            const double ip = 1.0 / p;
+            const double rk = (j == k ? 1.0 : rowb[b][j]) * ip;  // new row k
+#pragma unroll
+            for (int n = 0; n < NR; ++n) {
+                // column k of these rows: lane g R + k of this wave, g = this
+                // lane's group of R lanes (R < 64: 64 / R groups per wave)
+                double akn = 0.0;
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+                    const double r = readlane_f64(a[n], g * R + k);
+                    akn = grp == g ? r : akn;
+                }
+                const int i = cls + C * n;
+                a[n] = i == k ? rk : (j == k ? 0.0 : a[n]) - akn * rk;
+            }
         }
-        __syncthreads();
-        double *tmp = cur;
-        cur = nxt;
-        nxt = tmp;
     }
     if (bad) {
         if (fail_mode == 1) {
@@ -359,9 +420,14 @@ __global__ __launch_bounds__(256) void k_inv_gj(const double *__restrict__ grams
         if (t == 0 && flag) *flag = 1;
         return;
     }
+    if (own) {
+#pragma unroll
+        for (int n = 0; n < NR; ++n) full[(cls + C * n) * LD + j] = a[n];
+    }
+    __syncthreads();
     for (int e = t; e < RR; e += 256) {
-        const int i = e / R, j = e % R;
-        vinv[e] = 0.5 * (cur[i * LD + j] + cur[j * LD + i]);
+        const int i = e / R, jj = e % R;
+        vinv[e] = 0.5 * (full[i * LD + jj] + full[jj * LD + i]);
     }
     if (t == 0 && flag) *flag = 0;
 }
@@ -377,19 +443,16 @@ int launch_chol_inv(const double *grams, int nmode, int mode, int R, double diag
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 256);
         (void)hipFuncSetAttribute((const void *)k_chol_inv_w,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 256);
-        (void)hipFuncSetAttribute((const void *)k_inv_gj<64>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 256);
     });
     if (R == 8 || R == 16 || R == 32 || R == 64) {
-        const size_t lds = sizeof(double) * 2 * (size_t)R * (R + 1);
         if (R == 8)
-            k_inv_gj<8><<<1, 256, lds, s>>>(grams, nmode, mode, diag_add, vinv, flag, phi_out, fail_mode, rho_out);
+            k_inv_gj<8><<<1, 256, 0, s>>>(grams, nmode, mode, diag_add, vinv, flag, phi_out, fail_mode, rho_out);
         else if (R == 16)
-            k_inv_gj<16><<<1, 256, lds, s>>>(grams, nmode, mode, diag_add, vinv, flag, phi_out, fail_mode, rho_out);
+            k_inv_gj<16><<<1, 256, 0, s>>>(grams, nmode, mode, diag_add, vinv, flag, phi_out, fail_mode, rho_out);
         else if (R == 32)
-            k_inv_gj<32><<<1, 256, lds, s>>>(grams, nmode, mode, diag_add, vinv, flag, phi_out, fail_mode, rho_out);
+            k_inv_gj<32><<<1, 256, 0, s>>>(grams, nmode, mode, diag_add, vinv, flag, phi_out, fail_mode, rho_out);
         else
-            k_inv_gj<64><<<1, 256, lds, s>>>(grams, nmode, mode, diag_add, vinv, flag, phi_out, fail_mode, rho_out);
+            k_inv_gj<64><<<1, 256, 0, s>>>(grams, nmode, mode, diag_add, vinv, flag, phi_out, fail_mode, rho_out);
     } else if (R <= 64) {
         const size_t lds = sizeof(double) * 2 * (size_t)R * R;
         k_chol_inv_w<<<1, 64, lds, s>>>(grams, nmode, mode, R, diag_add, vinv, flag, phi_out,
@@ -467,8 +530,8 @@ __global__ __launch_bounds__(256) void k_solve(double *__restrict__ M, uint64_t 
 // k_solve for R in {8, 16, 32, 64}: thread (rl, c) keeps column c of vinv in
 // registers, so the inner product reads only the (broadcast) tile row from
 // LDS.  Same tiles, arithmetic order and partials as k_solve.
-template <int R>
-__global__ __launch_bounds__(256) void k_solve_r(double *__restrict__ M, uint64_t rows,
+template <int R, typename MT>
+__global__ __launch_bounds__(256) void k_solve_r(const MT *M, double *U, uint64_t rows,
                                                  const double *__restrict__ vinv,
                                                  double *__restrict__ partials) {
     constexpr int lanes = 256 / R, TR = lanes * 4;
@@ -480,12 +543,24 @@ __global__ __launch_bounds__(256) void k_solve_r(double *__restrict__ M, uint64_
     for (int k = 0; k < R; ++k) w[k] = vinv[k * R + c];
     double ss = 0.0, mx = -INFINITY, ip = 0.0;
     const uint64_t ntiles = (rows + TR - 1) / TR;
+    // the next tile's 4 elements per thread load while this tile is solved
+    static_assert(TR * R == 1024, "four elements per thread and tile");
+    double pre[4];
+    auto fetch = [&](uint64_t tile) {
+        const uint64_t r0 = tile * TR;
+        const int n = (int)((rows - r0) < (uint64_t)TR ? (rows - r0) : TR) * R;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) pre[q] = t + 256 * q < n ? (double)M[r0 * R + t + 256 * q] : 0.0;
+    };
+    if (blockIdx.x < ntiles) fetch(blockIdx.x);
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const uint64_t r0 = tile * TR;
         const int nr = (int)((rows - r0) < (uint64_t)TR ? (rows - r0) : TR);
         __syncthreads();
-        for (int e = t; e < nr * R; e += 256) T[e] = M[r0 * R + e];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) T[t + 256 * q] = pre[q];
         __syncthreads();
+        if (tile + gridDim.x < ntiles) fetch(tile + gridDim.x);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int rr = rl + q * lanes;
@@ -494,7 +569,7 @@ __global__ __launch_bounds__(256) void k_solve_r(double *__restrict__ M, uint64_
                 double s = 0.0;
 #pragma unroll
                 for (int k = 0; k < R; ++k) s += x[k] * w[k];
-                M[(r0 + rr) * R + c] = s;
+                U[(r0 + rr) * R + c] = s;
                 ss += s * s;
                 mx = s > mx ? s : mx;
                 ip += s * x[c];
@@ -534,10 +609,10 @@ int launch_solve(double *M, uint64_t rows, int R, const double *vinv, double *pa
         (void)hipFuncSetAttribute((const void *)k_solve, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024 - 256);
     });
-    if (R == 8) k_solve_r<8><<<nblk, 256, 0, s>>>(M, rows, vinv, partials);
-    else if (R == 16) k_solve_r<16><<<nblk, 256, 0, s>>>(M, rows, vinv, partials);
-    else if (R == 32) k_solve_r<32><<<nblk, 256, 0, s>>>(M, rows, vinv, partials);
-    else if (R == 64) k_solve_r<64><<<nblk, 256, 0, s>>>(M, rows, vinv, partials);
+    if (R == 8) k_solve_r<8, double><<<nblk, 256, 0, s>>>(M, M, rows, vinv, partials);
+    else if (R == 16) k_solve_r<16, double><<<nblk, 256, 0, s>>>(M, M, rows, vinv, partials);
+    else if (R == 32) k_solve_r<32, double><<<nblk, 256, 0, s>>>(M, M, rows, vinv, partials);
+    else if (R == 64) k_solve_r<64, double><<<nblk, 256, 0, s>>>(M, M, rows, vinv, partials);
     else k_solve<<<nblk, 256, lds, s>>>(M, rows, R, RP, vinv, partials);
     ALTO_LAUNCH_CHECK();
     return ALTO_OK;
