@@ -908,7 +908,7 @@ static int dispatch_prec(const MttkrpArgs &a, hipStream_t s) {
 // the output rows) times the candidates -- segment length {p/4, p/2, p, 2p}
 // around the heuristic p, fiber or decode-once kernel -- best of three runs
 // each on a scratch output.  The heuristic choice is kept unless a candidate
-// is more than 3% faster.  ALTO_AUTOTUNE=0 in the environment turns it off.
+// is more than 1% faster.  ALTO_AUTOTUNE=0 in the environment turns it off.
 constexpr uint64_t kTuneMinEntries = 1ULL << 20;
 
 struct TuneKey {
@@ -983,7 +983,7 @@ static int autotune(TensorImpl *t, MttkrpArgs &a, hipStream_t s) {
         }
         return best_ms;
     };
-    // the heuristic configuration stays unless another is more than 3% faster
+    // the heuristic configuration stays unless another is more than 1% faster
     TuneVal pick{a.per_group, a.fiber};
     const float base_ms = time_cfg(a.per_group, a.fiber);
     float best = base_ms;
@@ -994,7 +994,7 @@ static int autotune(TensorImpl *t, MttkrpArgs &a, hipStream_t s) {
         for (int i = 0; i < npg && st == ALTO_OK; ++i) {
             if (pgs[i] == a.per_group && (fib != 0) == a.fiber) continue;
             const float ms = time_cfg(pgs[i], fib != 0);
-            if (st == ALTO_OK && ms < best && ms < 0.97f * base_ms) {
+            if (st == ALTO_OK && ms < best && ms < 0.99f * base_ms) {
                 best = ms;
                 pick = TuneVal{pgs[i], fib != 0};
             }
