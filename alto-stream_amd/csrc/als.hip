@@ -164,10 +164,10 @@ static int als_run(TensorImpl *t, const alto_hip_als_opts *o, double *const *fac
     for (int m = 0; m < N; ++m) {
         uint64_t n = t->dims[m] * R;
         ALTO_HIP_TRY(hipMemcpyAsync(mt.p, factors[m], sizeof(double) * n, hipMemcpyHostToDevice, s));
-        int nb = row_blocks(t->dims[m]);
+        int gb = gram_blocks(t->dims[m], R);
         ALTO_TRY(launch_scale_gram(mt.as<double>(), nullptr, t->dims[m], R, Uf[m].p, prec,
-                                   gpart.as<double>(), nb, s));
-        ALTO_TRY(launch_gram_reduce(gpart.as<double>(), nb, R, grams.as<double>() + (size_t)m * R * R, s));
+                                   gpart.as<double>(), gb, s));
+        ALTO_TRY(launch_gram_reduce(gpart.as<double>(), gb, R, grams.as<double>() + (size_t)m * R * R, s));
     }
     ALTO_TRY(w.sumsq.alloc(sizeof(double) * kSumsqBlocks));
     ALTO_TRY(launch_sumsq(t->vals64.as<double>(), t->nnz, normX.as<double>(), w.sumsq.as<double>(), s));
@@ -197,8 +197,8 @@ static int als_run(TensorImpl *t, const alto_hip_als_opts *o, double *const *fac
                 ALTO_TRY(launch_solve(mt.as<double>(), I, R, vinv.as<double>(), part.as<double>(), nb, s));
                 ALTO_TRY(launch_lambda(part.as<double>(), nb, R, first ? 0 : 1, lam.as<double>(), s));
                 ALTO_TRY(launch_scale_gram(mt.as<double>(), lam.as<double>(), I, R, Uf[n].p, prec,
-                                           gpart.as<double>(), nb, s));
-                ALTO_TRY(launch_gram_reduce(gpart.as<double>(), nb, R,
+                                           gpart.as<double>(), gram_blocks(I, R), s));
+                ALTO_TRY(launch_gram_reduce(gpart.as<double>(), gram_blocks(I, R), R,
                                             grams.as<double>() + (size_t)n * R * R, s));
             }
             const int nb = row_blocks(t->dims[N - 1]);
@@ -305,8 +305,8 @@ static int als_run(TensorImpl *t, const alto_hip_als_opts *o, double *const *fac
             ALTO_TRY(launch_lambda(part.as<double>(), nb, R, it == 0 ? 0 : 1, lam.as<double>(), s));
             ALTO_HIP_TRY(hipEventRecord(E[4], s));
             ALTO_TRY(launch_scale_gram(mt.as<double>(), lam.as<double>(), I, R, Uf[n].p, prec,
-                                       gpart.as<double>(), nb, s));
-            ALTO_TRY(launch_gram_reduce(gpart.as<double>(), nb, R,
+                                       gpart.as<double>(), gram_blocks(I, R), s));
+            ALTO_TRY(launch_gram_reduce(gpart.as<double>(), gram_blocks(I, R), R,
                                         grams.as<double>() + (size_t)n * R * R, s));
             ALTO_HIP_TRY(hipEventRecord(E[5], s));
         }

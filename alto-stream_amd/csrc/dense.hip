@@ -38,6 +38,12 @@ int row_blocks(uint64_t rows) {
     return (int)b;
 }
 
+int gram_blocks(uint64_t rows, int R) {
+    const int b = row_blocks(rows);
+    const int cap = R > 32 ? 512 : R > 16 ? 1280 : kMaxBlocks;
+    return b < cap ? b : cap;
+}
+
 // One workgroup.  Phi = Hadamard of G_m (m != mode, in mode order); A = Phi +
 // diag_add I; Cholesky A = L L^T (lower, right-looking); L^{-1} in place
 // (column sweep from the right, as LAPACK dtrti2); vinv = L^{-T} L^{-1}.

@@ -73,6 +73,10 @@ int launch_cross_gram(const double *P, const double *U, uint64_t rows, int R, do
 int launch_colsumsq(const double *U, uint64_t rows, int R, double *partials, int nblk,
                     hipStream_t s);
 int row_blocks(uint64_t rows);
+// Row blocks of the Gram partials: row_blocks capped so that the blocks of the
+// matrix-core Gram kernel just fill the device at its occupancy (rank 64: 2
+// waves per SIMD, rank 32: 5) and the R x R partials stay small.
+int gram_blocks(uint64_t rows, int R);
 // u = M * vinv in place; partials[b][0..R) = sum u^2, [R..2R) = max u,
 // [2R..3R) = sum u * M (per row block b).
 int launch_solve(double *M, uint64_t rows, int R, const double *vinv, double *partials,
