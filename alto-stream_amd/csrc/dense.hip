@@ -18,6 +18,9 @@
 
 namespace alto {
 
+// 4 doubles: the accumulator of v_mfma_f64_16x16x4f64
+typedef double v4d __attribute__((ext_vector_type(4)));
+
 
 __global__ void k_fill_zero(double *p, uint64_t n) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -602,6 +605,116 @@ __global__ __launch_bounds__(256) void k_solve_r(const MT *M, double *U, uint64_
     }
 }
 
+// u = M * vinv in place on the FP64 matrix cores, R in {16, 32, 64}, with
+// the per-block column partials of k_solve_r.  A wave takes 16 rows at a
+// time: it loads them coalesced (the next 16 while the current ones are
+// multiplied), stages them in its own LDS tile, and forms the 16 x R product
+// block by block as v_mfma_f64_16x16x4f64(M[i][4s + l/16], vinv[4s + l/16][16J
+// + l%16]) over the R/4 k-steps; vinv sits in LDS.  D item q of lane l is row
+// l/16 + 4q, column 16J + l%16 of the block.  Dynamic LDS: R^2 + 64 R doubles.
+template <int R>
+__global__ __launch_bounds__(256) void k_solve_mfma(double *M, uint64_t rows, const double *__restrict__ vinv,
+                                                    double *__restrict__ partials) {
+    static_assert(R % 16 == 0 && R <= 64, "16-column blocks");
+    constexpr int NB = R / 16, KS = R / 4, PL = R / 4;  // column blocks, k-steps, tile doubles per lane
+    extern __shared__ double lds[];
+    double *V = lds, *T = lds + R * R;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int i16 = lane & 15, k4 = lane >> 4;
+    double *Tw = T + wave * 16 * R;
+    for (int e = t; e < R * R; e += 256) V[e] = vinv[e];
+    __syncthreads();
+    double ss[NB], mx[NB], ip[NB];
+#pragma unroll
+    for (int J = 0; J < NB; ++J) {
+        ss[J] = 0.0;
+        mx[J] = -INFINITY;
+        ip[J] = 0.0;
+    }
+    const uint64_t ntiles = (rows + 15) / 16, stride = (uint64_t)gridDim.x * 4, lim = rows * R;
+    double nx[PL];
+    auto load = [&](uint64_t tile) {
+        const uint64_t base = tile * 16 * R;
+#pragma unroll
+        for (int p = 0; p < PL; ++p) {
+            const uint64_t e = base + lane + 64 * p;
+            nx[p] = e < lim ? M[e] : 0.0;
+        }
+    };
+    uint64_t tile = (uint64_t)blockIdx.x * 4 + wave;
+    if (tile < ntiles) load(tile);
+    for (; tile < ntiles; tile += stride) {
+        // the wave's LDS operations run in order: these stores land before
+        // the loads below, and the loads before the next tile's stores
+#pragma unroll
+        for (int p = 0; p < PL; ++p) Tw[lane + 64 * p] = nx[p];
+        if (tile + stride < ntiles) load(tile + stride);
+        __builtin_amdgcn_wave_barrier();
+        v4d acc[NB];
+#pragma unroll
+        for (int J = 0; J < NB; ++J) acc[J] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            const double a = Tw[i16 * R + 4 * ks + k4];
+#pragma unroll
+            for (int J = 0; J < NB; ++J) {
+                const double b = V[(4 * ks + k4) * R + 16 * J + i16];
+                acc[J] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[J], 0, 0, 0);
+            }
+        }
+        const uint64_t r0 = tile * 16;
+#pragma unroll
+        for (int J = 0; J < NB; ++J)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int rr = k4 + 4 * q, c = 16 * J + i16;
+                if (r0 + rr < rows) {
+                    const double u = acc[J][q];
+                    M[(r0 + rr) * R + c] = u;
+                    ss[J] += u * u;
+                    mx[J] = u > mx[J] ? u : mx[J];
+                    ip[J] += u * Tw[rr * R + c];
+                }
+            }
+        __builtin_amdgcn_wave_barrier();
+    }
+    // column partials: the four row groups of the wave, then the four waves
+    // in order (the tiles' LDS is reused)
+    __syncthreads();
+    double *red = T;  // [wave][3][R]
+#pragma unroll
+    for (int J = 0; J < NB; ++J) {
+        double a = ss[J], m = mx[J], p = ip[J];
+        a += __shfl_xor(a, 16, 64);
+        a += __shfl_xor(a, 32, 64);
+        const double m1 = __shfl_xor(m, 16, 64);
+        m = m1 > m ? m1 : m;
+        const double m2 = __shfl_xor(m, 32, 64);
+        m = m2 > m ? m2 : m;
+        p += __shfl_xor(p, 16, 64);
+        p += __shfl_xor(p, 32, 64);
+        if (k4 == 0) {
+            red[(wave * 3 + 0) * R + 16 * J + i16] = a;
+            red[(wave * 3 + 1) * R + 16 * J + i16] = m;
+            red[(wave * 3 + 2) * R + 16 * J + i16] = p;
+        }
+    }
+    __syncthreads();
+    if (t < R) {
+        double a = red[t], m = red[R + t], p = red[2 * R + t];
+        for (int w = 1; w < 4; ++w) {
+            a += red[(w * 3 + 0) * R + t];
+            const double mw = red[(w * 3 + 1) * R + t];
+            m = mw > m ? mw : m;
+            p += red[(w * 3 + 2) * R + t];
+        }
+        double *out = partials + (size_t)blockIdx.x * 3 * R;
+        out[t] = a;
+        out[R + t] = m;
+        out[2 * R + t] = p;
+    }
+}
+
 int launch_solve(double *M, uint64_t rows, int R, const double *vinv, double *partials, int nblk,
                  hipStream_t s) {
     if (R > kMaxRank) return ALTO_ERR_UNSUPPORTED;
@@ -614,11 +727,14 @@ int launch_solve(double *M, uint64_t rows, int R, const double *vinv, double *pa
     std::call_once(attr, [] {
         (void)hipFuncSetAttribute((const void *)k_solve, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024 - 256);
+        (void)hipFuncSetAttribute((const void *)k_solve_mfma<64>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024 - 256);
     });
+    const size_t lds_mfma = sizeof(double) * ((size_t)R * R + 64 * (size_t)R);
     if (R == 8) k_solve_r<8, double><<<nblk, 256, 0, s>>>(M, M, rows, vinv, partials);
-    else if (R == 16) k_solve_r<16, double><<<nblk, 256, 0, s>>>(M, M, rows, vinv, partials);
-    else if (R == 32) k_solve_r<32, double><<<nblk, 256, 0, s>>>(M, M, rows, vinv, partials);
-    else if (R == 64) k_solve_r<64, double><<<nblk, 256, 0, s>>>(M, M, rows, vinv, partials);
+    else if (R == 16) k_solve_mfma<16><<<nblk, 256, lds_mfma, s>>>(M, rows, vinv, partials);
+    else if (R == 32) k_solve_mfma<32><<<nblk, 256, lds_mfma, s>>>(M, rows, vinv, partials);
+    else if (R == 64) k_solve_mfma<64><<<nblk, 256, lds_mfma, s>>>(M, rows, vinv, partials);
     else k_solve<<<nblk, 256, lds, s>>>(M, rows, R, RP, vinv, partials);
     ALTO_LAUNCH_CHECK();
     return ALTO_OK;
@@ -731,7 +847,6 @@ __global__ __launch_bounds__(256) void k_scale_gram(const double *__restrict__ u
 // bits), taking the Gram of the stored values; MODE 1 reads U in PT.  u and
 // Uout may be the same array (FP64 in place): each element is read and
 // written by one lane, the read first.
-typedef double v4d __attribute__((ext_vector_type(4)));
 
 template <int MODE, int R, typename PT>
 __global__ __launch_bounds__(256) void k_scale_gram_mfma(const double *u, const double *__restrict__ lambda,
