@@ -441,6 +441,118 @@ __global__ __launch_bounds__(256) void k_inv_gj(const double *__restrict__ grams
     if (t == 0 && flag) *flag = 0;
 }
 
+// The same for R in {8, 16, 32} in one wave, with no barrier: lane l owns
+// column l % R for the rows i = l / R (mod C), C = 64 / R.  Step k takes row
+// k (for the lane's column) and column k (for its rows) from the lanes that
+// own them (ds_bpermute), the pivot by v_readlane; the loop is unrolled, so
+// every register index is static.  Same arithmetic, fallbacks and output as
+// k_inv_gj.
+template <int R>
+__global__ __launch_bounds__(64) void k_inv_gj1(const double *__restrict__ grams, int nmode, int mode,
+                                               double diag_add, double *__restrict__ vinv,
+                                               int *__restrict__ flag, double *__restrict__ phi_out,
+                                               int fail_mode, double *__restrict__ rho_out) {
+    static_assert(R == 8 || R == 16 || R == 32, "one wave");
+    constexpr int RR = R * R, C = 64 / R, NR = R / C, LD = R + 1;
+    __shared__ double full[R * LD];
+    const int t = threadIdx.x, j = t % R, cls = t / R;
+    // Phi from the upper triangles, mode by mode, straight into registers
+    double a[NR];
+#pragma unroll
+    for (int n = 0; n < NR; ++n) a[n] = 1.0;
+    for (int m = 0; m < nmode; ++m) {
+        if (m == mode) continue;
+        const double *g = grams + (size_t)m * RR;
+#pragma unroll
+        for (int n = 0; n < NR; ++n) {
+            const int i = cls + C * n;
+            const int lo = i < j ? i : j, hi = i < j ? j : i;
+            a[n] *= g[lo * R + hi];
+        }
+    }
+    if (phi_out) {
+        for (int e = t; e < RR; e += 64) {
+            double w = 1.0;
+            for (int m = 0; m < nmode; ++m)
+                if (m != mode) w *= grams[(size_t)m * RR + e];
+            phi_out[e] = w;
+        }
+    }
+    double shift = diag_add;
+    if (rho_out) {  // ADMM penalty: rho = tr(Phi) / R (admm.cpp:254), summed in row order
+        double tr = 0.0;
+        for (int k = 0; k < R; ++k) {
+            double v = 1.0;
+            for (int m = 0; m < nmode; ++m)
+                if (m != mode) v *= grams[(size_t)m * RR + k * R + k];
+            tr += v;
+        }
+        shift = tr / (double)R;
+        if (t == 0) *rho_out = shift;
+    }
+#pragma unroll
+    for (int n = 0; n < NR; ++n)
+        if (cls + C * n == j) a[n] += shift;
+    bool bad = false;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int kn = k / C, kc = k % C;
+        const double p = readlane_f64(a[kn], kc * R + k);
+        if (!(p > 0.0)) {
+            bad = true;
+            break;
+        }
+        const double ip = 1.0 / p;
+        const double rkj = __shfl(a[kn], kc * R + j, 64);
+        const double rk = (j == k ? 1.0 : rkj) * ip;  // new row k
+        double akn[NR];
+#pragma unroll
+        for (int n = 0; n < NR; ++n) akn[n] = __shfl(a[n], cls * R + k, 64);
+#pragma unroll
+        for (int n = 0; n < NR; ++n) {
+            const int i = cls + C * n;
+            a[n] = i == k ? rk : (j == k ? 0.0 : a[n]) - akn[n] * rk;
+        }
+    }
+    if (bad) {
+        if (fail_mode == 1) {
+            for (int e = t; e < RR; e += 64) vinv[e] = (e / R == e % R) ? 1.0 : 0.0;
+        } else {
+            // reference fallback (cpd.hpp:438-476): minimum-norm solve of the
+            // unregularised diagonal
+            double mx = 0.0;
+            for (int r = 0; r < R; ++r) {
+                double v = 1.0;
+                for (int m = 0; m < nmode; ++m)
+                    if (m != mode) v *= grams[(size_t)m * RR + r * R + r];
+                mx = fabs(v) > mx ? fabs(v) : mx;
+            }
+            const double tol = mx * R * DBL_EPSILON;
+            for (int e = t; e < RR; e += 64) {
+                const int i = e / R, jj = e % R;
+                double v = 0.0;
+                if (i == jj) {
+                    double dd = 1.0;
+                    for (int m = 0; m < nmode; ++m)
+                        if (m != mode) dd *= grams[(size_t)m * RR + e];
+                    v = fabs(dd) > tol ? 1.0 / dd : 0.0;
+                }
+                vinv[e] = v;
+            }
+        }
+        if (t == 0 && flag) *flag = 1;
+        return;
+    }
+#pragma unroll
+    for (int n = 0; n < NR; ++n) full[(cls + C * n) * LD + j] = a[n];
+    __syncthreads();
+    for (int e = t; e < RR; e += 64) {
+        const int i = e / R, jj = e % R;
+        vinv[e] = 0.5 * (full[i * LD + jj] + full[jj * LD + i]);
+    }
+    if (t == 0 && flag) *flag = 0;
+}
+
 int launch_chol_inv(const double *grams, int nmode, int mode, int R, double diag_add,
                     bool copy_first, double *vinv, int *flag, double *phi_out, hipStream_t s,
                     int fail_mode, double *rho_out) {
@@ -455,11 +567,11 @@ int launch_chol_inv(const double *grams, int nmode, int mode, int R, double diag
     });
     if (R == 8 || R == 16 || R == 32 || R == 64) {
         if (R == 8)
-            k_inv_gj<8><<<1, 256, 0, s>>>(grams, nmode, mode, diag_add, vinv, flag, phi_out, fail_mode, rho_out);
+            k_inv_gj1<8><<<1, 64, 0, s>>>(grams, nmode, mode, diag_add, vinv, flag, phi_out, fail_mode, rho_out);
         else if (R == 16)
-            k_inv_gj<16><<<1, 256, 0, s>>>(grams, nmode, mode, diag_add, vinv, flag, phi_out, fail_mode, rho_out);
+            k_inv_gj1<16><<<1, 64, 0, s>>>(grams, nmode, mode, diag_add, vinv, flag, phi_out, fail_mode, rho_out);
         else if (R == 32)
-            k_inv_gj<32><<<1, 256, 0, s>>>(grams, nmode, mode, diag_add, vinv, flag, phi_out, fail_mode, rho_out);
+            k_inv_gj1<32><<<1, 64, 0, s>>>(grams, nmode, mode, diag_add, vinv, flag, phi_out, fail_mode, rho_out);
         else
             k_inv_gj<64><<<1, 256, 0, s>>>(grams, nmode, mode, diag_add, vinv, flag, phi_out, fail_mode, rho_out);
     } else if (R <= 64) {

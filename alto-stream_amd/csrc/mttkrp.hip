@@ -906,7 +906,7 @@ static int dispatch_prec(const MttkrpArgs &a, hipStream_t s) {
 // at least kTuneMinEntries entries, the first call of each bucket (modes,
 // rank, precision, output type, key words, log2 of the entry count, log2 of
 // the output rows) times the candidates -- segment length {p/4, p/2, p, 2p}
-// around the heuristic p, fiber or decode-once kernel -- best of three runs
+// around the heuristic p, fiber or decode-once kernel -- best of five runs
 // each on a scratch output.  The heuristic choice is kept unless a candidate
 // is more than 1% faster.  ALTO_AUTOTUNE=0 in the environment turns it off.
 constexpr uint64_t kTuneMinEntries = 1ULL << 20;
@@ -967,12 +967,12 @@ static int autotune(TensorImpl *t, MttkrpArgs &a, hipStream_t s) {
         if (!seen) pgs[npg++] = cand;
     }
     int st = ALTO_OK;
-    // best of three runs of one configuration
+    // best of five runs of one configuration
     auto time_cfg = [&](uint64_t pg, bool fib) -> float {
         c.per_group = pg;
         c.fiber = fib;
         float best_ms = 1e30f;
-        for (int rep = 0; rep < 3 && st == ALTO_OK; ++rep) {
+        for (int rep = 0; rep < 5 && st == ALTO_OK; ++rep) {
             float ms = 0.f;
             if (hipEventRecord(e0, s) != hipSuccess) st = ALTO_ERR_HIP;
             if (st == ALTO_OK) st = t->pkw == 1 ? dispatch_prec<1>(c, s) : dispatch_prec<2>(c, s);
