@@ -190,14 +190,19 @@ static int als_run(TensorImpl *t, const alto_hip_als_opts *o, double *const *fac
             for (int n = 0; n < N; ++n) {
                 const uint64_t I = t->dims[n];
                 const int nb = row_blocks(I);
-                ALTO_TRY(launch_fill_zero(mt.as<double>(), I * R, s));
+                // mt is zeroed by the previous mode's Gram kernel (which
+                // reads the solved factor out of it), except before the
+                // first MTTKRP of the run
+                if (first && n == 0) ALTO_TRY(launch_fill_zero(mt.as<double>(), I * R, s));
                 ALTO_TRY(run_mttkrp(t, n, prec, R, fptr, mt.p, 1, 2, 0, -1, s, false, true));
                 ALTO_TRY(launch_chol_inv(grams.as<double>(), N, n, R, 1e-3, true, vinv.as<double>(),
                                          flag.as<int>(), nullptr, s));
                 ALTO_TRY(launch_solve(mt.as<double>(), I, R, vinv.as<double>(), part.as<double>(), nb, s));
                 ALTO_TRY(launch_lambda(part.as<double>(), nb, R, first ? 0 : 1, lam.as<double>(), s));
+                const uint64_t Inext = std::max<uint64_t>(I, t->dims[(n + 1) % N]);
                 ALTO_TRY(launch_scale_gram(mt.as<double>(), lam.as<double>(), I, R, Uf[n].p, prec,
-                                           gpart.as<double>(), gram_blocks(I, R), s));
+                                           gpart.as<double>(), gram_blocks(I, R), s, mt.as<double>(),
+                                           Inext * R));
                 ALTO_TRY(launch_gram_reduce(gpart.as<double>(), gram_blocks(I, R), R,
                                             grams.as<double>() + (size_t)n * R * R, s));
             }

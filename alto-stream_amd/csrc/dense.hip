@@ -873,12 +873,13 @@ int launch_lambda(const double *partials, int nblk, int R, int which, double *la
 }
 
 template <int MODE, int NQ>  // MODE 0: scale f64 u by 1/lambda and store; 1: read stored factor
-__global__ __launch_bounds__(256) void k_scale_gram(const double *__restrict__ u,
+__global__ __launch_bounds__(256) void k_scale_gram(const double *u,
                                                     const double *__restrict__ lambda,
                                                     const void *__restrict__ Uin,
                                                     void *__restrict__ Uout, int prec,
                                                     uint64_t rows, int R,
-                                                    double *__restrict__ gpart) {
+                                                    double *__restrict__ gpart, double *zero,
+                                                    uint64_t zero_n) {
     extern __shared__ double T[];  // TR x R
     constexpr int TR = 32;
     const int t = threadIdx.x;
@@ -909,6 +910,7 @@ __global__ __launch_bounds__(256) void k_scale_gram(const double *__restrict__ u
                     ((uint16_t *)Uout)[gi] = b;
                     v = (double)bf16_to_f32(b);
                 }
+                if (zero) zero[gi] = 0.0;  // the element was read above
             } else {
                 if (prec == ALTO_PREC_F64) v = ((const double *)Uin)[gi];
                 else if (prec == ALTO_PREC_F32) v = (double)((const float *)Uin)[gi];
@@ -938,6 +940,9 @@ __global__ __launch_bounds__(256) void k_scale_gram(const double *__restrict__ u
             }
         }
     }
+    if (zero)  // beyond this factor's rows: nobody reads them here
+        for (uint64_t e = rows * R + (uint64_t)blockIdx.x * 256 + t; e < zero_n; e += (uint64_t)gridDim.x * 256)
+            zero[e] = 0.0;
     double *p = gpart + (size_t)blockIdx.x * RR;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
@@ -963,7 +968,7 @@ __global__ __launch_bounds__(256) void k_scale_gram(const double *__restrict__ u
 template <int MODE, int R, typename PT>
 __global__ __launch_bounds__(256) void k_scale_gram_mfma(const double *u, const double *__restrict__ lambda,
                                                          const PT *__restrict__ Uin, PT *Uout, uint64_t rows,
-                                                         double *__restrict__ gpart) {
+                                                         double *__restrict__ gpart, double *zero, uint64_t zero_n) {
     static_assert(R % 16 == 0 && R <= 64, "16-column blocks");
     constexpr int NB = R / 16, NP = NB * (NB + 1) / 2, RR = R * R;
     using RawT = typename std::conditional<MODE == 0, double, PT>::type;
@@ -1015,6 +1020,7 @@ __global__ __launch_bounds__(256) void k_scale_gram_mfma(const double *u, const 
                     v[I] = (double)bf16_to_f32(b);
                 }
                 if (row >= rows) v[I] = 0.0;
+                else if (zero) zero[gi] = 0.0;  // its value is consumed above
             } else {
                 if constexpr (sizeof(PT) == 2) v[I] = (double)bf16_to_f32(raw[I]);
                 else v[I] = (double)raw[I];
@@ -1026,6 +1032,10 @@ __global__ __launch_bounds__(256) void k_scale_gram_mfma(const double *u, const 
 #pragma unroll
             for (int J = I; J < NB; ++J, ++p) acc[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[I], v[J], acc[p], 0, 0, 0);
     }
+    if (zero)  // beyond this factor's rows: nobody reads them here
+        for (uint64_t e = rows * R + (uint64_t)blockIdx.x * 256 + threadIdx.x; e < zero_n;
+             e += (uint64_t)gridDim.x * 256)
+            zero[e] = 0.0;
     // D item q of lane l is element (l / 16 + 4 q, l % 16) of the block
     for (int w = 0; w < 4; ++w) {
         if (wave == w) {
@@ -1057,40 +1067,42 @@ __global__ __launch_bounds__(256) void k_scale_gram_mfma(const double *u, const 
 
 template <int MODE, int R>
 static void launch_sg_mfma(const double *u, const double *lambda, const void *Uin, void *Uout, int prec,
-                           uint64_t rows, double *gpart, int nblk, hipStream_t s) {
+                           uint64_t rows, double *gpart, int nblk, hipStream_t s, double *zero, uint64_t zero_n) {
     if (prec == ALTO_PREC_F64)
         k_scale_gram_mfma<MODE, R, double><<<nblk, 256, 0, s>>>(u, lambda, (const double *)Uin, (double *)Uout,
-                                                                 rows, gpart);
+                                                                 rows, gpart, zero, zero_n);
     else if (prec == ALTO_PREC_F32)
         k_scale_gram_mfma<MODE, R, float><<<nblk, 256, 0, s>>>(u, lambda, (const float *)Uin, (float *)Uout,
-                                                                rows, gpart);
+                                                                rows, gpart, zero, zero_n);
     else
         k_scale_gram_mfma<MODE, R, uint16_t><<<nblk, 256, 0, s>>>(u, lambda, (const uint16_t *)Uin,
-                                                                   (uint16_t *)Uout, rows, gpart);
+                                                                   (uint16_t *)Uout, rows, gpart, zero, zero_n);
 }
 
 template <int MODE>
 static void launch_sg(const double *u, const double *lambda, const void *Uin, void *Uout, int prec,
-                      uint64_t rows, int R, double *gpart, int nblk, hipStream_t s) {
-    if (R == 16) return launch_sg_mfma<MODE, 16>(u, lambda, Uin, Uout, prec, rows, gpart, nblk, s);
-    if (R == 32) return launch_sg_mfma<MODE, 32>(u, lambda, Uin, Uout, prec, rows, gpart, nblk, s);
-    if (R == 64) return launch_sg_mfma<MODE, 64>(u, lambda, Uin, Uout, prec, rows, gpart, nblk, s);
+                      uint64_t rows, int R, double *gpart, int nblk, hipStream_t s, double *zero = nullptr,
+                      uint64_t zero_n = 0) {
+    if (R == 16) return launch_sg_mfma<MODE, 16>(u, lambda, Uin, Uout, prec, rows, gpart, nblk, s, zero, zero_n);
+    if (R == 32) return launch_sg_mfma<MODE, 32>(u, lambda, Uin, Uout, prec, rows, gpart, nblk, s, zero, zero_n);
+    if (R == 64) return launch_sg_mfma<MODE, 64>(u, lambda, Uin, Uout, prec, rows, gpart, nblk, s, zero, zero_n);
     size_t lds = sizeof(double) * 32 * R;
     int nq = (R * R + 255) / 256;
     if (nq <= 1)
-        k_scale_gram<MODE, 1><<<nblk, 256, lds, s>>>(u, lambda, Uin, Uout, prec, rows, R, gpart);
+        k_scale_gram<MODE, 1><<<nblk, 256, lds, s>>>(u, lambda, Uin, Uout, prec, rows, R, gpart, zero, zero_n);
     else if (nq <= 4)
-        k_scale_gram<MODE, 4><<<nblk, 256, lds, s>>>(u, lambda, Uin, Uout, prec, rows, R, gpart);
+        k_scale_gram<MODE, 4><<<nblk, 256, lds, s>>>(u, lambda, Uin, Uout, prec, rows, R, gpart, zero, zero_n);
     else if (nq <= 16)
-        k_scale_gram<MODE, 16><<<nblk, 256, lds, s>>>(u, lambda, Uin, Uout, prec, rows, R, gpart);
+        k_scale_gram<MODE, 16><<<nblk, 256, lds, s>>>(u, lambda, Uin, Uout, prec, rows, R, gpart, zero, zero_n);
     else
-        k_scale_gram<MODE, 64><<<nblk, 256, lds, s>>>(u, lambda, Uin, Uout, prec, rows, R, gpart);
+        k_scale_gram<MODE, 64><<<nblk, 256, lds, s>>>(u, lambda, Uin, Uout, prec, rows, R, gpart, zero, zero_n);
 }
 
 int launch_scale_gram(const double *u, const double *lambda, uint64_t rows, int R, void *U,
-                      int prec, double *gpart, int nblk, hipStream_t s) {
+                      int prec, double *gpart, int nblk, hipStream_t s, double *zero, uint64_t zero_n) {
     if (R > kMaxRank) return ALTO_ERR_UNSUPPORTED;
-    launch_sg<0>(u, lambda, nullptr, U, prec, rows, R, gpart, nblk, s);
+    if (zero && (zero == U || zero_n < rows * (uint64_t)R)) return ALTO_ERR_ARG;
+    launch_sg<0>(u, lambda, nullptr, U, prec, rows, R, gpart, nblk, s, zero, zero_n);
     ALTO_LAUNCH_CHECK();
     return ALTO_OK;
 }

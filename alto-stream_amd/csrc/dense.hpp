@@ -86,8 +86,12 @@ int launch_lambda(const double *partials, int nblk, int R, int which, double *la
                   hipStream_t s);
 // Stores U = u / lambda (or u when lambda == nullptr) in precision prec,
 // and writes Gram partials of the stored values.
+// zero (optional, never U): u's storage, left zero in [0, zero_n) afterwards
+// (each element is zeroed after its value is read; zero_n >= rows R), so
+// the next MTTKRP can write into it without a separate fill.
 int launch_scale_gram(const double *u, const double *lambda, uint64_t rows, int R, void *U,
-                      int prec, double *gpart, int nblk, hipStream_t s);
+                      int prec, double *gpart, int nblk, hipStream_t s, double *zero = nullptr,
+                      uint64_t zero_n = 0);
 // Gram partials of an existing factor stored in precision prec.
 int launch_gram(const void *U, int prec, uint64_t rows, int R, double *gpart, int nblk,
                 hipStream_t s);
