@@ -355,6 +355,7 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
     ALTO_TRY(trows.alloc(sizeof(double) * std::max(nsl, 1) * R));
     ALTO_TRY(launch_fill_zero(trows.as<double>(), (uint64_t)std::max(nsl, 1) * R, st));
     TensorImpl at;
+    at.keep_sched_scratch = true;  // rebuilt for every slice
     const TileShape ts = tile_shape(R);
     const size_t lds_first = sizeof(double) * (2 * RR + 2 * (size_t)ts.TR * R);
     const size_t lds_iter = sizeof(double) * (RR + (size_t)ts.TR * R);

@@ -336,7 +336,10 @@ int ensure_schedule(TensorImpl *t, int mode, bool need64, hipStream_t s) {
     ord.mode[0] = mode;
     for (int q = 0; q < t->nmode - 1; ++q) ord.mode[q + 1] = sc.lvl[q];
     const bool wide = t->total_bits > 64;
-    DevBuf k0, k1, perm, sk, sv;
+    DevBuf lk0, lk1, lperm, lsk, lsv, llongs;
+    const bool keep = t->keep_sched_scratch;
+    DevBuf &k0 = keep ? t->sch_k0 : lk0, &k1 = keep ? t->sch_k1 : lk1, &perm = keep ? t->sch_perm : lperm,
+           &sk = keep ? t->sch_sk : lsk, &sv = keep ? t->sch_sv : lsv, &longs = keep ? t->sch_longs : llongs;
     ALTO_TRY(k0.alloc(sizeof(uint64_t) * n));
     if (wide) ALTO_TRY(k1.alloc(sizeof(uint64_t) * n));
     ALTO_TRY(perm.alloc(sizeof(uint32_t) * n));
@@ -365,7 +368,6 @@ int ensure_schedule(TensorImpl *t, int mode, bool need64, hipStream_t s) {
     ALTO_TRY(sc.pkeys.alloc(sizeof(uint64_t) * pkw * m));
     ALTO_TRY(sc.vals32.alloc(sizeof(float) * m));
     if (need64) ALTO_TRY(sc.vals64.alloc(sizeof(double) * m));
-    DevBuf longs;
     ALTO_TRY(longs.alloc(sizeof(uint64_t) * (n / kShortRun + 2)));
     unsigned long long *nlong = (unsigned long long *)(longs.as<uint64_t>() + n / kShortRun + 1);
     ALTO_HIP_TRY(hipMemsetAsync(nlong, 0, sizeof(unsigned long long), s));
@@ -379,7 +381,7 @@ int ensure_schedule(TensorImpl *t, int mode, bool need64, hipStream_t s) {
                                           need64 ? sc.vals64.as<double>() : nullptr,
                                           sc.vals32.as<float>(), longs.as<uint64_t>(), nlong);
     ALTO_LAUNCH_CHECK();
-    ALTO_HIP_TRY(hipStreamSynchronize(s));
+    if (!keep) ALTO_HIP_TRY(hipStreamSynchronize(s));  // the local scratch is released on return
     sc.n = m;
     sc.built = true;
     sc.has64 = need64;

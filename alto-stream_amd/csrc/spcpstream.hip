@@ -324,6 +324,7 @@ extern "C" int alto_hip_spcpstream(const alto_hip_coo *coo, int s, int R, int ma
         });
     }
     TensorImpl at;
+    at.keep_sched_scratch = true;  // rebuilt for every slice
     hipEvent_t ev[4];
     for (auto &e : ev) ALTO_HIP_TRY(hipEventCreate(&e));
     double t_build = 0;

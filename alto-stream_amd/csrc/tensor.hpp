@@ -56,6 +56,12 @@ struct TensorImpl {
     DevBuf scratch;     // generic per-call scratch
     DevBuf k0, k1, bad, ivbuf, chunkbuf;  // construction scratch, kept for rebuilds
     bool vals32_valid = false;
+    // Schedule-construction scratch kept between builds (a tensor rebuilt
+    // slice after slice keeps it, so a rebuild neither allocates nor frees
+    // device memory, which would synchronise the device); otherwise it is
+    // allocated and released per schedule.
+    bool keep_sched_scratch = false;
+    DevBuf sch_k0, sch_k1, sch_perm, sch_sk, sch_sv, sch_longs;
 };
 
 // (Re)builds t from a COO tensor, reusing t's device buffers (the GPU
