@@ -36,7 +36,7 @@ def _init(dims, R, seed):
 
 
 @pytest.mark.parametrize("nonneg", [True, False])
-@pytest.mark.parametrize("R", [4, 16])
+@pytest.mark.parametrize("R", [4, 16, 32])
 def test_cpstream_matches_oracle(gpu, nonneg, R):
     dims, cidx, vals = _stream_tensor(6, [300, 200], 6000, seed=90)
     init = _init(dims, R, 91)

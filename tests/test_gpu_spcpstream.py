@@ -29,7 +29,7 @@ def _compare(got, ref, R, rtol=1e-8):
                                rtol=rtol, atol=1e-9)
 
 
-@pytest.mark.parametrize("R", [4, 16])
+@pytest.mark.parametrize("R", [4, 16, 32])
 @pytest.mark.parametrize("eps", [1e-5, 0.0])
 def test_spcpstream_matches_oracle(gpu, R, eps):
     dims, cidx, vals = _stream_tensor(6, [300, 200], 6000, seed=110)
