@@ -82,6 +82,26 @@ def test_als_128bit_keys(gpu):
     np.testing.assert_allclose(gfits, rfits, rtol=0, atol=1e-8)
 
 
+@pytest.mark.parametrize("prec,tol", [(0, 1e-8), (2, 2e-3)])
+def test_als_5mode_128bit_rank64(gpu, prec, tol):
+    """Five modes of 2^14 (70 key bits: 128-bit keys, two packed words),
+    rank 64: the decode-once kernel with four gathered modes, the
+    matrix-core Gram and solve and the four-wave inverse."""
+    dims, cidx, vals, F = _setup([1 << 14] * 5, 30000, 88, 64)
+    ref = O.AltoTensor(dims, cidx, vals, nprtn=4)
+    rf, rl, rfits, rfe = ref.cpd_als(F, 64, 3, fit_mode=O.FIT_EXACT)
+    at = gpu.create_alto(dims, cidx, vals, nprtn=4)
+    assert at.kw == 2
+    gf, gl, gfits, gfe, _ = gpu.cpd_alto(at, F, 64, 3, prec=prec, fit_mode=gpu.FIT_EXACT)
+    print("prec", prec, "max |dfit|", np.abs(gfits - rfits).max(), flush=True)
+    assert np.abs(gfits - rfits).max() <= tol
+    assert np.abs(gfe - rfe).max() <= tol
+    if prec == 0:  # the model itself, not only its fit
+        np.testing.assert_allclose(gl, rl, rtol=1e-6)
+        for a, b in zip(gf, rf):
+            np.testing.assert_allclose(a, b, rtol=1e-6, atol=1e-9)
+
+
 def test_als_epsilon_stop_and_exact_fit_is_residual(gpu):
     dims, cidx, vals, F = _setup([8, 7, 6], 100, 84, 3, distinct=True)
     at = gpu.create_alto(dims, cidx, vals)
