@@ -395,6 +395,7 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
         return launch_gram_reduce(gpart.as<double>(), nb, R, G.as<double>() + m * RR, st);
     };
 
+    IterGraph ig(st);  // the outer-iteration graph, reused from slice to slice
     for (int k = 0; k < nsl; ++k) {
         const Slice &sl = slices[k];
         const uint64_t *bd = sl.dims;
@@ -532,8 +533,8 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
             return ALTO_OK;
         };
         // Without the constraint the iterations after the first replay one
-        // captured graph; with it they run directly (the ADMM reads its flag).
-        IterGraph ig(st);
+        // captured graph (kept across slices, updated in place); with it they
+        // run directly (the ADMM reads its flag).
         for (int i = 0; i < max_iters; ++i) {
             if (nonneg) ALTO_TRY(outer_iter());
             else ALTO_TRY(ig.run(i, outer_iter));

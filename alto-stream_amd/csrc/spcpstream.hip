@@ -346,6 +346,7 @@ extern "C" int alto_hip_spcpstream(const alto_hip_coo *coo, int s, int R, int ma
     };
     std::vector<int> outer_v(std::max(nsl, 1), 0);
 
+    IterGraph ig(st);  // the outer-iteration graph, reused from slice to slice
     for (int k = 0; k < nsl; ++k) {
         const Slice &sl = slices[k];
         const uint64_t *bd = sl.dims;
@@ -474,7 +475,6 @@ extern "C" int alto_hip_spcpstream(const alto_hip_coo *coo, int s, int R, int ma
             }
             return ALTO_OK;
         };
-        IterGraph ig(st);
         for (int i = 0; i < max_iters; ++i) {
             ALTO_TRY(ig.run(i, outer_iter));
             double hd = 0.0;

@@ -37,6 +37,7 @@ int stream_prepare(const alto_hip_coo *coo, int s, StreamInput *out, hipStream_t
 struct IterGraph {
     hipStream_t s = nullptr;
     hipGraphExec_t ge = nullptr;
+    bool current = false;  // ge holds this sequence's capture
     explicit IterGraph(hipStream_t st) : s(st) {}
     IterGraph(const IterGraph &) = delete;
     IterGraph &operator=(const IterGraph &) = delete;
@@ -46,10 +47,18 @@ struct IterGraph {
             (void)hipGraphExecDestroy(ge);
         }
     }
+    // Iteration 0 of a sequence runs directly; iteration 1 is captured and
+    // iterations >= 1 replay the capture.  A sequence after the first (the
+    // next slice) updates the executable graph in place when the capture has
+    // the same shape (hipGraphExecUpdate, cheaper than instantiating), and
+    // instantiates a new one otherwise.
     template <typename F>
     int run(int iter, F enqueue) {
-        if (iter == 0) return enqueue();
-        if (!ge) {
+        if (iter == 0) {
+            current = false;
+            return enqueue();
+        }
+        if (!current) {
             hipGraph_t g = nullptr;
             ALTO_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             const int st = enqueue();
@@ -59,9 +68,23 @@ struct IterGraph {
                 if (st != ALTO_OK) return st;
                 ALTO_HIP_TRY(ce);
             }
-            const hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            if (ge) {
+                // no launch of the previous capture may still be running
+                hipError_t ue = hipStreamSynchronize(s);
+                hipGraphNode_t bad = nullptr;
+                hipGraphExecUpdateResult res = hipGraphExecUpdateSuccess;
+                if (ue == hipSuccess) ue = hipGraphExecUpdate(ge, g, &bad, &res);
+                if (ue != hipSuccess || res != hipGraphExecUpdateSuccess) {
+                    (void)hipGetLastError();
+                    (void)hipGraphExecDestroy(ge);
+                    ge = nullptr;
+                }
+            }
+            hipError_t ie = hipSuccess;
+            if (!ge) ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
             ALTO_HIP_TRY(ie);
+            current = true;
         }
         ALTO_HIP_TRY(hipGraphLaunch(ge, s));
         return ALTO_OK;
