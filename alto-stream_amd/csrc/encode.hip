@@ -319,6 +319,7 @@ void fiber_levels(const TensorImpl *t, int mode, int *lvl) {
 int ensure_schedule(TensorImpl *t, int mode, bool need64, hipStream_t s) {
     ModeSchedule &sc = t->sched[mode];
     if (sc.built && (!need64 || sc.has64)) return ALTO_OK;
+    sc.hot.built = false;
     const uint64_t n = t->nnz;
     const int pkw = t->pkw;
     fiber_levels(t, mode, sc.lvl);
@@ -403,7 +404,11 @@ int build_alto(TensorImpl *t, const alto_hip_coo *coo, int nprtn, int key_words,
     t->nnz = coo->nnz;
     t->nprtn = nprtn;
     t->vals32_valid = false;
-    for (int m = 0; m < kMaxModes; ++m) t->sched[m].built = false;
+    for (int m = 0; m < kMaxModes; ++m) {
+        t->sched[m].built = false;
+        t->sched[m].hot.built = false;
+    }
+    t->hot.built = false;
     memcpy(t->dims, coo->dims, sizeof(uint64_t) * t->nmode);
     int total = 0;
     for (int m = 0; m < t->nmode; ++m) total += mode_bits(t->dims[m]);

@@ -24,6 +24,7 @@
 #include <tuple>
 #include <type_traits>
 
+#include "mttkrp.hpp"
 #include "tensor.hpp"
 
 namespace alto {
@@ -89,14 +90,8 @@ template <typename T> __device__ __forceinline__ T stream_load(const T *p, bool 
 // LDS across the workgroup, then with atomics for the workgroup's first and
 // last rows; bit 3 = no decode-once kernel; bit 4 = no fiber kernel (output
 // order only).
-// Sum of a row that this range owns (output-ordered schedule): added to the
-// output, or stored when the output held zeros on entry (flags bit 5), which
-// saves the dependent load of a read-modify-write.
-template <typename OT, typename CT>
-__device__ __forceinline__ void flush_owned(OT *p, CT v, bool zeroed) {
-    if (zeroed) *p = (OT)v;
-    else *p += (OT)v;
-}
+// Rows owned by one range are flushed with flush_owned (mttkrp.hpp): stored
+// when the output held zeros on entry (flags bit 5), else added.
 
 template <typename VT, typename FT, typename OT, int R, int PKW, int B>
 __global__ __launch_bounds__(256) void k_mttkrp_seg(const uint64_t *__restrict__ pkeys,
@@ -297,10 +292,6 @@ template <typename FT, int VPL> struct RowLoad {
     }
 };
 
-__device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned nb) {
-    const unsigned q = nb / 8, r = nb % 8, x = b % 8, slot = b / 8;
-    return x < r ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
-}
 
 // Combines one workgroup's range partials: slot e = 0 .. 2G-1 holds group
 // e >> 1's first (e even) or last (e odd) row partial, in output order, with
@@ -799,6 +790,8 @@ struct MttkrpArgs {
     int flags;
     bool decode_once;
     bool fiber;
+    bool hot;       // hot-row kernel (run_hot)
+    int hot_shape;  // its workgroup shape
     FibInfo fi;
 };
 
@@ -921,6 +914,8 @@ struct TuneKey {
 struct TuneVal {
     uint64_t per_group;
     bool fiber;
+    bool hot;
+    int hot_shape;
 };
 static std::mutex g_tune_mu;
 static std::map<TuneKey, TuneVal> g_tune;
@@ -931,6 +926,15 @@ static bool autotune_enabled() {
         return !(e && e[0] == '0');
     }();
     return on;
+}
+
+// One launch of the configuration in `a`: the hot-row kernel or the range
+// kernels.
+static int launch_cfg(TensorImpl *t, const MttkrpArgs &a, hipStream_t s) {
+    if (a.hot)
+        return run_hot(t, a.target, a.prec, a.rank, a.factors, a.out, a.out64, (a.flags & 32) != 0,
+                       a.hot_shape, s);
+    return t->pkw == 1 ? dispatch_prec<1>(a, s) : dispatch_prec<2>(a, s);
 }
 
 static int autotune(TensorImpl *t, MttkrpArgs &a, hipStream_t s) {
@@ -944,6 +948,8 @@ static int autotune(TensorImpl *t, MttkrpArgs &a, hipStream_t s) {
         if (it != g_tune.end()) {
             a.per_group = it->second.per_group;
             a.fiber = it->second.fiber;
+            a.hot = it->second.hot;
+            a.hot_shape = it->second.hot_shape;
             return ALTO_OK;
         }
     }
@@ -968,14 +974,16 @@ static int autotune(TensorImpl *t, MttkrpArgs &a, hipStream_t s) {
     }
     int st = ALTO_OK;
     // best of five runs of one configuration
-    auto time_cfg = [&](uint64_t pg, bool fib) -> float {
+    auto time_cfg = [&](uint64_t pg, bool fib, bool hot = false, int shape = 0) -> float {
         c.per_group = pg;
         c.fiber = fib;
+        c.hot = hot;
+        c.hot_shape = shape;
         float best_ms = 1e30f;
         for (int rep = 0; rep < 5 && st == ALTO_OK; ++rep) {
             float ms = 0.f;
             if (hipEventRecord(e0, s) != hipSuccess) st = ALTO_ERR_HIP;
-            if (st == ALTO_OK) st = t->pkw == 1 ? dispatch_prec<1>(c, s) : dispatch_prec<2>(c, s);
+            if (st == ALTO_OK) st = launch_cfg(t, c, s);
             if (st == ALTO_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
                 st = ALTO_ERR_HIP;
             if (st == ALTO_OK) (void)hipEventElapsedTime(&ms, e0, e1);
@@ -984,7 +992,7 @@ static int autotune(TensorImpl *t, MttkrpArgs &a, hipStream_t s) {
         return best_ms;
     };
     // the heuristic configuration stays unless another is more than 1% faster
-    TuneVal pick{a.per_group, a.fiber};
+    TuneVal pick{a.per_group, a.fiber, false, 0};
     const float base_ms = time_cfg(a.per_group, a.fiber);
     float best = base_ms;
     for (int fib = 0; fib < 2 && st == ALTO_OK; ++fib) {
@@ -996,7 +1004,17 @@ static int autotune(TensorImpl *t, MttkrpArgs &a, hipStream_t s) {
             const float ms = time_cfg(pgs[i], fib != 0);
             if (st == ALTO_OK && ms < best && ms < 0.99f * base_ms) {
                 best = ms;
-                pick = TuneVal{pgs[i], fib != 0};
+                pick = TuneVal{pgs[i], fib != 0, false, 0};
+            }
+        }
+    }
+    // the hot-row kernel in each workgroup shape (flags bit 8 excludes it)
+    if (!(a.flags & 256) && hot_supported(t, a.prec, a.rank)) {
+        for (int shape = 0; shape < 3 && st == ALTO_OK; ++shape) {
+            const float ms = time_cfg(a.per_group, false, true, shape);
+            if (st == ALTO_OK && ms < best && ms < 0.99f * base_ms) {
+                best = ms;
+                pick = TuneVal{a.per_group, false, true, shape};
             }
         }
     }
@@ -1010,6 +1028,8 @@ static int autotune(TensorImpl *t, MttkrpArgs &a, hipStream_t s) {
     }
     a.per_group = pick.per_group;
     a.fiber = pick.fiber;
+    a.hot = pick.hot;
+    a.hot_shape = pick.hot_shape;
     return ALTO_OK;
 }
 
@@ -1096,10 +1116,26 @@ int run_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const 
         uint64_t pg = a.nnz * lanes / (1ULL << 18);
         a.per_group = pg < 4 ? 4 : pg > 64 ? 64 : pg;
     }
+    // flags bit 7: the hot-row kernel, workgroup shape in bits 9-10
+    if (algo == 2 && (a.flags & 128)) {
+        if (!hot_supported(t, prec, rank)) {
+            set_error("the hot-row MTTKRP takes ranks 16, 32, 64 and 3 to 5 modes");
+            return ALTO_ERR_UNSUPPORTED;
+        }
+        a.hot = true;
+        a.hot_shape = (a.flags >> 9) & 3;
+    }
     if (defaults && algo == 2 && a.decode_once && a.nnz >= kTuneMinEntries && autotune_enabled())
         ALTO_TRY(autotune(t, a, s));
-    if (prepare_only) return ALTO_OK;
-    int st = t->pkw == 1 ? dispatch_prec<1>(a, s) : dispatch_prec<2>(a, s);
+    if (a.hot && out && ((uintptr_t)out % 16) && !(a.flags & 128))
+        a.hot = false;  // tuned choice, output not 16-byte aligned: the range kernels take it
+    if (prepare_only) {
+        if (a.hot) {  // the re-encoded keys are built once, outside any timed call
+            ALTO_TRY(ensure_hot_schedule(t, target, rank * (prec == ALTO_PREC_F64 ? 8 : prec == ALTO_PREC_F32 ? 4 : 2), s));
+        }
+        return ALTO_OK;
+    }
+    int st = launch_cfg(t, a, s);
     ALTO_TRY(st);
     ALTO_LAUNCH_CHECK();
     return ALTO_OK;

@@ -18,6 +18,28 @@ struct BitPos {
     uint8_t pos[kMaxModes][64];
 };
 
+// Rows of every mode ranked by how many nonzeros use them (the hot-row
+// MTTKRP keeps the most popular rows of the gathered modes in LDS).
+constexpr uint32_t kHotMax = 4096;  // ranked rows kept per mode
+struct HotModes {
+    bool built = false;
+    DevBuf list[kMaxModes];                // u32: the kHotMax (or fewer) most popular rows, in order
+    DevBuf rank[kMaxModes];                // u32 per row: its popularity rank if listed, else ~0
+    std::vector<uint32_t> cnt[kMaxModes];  // host: entry counts of the listed rows
+};
+
+// A schedule's keys re-encoded for the hot-row MTTKRP: gathered mode q keeps
+// K[q] rows in LDS, and a hot index of mode m is stored as I_m + rank.
+struct HotSched {
+    bool built = false;
+    int rb = 0;  // factor row bytes the LDS split was made for
+    int nq = 0;
+    int qmode[4] = {};
+    uint32_t K[4] = {};
+    PackInfo hpk{};
+    DevBuf keys;
+};
+
 // Output-ordered schedule of one mode (algo 2): the nonzeros re-sorted by
 // the mode's index, stably, so ALTO order is kept inside each row run.
 struct ModeSchedule {
@@ -28,6 +50,7 @@ struct ModeSchedule {
     DevBuf pkeys;   // packed keys in row order
     DevBuf vals32;  // FP32 values in row order
     DevBuf vals64;  // FP64 values in row order (built on demand)
+    HotSched hot;   // re-encoded keys of the hot-row MTTKRP (built on demand)
 };
 
 struct TensorImpl {
@@ -52,6 +75,8 @@ struct TensorImpl {
     std::vector<uint64_t> intervals;   // nprtn * nmode * 2
 
     ModeSchedule sched[kMaxModes];
+    HotModes hot;
+    DevBuf hot_bnd;     // boundary runs of the hot-row MTTKRP's workgroups
     SortScratch sort_ws;
     DevBuf scratch;     // generic per-call scratch
     DevBuf k0, k1, bad, ivbuf, chunkbuf;  // construction scratch, kept for rebuilds
@@ -88,6 +113,16 @@ int run_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const 
 int prepare_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const *factors,
                    int out_f64, hipStream_t s);
 void fiber_levels(const TensorImpl *t, int mode, int *lvl);
+
+// Hot-row MTTKRP (one workgroup per CU, popular rows in LDS): out += MTTKRP
+// of `target` over its built schedule.  Ranks 16, 32, 64; 3 to 5 modes.
+// shape: threads per workgroup / entries per gather batch (0: 1024 / 1,
+// 1: 768 / 2, 2: 512 / 4).
+bool hot_supported(const TensorImpl *t, int prec, int rank);
+int ensure_hot_modes(TensorImpl *t, hipStream_t s);
+int ensure_hot_schedule(TensorImpl *t, int target, int rb, hipStream_t s);
+int run_hot(TensorImpl *t, int target, int prec, int rank, const void *const *factors, void *out,
+            bool out64, bool zeroed, int shape, hipStream_t s);
 
 }  // namespace alto
 
