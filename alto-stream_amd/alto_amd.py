@@ -257,7 +257,8 @@ def mttkrp_alto_par(target, factors, rank, at, out, prec=PREC_F64, out_f64=True,
     _check_out(out, int(at.dims[target]), rank, out_f64 or prec == PREC_F64)
     check(lib().alto_hip_mttkrp_algo(_vp(at.handle), ctypes.c_int(target), ctypes.c_int(prec), ctypes.c_int(rank),
                                      _ptrs(factors), _vp(out.ptr), ctypes.c_int(1 if out_f64 else 0),
-                                     ctypes.c_int(algo), _vp(stream)), "mttkrp_alto_par")
+                                     ctypes.c_size_t(out.nbytes), ctypes.c_int(algo), _vp(stream)),
+          "mttkrp_alto_par")
 
 
 def mttkrp_config(target, factors, rank, at, out, prec=PREC_F64, out_f64=True, algo=2, segment=0, flags=-1,
@@ -267,8 +268,9 @@ def mttkrp_config(target, factors, rank, at, out, prec=PREC_F64, out_f64=True, a
     _check_out(out, int(at.dims[target]), rank, out_f64 or prec == PREC_F64)
     check(lib().alto_hip_mttkrp_config(_vp(at.handle), ctypes.c_int(target), ctypes.c_int(prec),
                                        ctypes.c_int(rank), _ptrs(factors), _vp(out.ptr),
-                                       ctypes.c_int(1 if out_f64 else 0), ctypes.c_int(algo),
-                                       ctypes.c_uint64(segment), ctypes.c_int(flags), _vp(stream)),
+                                       ctypes.c_int(1 if out_f64 else 0), ctypes.c_size_t(out.nbytes),
+                                       ctypes.c_int(algo), ctypes.c_uint64(segment), ctypes.c_int(flags),
+                                       _vp(stream)),
           "mttkrp_config")
 
 
@@ -343,8 +345,11 @@ def cpd_alto_batch(tensors, factors, ranks, max_iters, eps=1e-30, prec=PREC_F64,
 
 
 def cpstream(dims, cidx, vals, stream_mode, rank, max_iters, eps, nonneg, init_factors, max_slices=0,
-             prec=PREC_F64, stream=None):
-    """CP-stream (cpstream, src/streaming_cpd.cpp:26-366), host COO input."""
+             prec=PREC_F64, nnz_threshold=0, timeslice_limit=1, stream=None):
+    """CP-stream (cpstream, src/streaming_cpd.cpp:26-366), host COO input.
+    Time slices are batched as next_dynamic_batch does (timeslice_limit,
+    nnz_threshold; 0 = no threshold); per-batch outputs (outer_iters,
+    admm_iters) have one entry per batch, time_rows one row per time index."""
     c, keep = _coo(dims, cidx, vals, False)
     dims = np.ascontiguousarray(dims, dtype=np.uint64)
     N = len(dims)
@@ -353,7 +358,7 @@ def cpstream(dims, cidx, vals, stream_mode, rank, max_iters, eps, nonneg, init_f
     for i, f in enumerate(init):
         iarr[i] = f.ctypes.data_as(_f64p)
     nsl = int(dims[stream_mode])
-    outs = [np.zeros((int(dims[m]) if m != stream_mode else 1, rank)) for m in range(N)]
+    outs = [np.zeros((int(dims[m]), rank)) for m in range(N)]
     oarr = (_f64p * N)()
     for i, f in enumerate(outs):
         oarr[i] = f.ctypes.data_as(_f64p)
@@ -366,7 +371,8 @@ def cpstream(dims, cidx, vals, stream_mode, rank, max_iters, eps, nonneg, init_f
     ferr = np.zeros(1)
     n = lib().alto_hip_cpstream(ctypes.byref(c), ctypes.c_int(stream_mode), ctypes.c_int(rank),
                                 ctypes.c_int(max_iters), ctypes.c_double(eps), ctypes.c_int(1 if nonneg else 0),
-                                ctypes.c_int(max_slices), ctypes.c_int(prec), iarr, oarr,
+                                ctypes.c_int(max_slices), ctypes.c_uint64(nnz_threshold),
+                                ctypes.c_uint64(timeslice_limit), ctypes.c_int(prec), iarr, oarr,
                                 out_dims.ctypes.data_as(_u64p),
                                 trows.ctypes.data_as(_f64p), outer.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                                 admm.ctypes.data_as(_f64p), grams.ctypes.data_as(_f64p),
@@ -374,8 +380,10 @@ def cpstream(dims, cidx, vals, stream_mode, rank, max_iters, eps, nonneg, init_f
     if n < 0:
         check(n, "cpstream")
     outs = [o[: int(out_dims[m])] for m, o in enumerate(outs)]
-    return dict(factors=outs, dims=out_dims, time_rows=trows[:n], outer_iters=outer[:n],
-                admm_iters=admm[:n], grams=grams, slices=n, timings=tim, final_error=float(ferr[0]))
+    nt = int(tim[3])
+    return dict(factors=outs, dims=out_dims, time_rows=trows[:nt], outer_iters=outer[:n],
+                admm_iters=admm[:n], grams=grams, slices=n, time_slices=nt, timings=tim,
+                final_error=float(ferr[0]))
 
 
 def stream_prepare(dims, cidx, vals, stream_mode, stream=None):
@@ -402,7 +410,7 @@ def stream_prepare(dims, cidx, vals, stream_mode, stream=None):
 
 
 def spcpstream(dims, cidx, vals, stream_mode, rank, max_iters, eps, init_factors, max_slices=0,
-               prec=PREC_F64, stream=None):
+               prec=PREC_F64, nnz_threshold=0, timeslice_limit=1, stream=None):
     """spCP-stream (cpstream with use_spcpstream, src/streaming_cpd.cpp:26-366,
     spcpstream_alto_iter 466-1085), host COO input.  Outputs as cpstream,
     without ADMM counts."""
@@ -414,7 +422,7 @@ def spcpstream(dims, cidx, vals, stream_mode, rank, max_iters, eps, init_factors
     for i, f in enumerate(init):
         iarr[i] = f.ctypes.data_as(_f64p)
     nsl = int(dims[stream_mode])
-    outs = [np.zeros((int(dims[m]) if m != stream_mode else 1, rank)) for m in range(N)]
+    outs = [np.zeros((int(dims[m]), rank)) for m in range(N)]
     oarr = (_f64p * N)()
     for i, f in enumerate(outs):
         oarr[i] = f.ctypes.data_as(_f64p)
@@ -426,6 +434,7 @@ def spcpstream(dims, cidx, vals, stream_mode, rank, max_iters, eps, init_factors
     ferr = np.zeros(1)
     n = lib().alto_hip_spcpstream(ctypes.byref(c), ctypes.c_int(stream_mode), ctypes.c_int(rank),
                                   ctypes.c_int(max_iters), ctypes.c_double(eps), ctypes.c_int(max_slices),
+                                  ctypes.c_uint64(nnz_threshold), ctypes.c_uint64(timeslice_limit),
                                   ctypes.c_int(prec), iarr, oarr, out_dims.ctypes.data_as(_u64p),
                                   trows.ctypes.data_as(_f64p), outer.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                                   grams.ctypes.data_as(_f64p), ferr.ctypes.data_as(_f64p),
@@ -433,8 +442,9 @@ def spcpstream(dims, cidx, vals, stream_mode, rank, max_iters, eps, init_factors
     if n < 0:
         check(n, "spcpstream")
     outs = [o[: int(out_dims[m])] for m, o in enumerate(outs)]
-    return dict(factors=outs, dims=out_dims, time_rows=trows[:n], outer_iters=outer[:n], grams=grams,
-                slices=n, timings=tim, final_error=float(ferr[0]))
+    nt = int(tim[3])
+    return dict(factors=outs, dims=out_dims, time_rows=trows[:nt], outer_iters=outer[:n], grams=grams,
+                slices=n, time_slices=nt, timings=tim, final_error=float(ferr[0]))
 
 
 def rowsparse_mttkrp(dims, cidx, vals, mode, stream_mode, rank, rsp_vals, ridx, stream=None):

@@ -506,3 +506,70 @@ extern "C" int alto_hip_pseudo_inverse(int nmode, int rank, const double *grams,
     if (fallback) *fallback = hf;
     return ALTO_OK;
 }
+
+namespace alto {
+
+// partials[b * 3R + 2R + c] = lambda_c * sum over the rows of block b of
+// U[i, c] * MT[i, c] (the layout k_fit reads).
+__global__ __launch_bounds__(256) void k_fit_inner(const double *__restrict__ U, const double *__restrict__ MT,
+                                                   const double *__restrict__ lambda, uint64_t rows, int R,
+                                                   double *__restrict__ part) {
+    const int c = threadIdx.x;
+    if (c >= R) return;
+    const uint64_t r0 = (uint64_t)blockIdx.x * 64, r1 = r0 + 64 < rows ? r0 + 64 : rows;
+    double s = 0.0;
+    for (uint64_t i = r0; i < r1; ++i) s += U[i * R + c] * MT[i * R + c];
+    part[(size_t)blockIdx.x * 3 * R + 2 * R + c] = lambda[c] * s;
+}
+
+// Symmetrises nmode R x R matrices from their row-major upper triangles.
+__global__ void k_sym_upper(double *__restrict__ g, int nmode, int R) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nmode * R * R) return;
+    const int m = e / (R * R), ij = e % (R * R), i = ij / R, j = ij % R;
+    if (i > j) g[e] = g[(size_t)m * R * R + j * R + i];
+}
+
+}  // namespace alto
+
+extern "C" int alto_hip_cpd_fit(int nmode, int rank, uint64_t rows, const double *U_last,
+                                const double *mttkrp_last, const double *lambda, const double *grams,
+                                double normX, int fit_mode, double *fit, void *stream) {
+    if (nmode < 2 || nmode > kMaxModes || rank < 1 || rank > kMaxRank || !U_last || !mttkrp_last ||
+        !lambda || !grams || !fit || (rows && !U_last))
+        return ALTO_ERR_ARG;
+    if (fit_mode != ALTO_FIT_EXACT && fit_mode != ALTO_FIT_REFERENCE_DIAG) return ALTO_ERR_ARG;
+    ALTO_TRY(ensure_init());
+    hipStream_t s = to_stream(stream);
+    const int R = rank;
+    const size_t RR = (size_t)R * R;
+    const uint64_t nb = rows ? (rows + 63) / 64 : 1;
+    if (nb > (1ULL << 31)) return ALTO_ERR_UNSUPPORTED;
+    DevBuf dU, dM, dl, dg, dp, dn, df;
+    ALTO_TRY(dU.alloc(sizeof(double) * (rows ? rows : 1) * R));
+    ALTO_TRY(dM.alloc(sizeof(double) * (rows ? rows : 1) * R));
+    ALTO_TRY(dl.alloc(sizeof(double) * R));
+    ALTO_TRY(dg.alloc(sizeof(double) * nmode * RR));
+    ALTO_TRY(dp.alloc(sizeof(double) * nb * 3 * R));
+    ALTO_TRY(dn.alloc(sizeof(double)));
+    ALTO_TRY(df.alloc(sizeof(double) * 2));
+    if (rows) {
+        ALTO_HIP_TRY(hipMemcpyAsync(dU.p, U_last, sizeof(double) * rows * R, hipMemcpyHostToDevice, s));
+        ALTO_HIP_TRY(hipMemcpyAsync(dM.p, mttkrp_last, sizeof(double) * rows * R, hipMemcpyHostToDevice, s));
+    }
+    ALTO_HIP_TRY(hipMemcpyAsync(dl.p, lambda, sizeof(double) * R, hipMemcpyHostToDevice, s));
+    ALTO_HIP_TRY(hipMemcpyAsync(dg.p, grams, sizeof(double) * nmode * RR, hipMemcpyHostToDevice, s));
+    ALTO_HIP_TRY(hipMemcpyAsync(dn.p, &normX, sizeof(double), hipMemcpyHostToDevice, s));
+    ALTO_TRY(launch_fill_zero(dp.as<double>(), nb * 3 * R, s));
+    if (rows) k_fit_inner<<<(unsigned)nb, 256, 0, s>>>(dU.as<double>(), dM.as<double>(), dl.as<double>(), rows, R,
+                                                      dp.as<double>());
+    k_sym_upper<<<cdiv(nmode * RR, 256), 256, 0, s>>>(dg.as<double>(), nmode, R);
+    k_fit<<<1, 256, sizeof(double) * (RR + R), s>>>(dg.as<double>(), nmode, R, dl.as<double>(), dp.as<double>(),
+                                                   (int)nb, dn.as<double>(), fit_mode, df.as<double>(), nullptr, 0);
+    ALTO_LAUNCH_CHECK();
+    double out[2];
+    ALTO_HIP_TRY(hipMemcpyAsync(out, df.p, sizeof(out), hipMemcpyDeviceToHost, s));
+    ALTO_HIP_TRY(hipStreamSynchronize(s));
+    *fit = out[0];
+    return ALTO_OK;
+}

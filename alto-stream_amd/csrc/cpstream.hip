@@ -1,7 +1,9 @@
 // CP-stream on the GPU: cpstream (src/streaming_cpd.cpp:26-366) with the
-// CPSTREAM_ALTO model, one time index per slice (timeslice limit 1, no nnz
-// threshold), driving cpstream_alto_iter (streaming_cpd.cpp:1198-1519) and
-// admm_cf_ (src/admm.cpp:216-429).
+// CPSTREAM_ALTO model, driving cpstream_alto_iter (streaming_cpd.cpp:
+// 1198-1519) and admm_cf_ (src/admm.cpp:216-429).  Time slices are grouped
+// into batches as next_dynamic_batch does (streaming_sptensor.cpp:196-264):
+// consecutive time indices until timeslice_limit of them or at least
+// nnz_threshold nonzeros; a batch's stream mode has one row per time index.
 //
 // Host side (C++): StreamingSparseTensor (streaming_sptensor.cpp:69-264):
 // stable lexicographic sort by (s, s+1, ...) mod N, first-appearance
@@ -274,9 +276,11 @@ __global__ void k_zero_flag(int *p) {
     if (threadIdx.x == 0) p[0] = 0;
 }
 
-__global__ void k_zero_u64(uint64_t *p, uint64_t n) {
+// Stream-mode index of a batch's nonzeros: time index - the batch's first.
+__global__ void k_local_time(uint64_t *__restrict__ out, const uint64_t *__restrict__ t, uint64_t n,
+                             uint64_t t0) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
+    if (i < n) out[i] = t[i] - t0;
 }
 
 static constexpr int kAdmmMaxIter = 25;
@@ -286,7 +290,8 @@ static constexpr int kAdmmMaxIter = 25;
 using namespace alto;
 
 extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_iters,
-                                 double epsilon, int nonneg, int max_slices, int prec,
+                                 double epsilon, int nonneg, int max_slices, uint64_t nnz_threshold,
+                                 uint64_t timeslice_limit, int prec,
                                  const double *const *init_factors, double *const *out_factors,
                                  uint64_t *out_dims, double *out_time_rows, int *out_outer_iters,
                                  double *out_admm_iters, double *out_grams,
@@ -307,9 +312,10 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
     // ---- StreamingSparseTensor: sort by (s, s+1, ...), relabel, slices ----
     StreamInput sin;
     ALTO_TRY(stream_prepare(coo, s, &sin, st));
-    const auto &slices = sin.slices;
-    int nsl = (int)slices.size();
+    const std::vector<Slice> slices = stream_batches(sin.slices, s, nnz_threshold, timeslice_limit);
+    int nsl = (int)slices.size();  // batches
     if (max_slices > 0 && nsl > max_slices) nsl = max_slices;
+    const uint64_t ntime = nsl ? slices[nsl - 1].t0 + slices[nsl - 1].dims[s] : 0;  // time indices processed
 
     // ---- device state ----
     uint64_t maxd[kMaxModes], maxall = 1, maxslice = 1;
@@ -326,7 +332,6 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
     DevBuf *dci = sin.cid;
     DevBuf &dvals = sin.vals;
     ALTO_TRY(zidx.alloc(sizeof(uint64_t) * maxslice));
-    k_zero_u64<<<cdiv(maxslice, 256), 256, 0, st>>>(zidx.as<uint64_t>(), maxslice);
     for (int m = 0; m < N; ++m) {
         ALTO_TRY(U[m].alloc(sizeof(double) * maxd[m] * R));
         ALTO_TRY(P[m].alloc(sizeof(double) * maxd[m] * R));
@@ -352,8 +357,8 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
     ALTO_TRY(colp.alloc(sizeof(double) * 3 * R * kMaxBlocks));
     ALTO_TRY(lam.alloc(sizeof(double) * R));
     DevBuf trows;
-    ALTO_TRY(trows.alloc(sizeof(double) * std::max(nsl, 1) * R));
-    ALTO_TRY(launch_fill_zero(trows.as<double>(), (uint64_t)std::max(nsl, 1) * R, st));
+    ALTO_TRY(trows.alloc(sizeof(double) * std::max<uint64_t>(ntime, 1) * R));
+    ALTO_TRY(launch_fill_zero(trows.as<double>(), std::max<uint64_t>(ntime, 1) * R, st));
     TensorImpl at;
     at.keep_sched_scratch = true;  // rebuilt for every slice
     const TileShape ts = tile_shape(R);
@@ -372,8 +377,9 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 256);
         });
     }
+    EventSet evs;
     hipEvent_t ev[4];
-    for (auto &e : ev) ALTO_HIP_TRY(hipEventCreate(&e));
+    for (auto &e : ev) ALTO_TRY(evs.make(&e));
     double t_build = 0, t_mttkrp = 0;
     ALTO_HIP_TRY(hipEventRecord(ev[0], st));
 
@@ -405,7 +411,7 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
                                             hipMemcpyHostToDevice, st));
                 ALTO_TRY(launch_fill_zero(P[m].as<double>(), maxd[m] * R, st));
             }
-            ALTO_TRY(launch_fill_zero(U[s].as<double>(), R, st));
+            ALTO_TRY(launch_fill_zero(U[s].as<double>(), bd[s] * R, st));
             for (int m = 0; m < N; ++m) ALTO_TRY(gram(m, bd[m]));
             ALTO_TRY(launch_fill_zero(G.as<double>() + s * RR, RR, st));
         } else {
@@ -426,6 +432,9 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
         // update_salto (alto.hpp:195-299) into reused buffers
         {
             const uint64_t *cp[kMaxModes];
+            k_local_time<<<cdiv(sl.end - sl.start, 256), 256, 0, st>>>(
+                zidx.as<uint64_t>(), dci[s].as<uint64_t>() + sl.start, sl.end - sl.start, sl.t0);
+            ALTO_LAUNCH_CHECK();
             for (int m = 0; m < N; ++m)
                 cp[m] = m == s ? zidx.as<uint64_t>() : dci[m].as<uint64_t>() + sl.start;
             alto_hip_coo sc{N, bd, sl.end - sl.start, cp, dvals.as<double>() + sl.start, 1};
@@ -544,7 +553,7 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
             ALTO_HIP_TRY(hipMemcpyAsync(hf, flag.p, sizeof(int) * 3, hipMemcpyDeviceToHost, st));
             ALTO_HIP_TRY(hipStreamSynchronize(st));
             if (nonneg && hf[2]) {
-                set_error("Cholesky factorisation failed in ADMM (mode %d, slice %d)", -1, k);
+                set_error("Cholesky factorisation failed in ADMM (a mode's Gram Hadamard, batch %d)", k);
                 return ALTO_ERR_NUMERIC;
             }
             float bms = 0;
@@ -560,11 +569,11 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
                                         hipMemcpyDeviceToDevice, st));
         double ait = 0.0;
         if (out_time_rows)
-            ALTO_HIP_TRY(hipMemcpyAsync(out_time_rows + (size_t)k * R, U[s].p, sizeof(double) * R,
+            ALTO_HIP_TRY(hipMemcpyAsync(out_time_rows + (size_t)sl.t0 * R, U[s].p, sizeof(double) * bd[s] * R,
                                         hipMemcpyDeviceToHost, st));
         if (out_final_error)
-            ALTO_HIP_TRY(hipMemcpyAsync(trows.as<double>() + (size_t)k * R, U[s].p, sizeof(double) * R,
-                                        hipMemcpyDeviceToDevice, st));
+            ALTO_HIP_TRY(hipMemcpyAsync(trows.as<double>() + (size_t)sl.t0 * R, U[s].p,
+                                        sizeof(double) * bd[s] * R, hipMemcpyDeviceToDevice, st));
         ALTO_HIP_TRY(hipMemcpyAsync(&ait, admm_it.p, sizeof(double), hipMemcpyDeviceToHost, st));
         ALTO_HIP_TRY(hipStreamSynchronize(st));
         if (out_outer_iters) out_outer_iters[k] = outer;
@@ -584,7 +593,7 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
         for (int m = 0; m < N; ++m) {
             cp[m] = m == s ? tidx.as<uint64_t>() : dci[m].as<uint64_t>();
             fp[m] = m == s ? trows.as<double>() : U[m].as<double>();
-            rows[m] = m == s ? (uint64_t)std::max(nsl, 1) : std::max<uint64_t>(cur[m], 1);
+            rows[m] = m == s ? std::max<uint64_t>(ntime, 1) : std::max<uint64_t>(cur[m], 1);
         }
         double eo[4];
         ALTO_TRY(cpd_error_dev(N, rows, used, cp, dvals.as<double>(), fp, R, st, eo));
@@ -599,12 +608,11 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
     if (out_grams)
         ALTO_HIP_TRY(hipMemcpyAsync(out_grams, G.p, sizeof(double) * N * RR, hipMemcpyDeviceToHost, st));
     ALTO_HIP_TRY(hipStreamSynchronize(st));
-    for (auto &e : ev) (void)hipEventDestroy(e);
     if (timings) {
         timings[0] = tot_ms * 1e-3;
         timings[1] = t_build;
         timings[2] = t_mttkrp;
-        timings[3] = (double)nsl;
+        timings[3] = (double)ntime;
     }
     return nsl;
 }

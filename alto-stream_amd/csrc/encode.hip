@@ -532,6 +532,12 @@ using namespace alto;
 
 extern "C" {
 
+int alto_hip_tensor_update(alto_hip_tensor t, const alto_hip_coo *coo, int nprtn, void *stream) {
+    if (!t || !coo || nprtn < 1) return ALTO_ERR_ARG;
+    ALTO_TRY(ensure_init());
+    return build_alto(&t->impl, coo, nprtn, 0, true, to_stream(stream));
+}
+
 int alto_hip_tensor_create(const alto_hip_coo *coo, int nprtn, int key_words, void *stream,
                            alto_hip_tensor *out) {
     if (!coo || !out || nprtn < 1) return ALTO_ERR_ARG;

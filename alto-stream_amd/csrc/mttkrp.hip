@@ -1166,27 +1166,42 @@ using namespace alto;
 
 extern "C" {
 
+// The output must hold dims[target] x rank elements of what the kernels
+// write: FP64 for ALTO_PREC_F64 or out_f64, else FP32.
+static int check_out_bytes(alto_hip_tensor t, int target, int prec, int rank, int out_f64,
+                           size_t out_bytes) {
+    if (!t || target < 0 || target >= t->impl.nmode || rank < 1) return ALTO_ERR_ARG;
+    const size_t esz = (prec == ALTO_PREC_F64 || out_f64) ? 8 : 4;
+    const size_t need = (size_t)t->impl.dims[target] * (size_t)rank * esz;
+    if (out_bytes < need) {
+        set_error("MTTKRP output holds %zu bytes, mode %d needs %llu x %d x %zu = %zu", out_bytes, target,
+                  (unsigned long long)t->impl.dims[target], rank, esz, need);
+        return ALTO_ERR_ARG;
+    }
+    return ALTO_OK;
+}
+
 int alto_hip_mttkrp_algo(alto_hip_tensor t, int target, int prec, int rank,
-                         const void *const *factors, void *out, int out_f64, int algo,
+                         const void *const *factors, void *out, int out_f64, size_t out_bytes, int algo,
                          void *stream) {
-    if (!t) return ALTO_ERR_ARG;
+    ALTO_TRY(check_out_bytes(t, target, prec, rank, out_f64, out_bytes));
     ALTO_TRY(ensure_init());
     return run_mttkrp(&t->impl, target, prec, rank, factors, out, out_f64, algo, 0, -1,
                       to_stream(stream));
 }
 
 int alto_hip_mttkrp_config(alto_hip_tensor t, int target, int prec, int rank,
-                           const void *const *factors, void *out, int out_f64, int algo,
+                           const void *const *factors, void *out, int out_f64, size_t out_bytes, int algo,
                            uint64_t segment, int flags, void *stream) {
-    if (!t) return ALTO_ERR_ARG;
+    ALTO_TRY(check_out_bytes(t, target, prec, rank, out_f64, out_bytes));
     ALTO_TRY(ensure_init());
     return run_mttkrp(&t->impl, target, prec, rank, factors, out, out_f64, algo, segment, flags,
                       to_stream(stream));
 }
 
 int alto_hip_mttkrp(alto_hip_tensor t, int target, int prec, int rank, const void *const *factors,
-                    void *out, int out_f64, void *stream) {
-    return alto_hip_mttkrp_algo(t, target, prec, rank, factors, out, out_f64, 0, stream);
+                    void *out, int out_f64, size_t out_bytes, void *stream) {
+    return alto_hip_mttkrp_algo(t, target, prec, rank, factors, out, out_f64, out_bytes, 0, stream);
 }
 
 int alto_hip_mttkrp_coo(const alto_hip_coo *coo, int target, int rank,

@@ -1,6 +1,8 @@
 // spCP-stream on the GPU: cpstream with use_spcpstream (model SPCPSTREAM_ALTO,
 // src/streaming_cpd.cpp:26-366, main.cpp:163, 397) driving
-// spcpstream_alto_iter (streaming_cpd.cpp:466-1085), one time index per slice.
+// spcpstream_alto_iter (streaming_cpd.cpp:466-1085).  Time slices are grouped
+// into batches as next_dynamic_batch does (streaming_sptensor.cpp:196-264);
+// a batch's stream mode has one row per time index.
 //
 // Per slice the update touches only the rows that occur in the slice:
 //   nonzero_slices (streaming_cpd.cpp:1124-1161) -> per mode the ascending
@@ -37,6 +39,19 @@ __global__ __launch_bounds__(256) void k_sp_mark(const uint64_t *__restrict__ id
                                                  uint32_t *__restrict__ flag) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) flag[idx[i]] = 1u;
+}
+
+// Stream-mode index of a batch's nonzeros (time index - the batch's first)
+// and the identity row list of its time rows.
+__global__ __launch_bounds__(256) void k_sp_local_time(uint64_t *__restrict__ out,
+                                                       const uint64_t *__restrict__ t, uint64_t n,
+                                                       uint64_t t0) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = t[i] - t0;
+}
+__global__ __launch_bounds__(256) void k_sp_iota_u64(uint64_t *__restrict__ out, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = i;
 }
 
 __global__ __launch_bounds__(256) void k_sp_zero_u32(uint32_t *__restrict__ p, uint64_t n) {
@@ -242,7 +257,8 @@ __global__ __launch_bounds__(256) void k_sp_final(double *__restrict__ U, const 
 using namespace alto;
 
 extern "C" int alto_hip_spcpstream(const alto_hip_coo *coo, int s, int R, int max_iters,
-                                   double epsilon, int max_slices, int prec,
+                                   double epsilon, int max_slices, uint64_t nnz_threshold,
+                                   uint64_t timeslice_limit, int prec,
                                    const double *const *init_factors, double *const *out_factors,
                                    uint64_t *out_dims, double *out_time_rows, int *out_outer_iters,
                                    double *out_grams, double *out_final_error, double *timings,
@@ -263,9 +279,10 @@ extern "C" int alto_hip_spcpstream(const alto_hip_coo *coo, int s, int R, int ma
 
     StreamInput sin;
     ALTO_TRY(stream_prepare(coo, s, &sin, st));
-    const auto &slices = sin.slices;
-    int nsl = (int)slices.size();
+    const std::vector<Slice> slices = stream_batches(sin.slices, s, nnz_threshold, timeslice_limit);
+    int nsl = (int)slices.size();  // batches
     if (max_slices > 0 && nsl > max_slices) nsl = max_slices;
+    const uint64_t ntime = nsl ? slices[nsl - 1].t0 + slices[nsl - 1].dims[s] : 0;  // time indices processed
 
     // ---- device state ----
     uint64_t maxd[kMaxModes], maxslice = 1;
@@ -289,8 +306,10 @@ extern "C" int alto_hip_spcpstream(const alto_hip_coo *coo, int s, int R, int ma
         ALTO_TRY(rowind[m].alloc(sizeof(uint64_t) * maxd[m]));
         ALTO_TRY(cco[m].alloc(sizeof(uint64_t) * maxslice));
     }
+    DevBuf sidx;
     ALTO_TRY(zidx.alloc(sizeof(uint64_t) * maxslice));
-    ALTO_HIP_TRY(hipMemsetAsync(zidx.p, 0, sizeof(uint64_t) * maxslice, st));
+    ALTO_TRY(sidx.alloc(sizeof(uint64_t) * maxd[s]));
+    k_sp_iota_u64<<<cdiv(maxd[s], 256), 256, 0, st>>>(sidx.as<uint64_t>(), maxd[s]);
     for (int m = 0; m < N; ++m) {
         ALTO_TRY(U[m].alloc(sizeof(double) * maxd[m] * R));
         ALTO_TRY(P[m].alloc(sizeof(double) * maxd[m] * R));
@@ -309,8 +328,8 @@ extern "C" int alto_hip_spcpstream(const alto_hip_coo *coo, int s, int R, int ma
     ALTO_TRY(lam.alloc(sizeof(double) * R));
     ALTO_TRY(delta.alloc(sizeof(double)));
     ALTO_TRY(iflag.alloc(sizeof(int) * 4));
-    ALTO_TRY(trows.alloc(sizeof(double) * std::max(nsl, 1) * R));
-    ALTO_TRY(launch_fill_zero(trows.as<double>(), (uint64_t)std::max(nsl, 1) * R, st));
+    ALTO_TRY(trows.alloc(sizeof(double) * std::max<uint64_t>(ntime, 1) * R));
+    ALTO_TRY(launch_fill_zero(trows.as<double>(), std::max<uint64_t>(ntime, 1) * R, st));
     const int TR = std::max(1, 256 / R);
     const size_t lds_rows = sizeof(double) * (2 * RR + (size_t)TR * R);
     const size_t lds_rr = sizeof(double) * (2 * RR + 2 * (size_t)R);
@@ -325,8 +344,9 @@ extern "C" int alto_hip_spcpstream(const alto_hip_coo *coo, int s, int R, int ma
     }
     TensorImpl at;
     at.keep_sched_scratch = true;  // rebuilt for every slice
+    EventSet evs;
     hipEvent_t ev[4];
-    for (auto &e : ev) ALTO_HIP_TRY(hipEventCreate(&e));
+    for (auto &e : ev) ALTO_TRY(evs.make(&e));
     double t_build = 0;
     ALTO_HIP_TRY(hipEventRecord(ev[0], st));
 
@@ -358,7 +378,7 @@ extern "C" int alto_hip_spcpstream(const alto_hip_coo *coo, int s, int R, int ma
                                             hipMemcpyHostToDevice, st));
                 ALTO_TRY(launch_fill_zero(P[m].as<double>(), maxd[m] * R, st));
             }
-            ALTO_TRY(launch_fill_zero(U[s].as<double>(), R, st));
+            ALTO_TRY(launch_fill_zero(U[s].as<double>(), bd[s] * R, st));
             ALTO_TRY(launch_fill_zero(mat(G, s), RR, st));
             // first-slice normalisation (streaming_cpd.cpp:585-598): 2-norm
             // columns of the non-stream modes, Grams of the normalised factors
@@ -406,8 +426,12 @@ extern "C" int alto_hip_spcpstream(const alto_hip_coo *coo, int s, int R, int ma
             ALTO_LAUNCH_CHECK();
         }
         ALTO_HIP_TRY(hipStreamSynchronize(st));
-        for (int m = 0; m < N; ++m) nnzr[m] = m == s ? 1 : (uint64_t)lastp[m] + lastf[m];
-        // the slice over compact coordinates (update_salto_rowsparse)
+        for (int m = 0; m < N; ++m) nnzr[m] = m == s ? bd[s] : (uint64_t)lastp[m] + lastf[m];
+        // the batch over compact coordinates (update_salto_rowsparse); every
+        // time index of a batch has nonzeros, so its stream rows are all used
+        k_sp_local_time<<<cdiv(bn, 256), 256, 0, st>>>(zidx.as<uint64_t>(), dci[s].as<uint64_t>() + sl.start,
+                                                       bn, sl.t0);
+        ALTO_LAUNCH_CHECK();
         {
             const uint64_t *cp[kMaxModes];
             for (int m = 0; m < N; ++m) cp[m] = m == s ? zidx.as<uint64_t>() : cco[m].as<uint64_t>();
@@ -417,7 +441,7 @@ extern "C" int alto_hip_spcpstream(const alto_hip_coo *coo, int s, int R, int ma
         ALTO_HIP_TRY(hipEventRecord(ev[3], st));
         // ---- row-sparse matrices and the c / h Grams (streaming_cpd.cpp:607-656) ----
         for (int m = 0; m < N; ++m) {
-            const uint64_t *ri = m == s ? zidx.as<uint64_t>() : rowind[m].as<uint64_t>();
+            const uint64_t *ri = m == s ? sidx.as<uint64_t>() : rowind[m].as<uint64_t>();
             k_sp_gather<<<cdiv(nnzr[m] * R, 256), 256, 0, st>>>(P[m].as<double>(), ri, nnzr[m], R,
                                                                  Ap[m].as<double>());
             k_sp_gather<<<cdiv(nnzr[m] * R, 256), 256, 0, st>>>(U[m].as<double>(), ri, nnzr[m], R,
@@ -490,7 +514,8 @@ extern "C" int alto_hip_spcpstream(const alto_hip_coo *coo, int s, int R, int ma
         // ---- z rows, nz rows, forgetting, previous Grams (1017-1058) ----
         for (int m = 0; m < N; ++m) {
             if (m == s) {
-                ALTO_HIP_TRY(hipMemcpyAsync(U[s].p, A[s].p, sizeof(double) * R, hipMemcpyDeviceToDevice, st));
+                ALTO_HIP_TRY(hipMemcpyAsync(U[s].p, A[s].p, sizeof(double) * bd[s] * R, hipMemcpyDeviceToDevice,
+                                            st));
                 continue;
             }
             const uint64_t n = bd[m] * R;
@@ -507,9 +532,9 @@ extern "C" int alto_hip_spcpstream(const alto_hip_coo *coo, int s, int R, int ma
         for (int m = 0; m < N; ++m)
             ALTO_HIP_TRY(hipMemcpyAsync(P[m].p, U[m].p, sizeof(double) * bd[m] * R, hipMemcpyDeviceToDevice, st));
         if (out_time_rows)
-            ALTO_HIP_TRY(hipMemcpyAsync(out_time_rows + (size_t)k * R, U[s].p, sizeof(double) * R,
+            ALTO_HIP_TRY(hipMemcpyAsync(out_time_rows + (size_t)sl.t0 * R, U[s].p, sizeof(double) * bd[s] * R,
                                         hipMemcpyDeviceToHost, st));
-        ALTO_HIP_TRY(hipMemcpyAsync(trows.as<double>() + (size_t)k * R, U[s].p, sizeof(double) * R,
+        ALTO_HIP_TRY(hipMemcpyAsync(trows.as<double>() + (size_t)sl.t0 * R, U[s].p, sizeof(double) * bd[s] * R,
                                     hipMemcpyDeviceToDevice, st));
         outer_v[k] = outer;
     }
@@ -528,7 +553,7 @@ extern "C" int alto_hip_spcpstream(const alto_hip_coo *coo, int s, int R, int ma
         for (int m = 0; m < N; ++m) {
             cp[m] = m == s ? tidx.as<uint64_t>() : dci[m].as<uint64_t>();
             fp[m] = m == s ? trows.as<double>() : U[m].as<double>();
-            rows[m] = m == s ? (uint64_t)std::max(nsl, 1) : std::max<uint64_t>(cur[m], 1);
+            rows[m] = m == s ? std::max<uint64_t>(ntime, 1) : std::max<uint64_t>(cur[m], 1);
         }
         double eo[4];
         ALTO_TRY(cpd_error_dev(N, rows, used, cp, dvals.as<double>(), fp, R, st, eo));
@@ -543,12 +568,11 @@ extern "C" int alto_hip_spcpstream(const alto_hip_coo *coo, int s, int R, int ma
     if (out_grams)
         ALTO_HIP_TRY(hipMemcpyAsync(out_grams, G.p, sizeof(double) * N * RR, hipMemcpyDeviceToHost, st));
     ALTO_HIP_TRY(hipStreamSynchronize(st));
-    for (auto &e : ev) (void)hipEventDestroy(e);
     if (timings) {
         timings[0] = tot_ms * 1e-3;
         timings[1] = t_build;
         timings[2] = 0.0;
-        timings[3] = (double)nsl;
+        timings[3] = (double)ntime;
     }
     return nsl;
 }

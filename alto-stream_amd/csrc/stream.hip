@@ -275,6 +275,7 @@ int stream_prepare(const alto_hip_coo *coo, int s, StreamInput *out, hipStream_t
             out->slices[t].end = a;
             ends[t] = a;
             out->slices[t].dims[s] = 1;
+            out->slices[t].t0 = t;
         }
     }
     DevBuf dends, dcnt;
@@ -353,3 +354,33 @@ extern "C" int alto_hip_stream_prepare(const alto_hip_coo *coo, int stream_mode,
     }
     return ALTO_OK;
 }
+
+namespace alto {
+
+// next_dynamic_batch (src/streaming_sptensor.cpp:196-264): from the first
+// time index not yet taken, take whole time slices while fewer than
+// nnz_threshold nonzeros are taken, stopping after timeslice_limit of them.
+std::vector<Slice> stream_batches(const std::vector<Slice> &slices, int s, uint64_t nnz_threshold,
+                                  uint64_t timeslice_limit) {
+    const uint64_t thr = nnz_threshold ? nnz_threshold : ~0ULL;
+    const uint64_t lim = timeslice_limit ? timeslice_limit : 1;
+    std::vector<Slice> out;
+    size_t k = 0;
+    while (k < slices.size()) {
+        Slice b = slices[k];
+        uint64_t taken = 0, cnt = 0;
+        do {
+            taken += slices[k].end - slices[k].start;
+            ++cnt;
+            ++k;
+        } while (k < slices.size() && taken < thr && cnt < lim);
+        const Slice &last = slices[k - 1];
+        b.end = last.end;
+        for (int m = 0; m < kMaxModes; ++m) b.dims[m] = last.dims[m];
+        b.dims[s] = cnt;
+        out.push_back(b);
+    }
+    return out;
+}
+
+}  // namespace alto

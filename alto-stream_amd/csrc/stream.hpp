@@ -13,6 +13,7 @@ namespace alto {
 struct Slice {
     uint64_t start, end;
     uint64_t dims[kMaxModes];
+    uint64_t t0;  // first time index (a batch holds time indices t0 .. t0 + dims[s] - 1)
 };
 
 // StreamingSparseTensor (src/streaming_sptensor.cpp:69-264) with
@@ -23,6 +24,30 @@ struct StreamInput {
     DevBuf cid[kMaxModes];  // device, per mode, relabelled (stream mode: time index)
     DevBuf vals;            // device
     std::vector<Slice> slices;
+};
+
+// Groups consecutive time slices into batches as next_dynamic_batch does
+// (src/streaming_sptensor.cpp:196-264): at most timeslice_limit slices (0
+// counts as 1), and no further slice once nnz_threshold nonzeros are taken
+// (0: no threshold).  A batch's dims are those after its last slice, with
+// dims[s] = its number of time slices.
+std::vector<Slice> stream_batches(const std::vector<Slice> &slices, int s, uint64_t nnz_threshold,
+                                  uint64_t timeslice_limit);
+
+// HIP events destroyed with the set (also on every early return).
+struct EventSet {
+    std::vector<hipEvent_t> ev;
+    EventSet() = default;
+    EventSet(const EventSet &) = delete;
+    EventSet &operator=(const EventSet &) = delete;
+    ~EventSet() {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+    int make(hipEvent_t *e) {
+        ALTO_HIP_TRY(hipEventCreate(e));
+        ev.push_back(*e);
+        return ALTO_OK;
+    }
 };
 
 // Host or device COO in (nnz < 2^32); rejects coordinates >= dims

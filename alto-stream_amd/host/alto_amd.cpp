@@ -257,6 +257,29 @@ DeviceFactors::DeviceFactors(int nmode_, const IType *dims_, IType rank_, int pr
 
 DeviceFactors::~DeviceFactors() {
     for (auto p : U) alto_hip_free(p);
+    if (acc) alto_hip_free(acc);
+}
+
+size_t DeviceFactors::bytes(int m) const { return esize(prec) * dims[m] * rank; }
+
+void DeviceFactors::mttkrp(int target, alto_hip_tensor t) {
+    const size_t n = dims[target] * rank;
+    if (prec != ALTO_PREC_BF16) {
+        check(alto_hip_mttkrp(t, target, prec, (int)rank, (const void *const *)U.data(), U[target], 0,
+                              bytes(target), nullptr),
+              "mttkrp_alto_par");
+        return;
+    }
+    if (acc_bytes < 4 * n) {
+        if (acc) alto_hip_free(acc);
+        acc = nullptr;
+        check(alto_hip_malloc(&acc, 4 * n), "malloc");
+        acc_bytes = 4 * n;
+    }
+    check(alto_hip_convert(acc, ALTO_PREC_F32, U[target], ALTO_PREC_BF16, n, nullptr), "convert");
+    check(alto_hip_mttkrp(t, target, prec, (int)rank, (const void *const *)U.data(), acc, 0, acc_bytes, nullptr),
+          "mttkrp_alto_par");
+    check(alto_hip_convert(U[target], ALTO_PREC_BF16, acc, ALTO_PREC_F32, n, nullptr), "convert");
 }
 
 void DeviceFactors::upload(int m, const FType *host) {
@@ -360,10 +383,44 @@ RowSparseMatrix *rowsparse_mttkrp(SparseTensor *X, RowSparseMatrix **rsp_mats, I
     return M;
 }
 
+// ---- constraints.cpp:42-68 ----
+cpd_constraint *init_constraint() {
+    cpd_constraint *con = (cpd_constraint *)calloc(1, sizeof(cpd_constraint));
+    con->solve_type = UNCONSTRAINED;
+    snprintf(con->description, sizeof(con->description), "UNCONSTRAINED");
+    return con;
+}
+void free_constraint(cpd_constraint *con) { free(con); }
+void apply_nonneg_constraint(cpd_constraint *con) {
+    snprintf(con->description, sizeof(con->description), "NON-NEGATIVE");
+    con->solve_type = NON_NEGATIVE;
+}
+void apply_lasso_constraint(cpd_constraint *con) {
+    snprintf(con->description, sizeof(con->description), "LASSO (SPARSITY)");
+    con->solve_type = LASSO;
+}
+
+// ---- streaming_cpd.cpp:368-464 ----
+double cpd_error(SparseTensor *X, KruskalModel *M) {
+    alto_hip_coo c{X->nmodes, (const uint64_t *)X->dims, X->nnz, (const uint64_t *const *)X->cidx, X->vals, 0};
+    double err = 0.0;
+    check(alto_hip_cpd_error(&c, (int)M->rank, (const double *const *)M->U, &err, nullptr, nullptr), "cpd_error");
+    for (IType r = 0; r < M->rank; ++r) M->lambda[r] = 1.0;
+    return err;
+}
+
 // ---- streaming_cpd.cpp:26-366 ----
-int cpstream(SparseTensor *X, int rank, int max_iters, int streaming_mode, FType epsilon, IType seed,
-             bool nonneg, int max_slices, int prec, bool use_spcpstream) {
+void cpstream(SparseTensor *X, int rank, int max_iters, int streaming_mode, FType epsilon, IType seed,
+              cpd_constraint *con, bool use_spcpstream, IType nnz_threshold, IType timeslice_limit) {
     (void)seed;  // the reference seeds once with srand (main.cpp:188)
+    const bool nonneg = con && (con->solve_type == NON_NEGATIVE || con->solve_type == LASSO);
+    cpstream_amd(X, rank, max_iters, streaming_mode, epsilon, nonneg, use_spcpstream, nnz_threshold,
+                 timeslice_limit);
+    DestroySparseTensor(X);  // as the reference (streaming_cpd.cpp:348)
+}
+
+int cpstream_amd(SparseTensor *X, int rank, int max_iters, int streaming_mode, FType epsilon, bool nonneg,
+                 bool use_spcpstream, IType nnz_threshold, IType timeslice_limit, int max_slices, int prec) {
     const int N = X->nmodes;
     std::vector<std::vector<double>> init(N);
     std::vector<const double *> ip(N);
@@ -378,20 +435,27 @@ int cpstream(SparseTensor *X, int rank, int max_iters, int streaming_mode, FType
     std::vector<double> admm(X->dims[streaming_mode]);
     double tim[4] = {0, 0, 0, 0};
     double err = 0.0;
+    printf("Processing Streaming Sparse Tensor (nnz threshold: %llu, timeslice_limit: %llu)\n",
+           (unsigned long long)nnz_threshold, (unsigned long long)timeslice_limit);
     fprintf(stderr, "==== Executing %s (ALTO) ====\n", use_spcpstream ? "spCPSTREAM" : "CPSTREAM");
+    // the C ABI reads 0 as "no threshold"; the reference's default is ULLONG_MAX
+    const uint64_t thr = nnz_threshold == ~0ULL ? 0 : (uint64_t)nnz_threshold;
     int n = use_spcpstream
-                ? alto_hip_spcpstream(&c, streaming_mode, rank, max_iters, epsilon, max_slices, prec, ip.data(),
-                                      nullptr, nullptr, trows.data(), outer.data(), nullptr, &err, tim, nullptr)
-                : alto_hip_cpstream(&c, streaming_mode, rank, max_iters, epsilon, nonneg ? 1 : 0, max_slices,
-                                    prec, ip.data(), nullptr, nullptr, trows.data(), outer.data(), admm.data(),
-                                    nullptr, &err, tim, nullptr);
+                ? alto_hip_spcpstream(&c, streaming_mode, rank, max_iters, epsilon, max_slices, thr,
+                                      (uint64_t)timeslice_limit, prec, ip.data(), nullptr, nullptr, trows.data(),
+                                      outer.data(), nullptr, &err, tim, nullptr)
+                : alto_hip_cpstream(&c, streaming_mode, rank, max_iters, epsilon, nonneg ? 1 : 0, max_slices, thr,
+                                    (uint64_t)timeslice_limit, prec, ip.data(), nullptr, nullptr, trows.data(),
+                                    outer.data(), admm.data(), nullptr, &err, tim, nullptr);
     if (n < 0) altoamd::fail(use_spcpstream ? "spcpstream" : "cpstream", n);
     for (int k = 0; k < n; ++k) {
-        if (use_spcpstream) fprintf(stderr, "slice %d: outer iterations %d\n", k, outer[k]);
-        else fprintf(stderr, "slice %d: outer iterations %d, ADMM iterations %.0f\n", k, outer[k], admm[k]);
+        if (use_spcpstream) fprintf(stderr, "batch %d: outer iterations %d\n", k, outer[k]);
+        else fprintf(stderr, "batch %d: outer iterations %d, ADMM iterations %.0f\n", k, outer[k], admm[k]);
     }
-    fprintf(stderr, "Total CPSTREAM time (#it, total, iter)\n%d\t%f\t%f\n", n, tim[0], tim[0]);
-    printf("Total CPSTREAM time: %f s (%d slices, %.2f slices/s)\n", tim[0], n, n / (tim[0] > 0 ? tim[0] : 1));
+    const int nt = (int)tim[3];
+    fprintf(stderr, "Total CPSTREAM time (#it, total, iter)\n%d\t%f\t%f\n", nt, tim[0], tim[0]);
+    printf("Total CPSTREAM time: %f s (%d time slices in %d batches, %.2f slices/s)\n", tim[0], nt, n,
+           nt / (tim[0] > 0 ? tim[0] : 1));
     printf("final fit error: %f\n", err * err);  // streaming_cpd.cpp:345
     return n;
 }

@@ -5,9 +5,13 @@
 //   KruskalModel / Create / RandomInit     include/kruskal_model.hpp:9-57
 //   AltoTensor<LIT>, create_alto,          include/alto.hpp:35-111
 //     destroy_alto, mttkrp_alto_par
+//   init_salto, update_salto,              include/alto.hpp:66-69, 78-82
+//     create_da_mem, destroy_da_mem
 //   mttkrp, mttkrp_par                     include/mttkrp.hpp:9-11
-//   cpd_alto<LIT>                          include/cpd.hpp:33-45
-//   cpstream                               include/streaming_cpd.hpp:21-33
+//   cpd_alto<LIT>, cpd_fit_alto<LIT>       include/cpd.hpp:33-38
+//   Matrix                                 include/matrix.hpp:14-20
+//   cpd_constraint and its setters         include/constraints.hpp:6-32
+//   cpstream, cpd_error                    include/streaming_cpd.hpp:21-31, 88
 // Everything runs through the C ABI of libalto_amd.so (include/alto_hip.h);
 // no function here computes on the CPU except file parsing and the glibc
 // rand() initialisation the reference uses.
@@ -16,6 +20,8 @@
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
+
+#include <omp.h>  // omp_lock_t of the reference signatures (no OpenMP call is made)
 
 #include <string>
 #include <vector>
@@ -97,6 +103,47 @@ template <typename LIT> void destroy_alto(AltoTensor<LIT> *at) {
     delete at;
 }
 
+// init_salto (alto.hpp:151-193): the streaming ALTO tensor, here built from
+// spt at once (update_salto rebuilds it).  stream_mode is kept for
+// reference-style callers; the GPU MTTKRP has no separate stream variant.
+template <typename LIT>
+void init_salto(SparseTensor *spt, AltoTensor<LIT> **at, int max_nprtn, int stream_mode) {
+    (void)stream_mode;
+    create_alto(spt, at, max_nprtn > 0 ? max_nprtn : 1);
+}
+
+// update_salto (alto.hpp:195-299): rebuilds the tensor from a new slice,
+// reusing its device buffers.
+template <typename LIT> void update_salto(SparseTensor *spt, AltoTensor<LIT> *at, int nprtn) {
+    alto_hip_coo c{spt->nmodes, (const uint64_t *)spt->dims, spt->nnz,
+                   (const uint64_t *const *)spt->cidx, spt->vals, 0};
+    altoamd::check(alto_hip_tensor_update(at->handle, &c, nprtn > 0 ? nprtn : 1, nullptr), "update_salto");
+    at->nmode = spt->nmodes;
+    at->nprtn = nprtn;
+    at->nnz = spt->nnz;
+    at->dims = (IType *)realloc(at->dims, sizeof(IType) * at->nmode);
+    for (int m = 0; m < at->nmode; ++m) at->dims[m] = spt->dims[m];
+}
+
+// create_da_mem / destroy_da_mem (alto.hpp:606-679): the reference's
+// per-partition output copies.  The GPU MTTKRP needs none (its per-mode
+// output-ordered schedules live with the tensor), so *ofibs is NULL and
+// destroy does nothing; both exist so reference call sites compile.
+template <typename LIT>
+void create_da_mem(int target_mode, IType rank, AltoTensor<LIT> *at, FType ***ofibs) {
+    (void)target_mode;
+    (void)rank;
+    (void)at;
+    *ofibs = nullptr;
+}
+template <typename LIT>
+void destroy_da_mem(AltoTensor<LIT> *at, FType **ofibs, IType rank, int target_mode) {
+    (void)at;
+    (void)ofibs;
+    (void)rank;
+    (void)target_mode;
+}
+
 // Device copies of a host model for repeated MTTKRPs (the GPU replacement
 // of the reference's da_mem/ofibs scratch, alto.hpp:606-679).
 struct DeviceFactors {
@@ -109,14 +156,21 @@ struct DeviceFactors {
     void upload(int m, const FType *host);
     void download(int m, FType *host);
     void zero(int m);
+    size_t bytes(int m) const;  // capacity of U[m]
+    // U[target] += MTTKRP with the output type the kernels write: FP64 for
+    // FP64 factors, FP32 for FP32 factors; BF16 factors accumulate through an
+    // FP32 copy of U[target] that is rounded back (the kernels never write
+    // BF16).
+    void mttkrp(int target, alto_hip_tensor t);
+    void *acc = nullptr;  // FP32 accumulation buffer of the BF16 path
+    size_t acc_bytes = 0;
 };
 
 // mttkrp_alto_par (alto.hpp:902-908) on device factors: U[target] += MTTKRP.
 template <typename LIT>
 void mttkrp_alto_par(int target, DeviceFactors &F, IType rank, AltoTensor<LIT> *at) {
-    altoamd::check(alto_hip_mttkrp(at->handle, target, F.prec, (int)rank, (const void *const *)F.U.data(),
-                                   F.U[target], 1, nullptr),
-                   "mttkrp_alto_par");
+    (void)rank;
+    F.mttkrp(target, at->handle);
 }
 
 // Host-pointer form: factors[target] += MTTKRP (copies in and out).
@@ -126,6 +180,17 @@ void mttkrp_alto_par(int target, FType **factors, IType rank, AltoTensor<LIT> *a
     for (int m = 0; m < at->nmode; ++m) F.upload(m, factors[m]);
     mttkrp_alto_par(target, F, rank, at);
     F.download(target, factors[target]);
+}
+
+// The reference's signature (alto.hpp:904): the write locks, the da_mem
+// copies and the op_sched row lists are not needed on the GPU and ignored.
+template <typename LIT>
+void mttkrp_alto_par(int target_mode, FType **factors, IType rank, AltoTensor<LIT> *at, omp_lock_t *wlocks,
+                     FType **ofibs, std::vector<std::vector<size_t>> &oidx) {
+    (void)wlocks;
+    (void)ofibs;
+    (void)oidx;
+    mttkrp_alto_par(target_mode, factors, rank, at);
 }
 
 // mttkrp.cpp:122-156 (COO MTTKRP, the reference's -c oracle) on the GPU:
@@ -153,6 +218,14 @@ void rspmat_free(RowSparseMatrix *mat);
 RowSparseMatrix *rowsparse_mttkrp(SparseTensor *X, RowSparseMatrix **rsp_mats, IType mode,
                                   IType streaming_mode, std::vector<std::vector<int>> &ridx);
 
+// ---- Matrix (include/matrix.hpp:14-20): a row-major view ----
+struct Matrix {
+    IType I;
+    IType J;
+    FType *vals;
+    int rowmajor;
+};
+
 // cpd_alto (cpd.hpp:47-191): CP-ALS on the GPU, printing the reference's lines.
 template <typename LIT>
 void cpd_alto(AltoTensor<LIT> *AT, KruskalModel *M, int max_iters, double epsilon,
@@ -164,10 +237,56 @@ void cpd_alto(AltoTensor<LIT> *AT, KruskalModel *M, int max_iters, double epsilo
                    "cpd_alto");
 }
 
-// cpstream (streaming_cpd.cpp:26-366), timeslice limit 1: CPSTREAM_ALTO, or
-// SPCPSTREAM_ALTO when use_spcpstream (spcpstream_alto_iter, 466-1085; the
-// constraint is then unused, as in the reference).  Rows are initialised
-// with the reference's rand_val in mode order over the final dims.  Returns
-// the number of slices.
-int cpstream(SparseTensor *X, int rank, int max_iters, int streaming_mode, FType epsilon, IType seed,
-             bool nonneg, int max_slices = 0, int prec = ALTO_PREC_F64, bool use_spcpstream = false);
+// cpd_fit_alto (cpd.hpp:193-275) on the GPU: the reference's relative
+// residual from the last mode's factor and MTTKRP, lambda and the Grams
+// (their upper triangles; the lower ones are read as zeros, as the
+// reference's never-written lower triangles).
+template <typename LIT>
+double cpd_fit_alto(AltoTensor<LIT> *AT, KruskalModel *M, Matrix **grams, FType *U_mttkrp, FType normAT) {
+    const int N = AT->nmode;
+    const int R = (int)M->rank;
+    std::vector<double> g((size_t)N * R * R);
+    for (int m = 0; m < N; ++m)
+        for (int e = 0; e < R * R; ++e) g[(size_t)m * R * R + e] = grams[m]->vals[e];
+    double fit = 0.0;
+    altoamd::check(alto_hip_cpd_fit(N, R, AT->dims[N - 1], M->U[N - 1], U_mttkrp, M->lambda, g.data(), normAT,
+                                    ALTO_FIT_REFERENCE_DIAG, &fit, nullptr),
+                   "cpd_fit_alto");
+    return fit;
+}
+
+// ---- cpd_constraint (include/constraints.hpp:6-32, src/constraints.cpp) ----
+// The GPU ADMM applies the non-negative projection itself, so func stays
+// NULL; LASSO projects like NON_NEGATIVE, as the reference's setter makes it
+// (constraints.cpp:63-68).
+typedef enum { UNCONSTRAINED, NON_NEGATIVE, LASSO, MAX_COL_NORM } con_solve_type;
+typedef struct {
+    con_solve_type solve_type;
+    char description[256];
+    void (*func)(FType *primal, IType nrows, IType ncols);
+} cpd_constraint;
+cpd_constraint *init_constraint();
+void free_constraint(cpd_constraint *con);
+void apply_nonneg_constraint(cpd_constraint *con);
+void apply_lasso_constraint(cpd_constraint *con);
+
+// cpstream (streaming_cpd.cpp:26-366), the reference's signature
+// (streaming_cpd.hpp:21-31): CPSTREAM_ALTO, or SPCPSTREAM_ALTO when
+// use_spcpstream (spcpstream_alto_iter, 466-1085; the constraint is then
+// unused, as in the reference).  Time slices are batched as
+// next_dynamic_batch (streaming_sptensor.cpp:196-264): nnz_threshold (the
+// reference's default ULLONG_MAX) and timeslice_limit (default 1).  Rows
+// are initialised with the reference's rand_val in mode order over the
+// final dims.  Like the reference, it destroys X (streaming_cpd.cpp:348).
+void cpstream(SparseTensor *X, int rank, int max_iters, int streaming_mode, FType epsilon, IType seed,
+              cpd_constraint *con, bool use_spcpstream, IType nnz_threshold, IType timeslice_limit);
+// The same without destroying X, with the MTTKRP precision and a batch cap
+// (max_slices batches, 0 = all).  Returns the number of batches.
+int cpstream_amd(SparseTensor *X, int rank, int max_iters, int streaming_mode, FType epsilon, bool nonneg,
+                 bool use_spcpstream, IType nnz_threshold, IType timeslice_limit, int max_slices = 0,
+                 int prec = ALTO_PREC_F64);
+
+// cpd_error (streaming_cpd.cpp:368-464) on the GPU: sets M's weights to 1,
+// as the reference does, and returns sqrt(||X - M||^2) / ||X|| (the
+// reference prints its square as the final fit error).
+double cpd_error(SparseTensor *X, KruskalModel *M);

@@ -4,8 +4,13 @@
 //   -l cpstream_alto  CP-stream (cpstream)              main.cpp:376-407
 //   -l spcpstream_alto  spCP-stream (cpstream, use_spcpstream)  main.cpp:163, 397
 //   -p                MTTKRP benchmark (BenchmarkAlto)  main.cpp:412-530
-//   -c                ALTO vs COO MTTKRP check           main.cpp:532-649
+//   -c                ALTO (GPU) vs sequential COO MTTKRP (host) check  main.cpp:532-649
+//   -g / -k           dynamic time batching of CP-stream: timeslice limit and
+//                     nnz threshold (next_dynamic_batch, main.cpp:130-131, 235-238)
 // Extra long option: --prec f64|f32|bf16 (MTTKRP / factor storage precision).
+// -d / -s / -f drive the reference's synthetic Poisson generator, which this
+// build does not include: they are refused.  -o is accepted and unused, as in
+// the reference (its model export is commented out, main.cpp:355).
 #include <getopt.h>
 #include <math.h>
 #include <stdio.h>
@@ -14,6 +19,7 @@
 #include <time.h>
 
 #include <chrono>
+#include <climits>
 #include <string>
 #include <vector>
 
@@ -55,11 +61,13 @@ static void Usage(const char *call) {
     fprintf(stderr, "\t-e or --epsilon      Convergence criteria\n");
     fprintf(stderr, "\t-x or --seed         Random value seed\n");
     fprintf(stderr, "\t-a or --streaming-mode  Streaming mode (CP-stream)\n");
+    fprintf(stderr, "\t-g or --timeslice-limit Time slices per CP-stream batch (default 1)\n");
+    fprintf(stderr, "\t-k or --nnz-threshold   Nonzeros that end a CP-stream batch (default: none)\n");
     fprintf(stderr, "\t-u or --zero-based   Assume input tensor is zero-based (default: one-based)\n");
-    fprintf(stderr, "\t-c or --check        Run ALTO (GPU) validation against COO MTTKRP (GPU)\n");
+    fprintf(stderr, "\t-c or --check        Run ALTO (GPU) validation against the sequential COO MTTKRP\n");
     fprintf(stderr, "\t-p or --bench        Run ALTO (GPU) MTTKRP benchmark\n");
     fprintf(stderr, "\t-n or --partitions   Number of ALTO line partitions (default 1)\n");
-    fprintf(stderr, "\t--prec               f64 | f32 | bf16 (default f64)\n");
+    fprintf(stderr, "\t--prec               f64 | f32 | bf16 (default f64; CP-stream: f64 | f32)\n");
 }
 
 static void PrintTensorInfo(IType rank, int max_iters, SparseTensor *X) {
@@ -83,11 +91,9 @@ static void BenchmarkAlto(SparseTensor *X, int max_iters, IType rank, int target
     printf("ALTO creation time:   %f\n", now() - t0);
     DeviceFactors F(X->nmodes, X->dims, rank, prec);
     for (int m = 0; m < X->nmodes; ++m) F.upload(m, M->U[m]);
-    auto run = [&](int mode) {
-        altoamd::check(alto_hip_mttkrp(AT->handle, mode, prec, (int)rank, (const void *const *)F.U.data(),
-                                       F.U[mode], 0, nullptr),
-                       "mttkrp_alto_par");
-    };
+    // U[mode] += MTTKRP in the factors' own storage (BF16 accumulates
+    // through an FP32 copy), as the reference's BenchmarkAlto does in FP64
+    auto run = [&](int mode) { F.mttkrp(mode, AT->handle); };
     int tm = target_mode < 0 ? 0 : target_mode;
     run(tm);  // warm-up
     altoamd::check(alto_hip_stream_sync(nullptr), "sync");
@@ -106,6 +112,22 @@ static void BenchmarkAlto(SparseTensor *X, int max_iters, IType rank, int target
     DestroyKruskalModel(M);
 }
 
+// mttkrp (src/mttkrp.cpp:122-156): the sequential COO MTTKRP the reference's
+// -c check compares against, on the host.
+static void SeqCooMttkrp(SparseTensor *X, FType **U, IType rank, int mode, FType *out) {
+    std::vector<FType> row(rank);
+    for (IType i = 0; i < X->nnz; ++i) {
+        for (IType r = 0; r < rank; ++r) row[r] = X->vals[i];
+        for (int m = 0; m < X->nmodes; ++m) {
+            if (m == mode) continue;
+            const FType *u = U[m] + X->cidx[m][i] * rank;
+            for (IType r = 0; r < rank; ++r) row[r] *= u[r];
+        }
+        FType *o = out + X->cidx[mode][i] * rank;
+        for (IType r = 0; r < rank; ++r) o[r] += row[r];
+    }
+}
+
 static void RunAltoCheck(SparseTensor *X, IType rank, int target_mode, int nprtn) {
     printf("===Check for mode %d===\n", target_mode);
     PrintTensorInfo(rank, 1, X);
@@ -119,11 +141,10 @@ static void RunAltoCheck(SparseTensor *X, IType rank, int target_mode, int nprtn
     AltoTensor<LIType> *AT;
     create_alto(X, &AT, nprtn);
     for (int m = m0; m < m1; ++m) {
-        // truth: COO MTTKRP (the reference's sequential mttkrp) on the GPU
+        // truth: the reference's sequential COO mttkrp, on the host
         std::vector<FType> saved(M->U[m], M->U[m] + X->dims[m] * rank);
-        mttkrp(X, M, m);
-        truth[m].assign(M->U[m], M->U[m] + X->dims[m] * rank);
-        memcpy(M->U[m], saved.data(), sizeof(FType) * saved.size());
+        truth[m] = saved;
+        SeqCooMttkrp(X, M->U, rank, m, truth[m].data());
         mttkrp_alto_par(m, fac.data(), rank, AT);
         alto[m].assign(M->U[m], M->U[m] + X->dims[m] * rank);
         memcpy(M->U[m], saved.data(), sizeof(FType) * saved.size());
@@ -151,6 +172,8 @@ int main(int argc, char **argv) {
     int prec = ALTO_PREC_F64;
     int model = -1;  // 0 als, 1 cpstream_alto, 2 spcpstream_alto
     bool nonneg = false;
+    IType nnz_threshold = ULLONG_MAX;  // main.cpp:130-131
+    IType timeslice_limit = 1;
     int c;
     while ((c = getopt_long(argc, argv, short_opt, long_opt, NULL)) != -1) {
         switch (c) {
@@ -173,7 +196,12 @@ int main(int argc, char **argv) {
         case 'r': rank = (IType)atoll(optarg); break;
         case 'm': max_iters = atoi(optarg); break;
         case 'x': srand(atoi(optarg)); break;
-        case 'd': case 's': case 'f': case 'g': case 'k': break;
+        case 'd': case 's': case 'f':
+            fprintf(stderr, "-%c drives the synthetic tensor generator, which this build does not include; "
+                            "pass a tensor with -i or -b\n", c);
+            return -1;
+        case 'g': timeslice_limit = (IType)strtoull(optarg, nullptr, 10); break;
+        case 'k': nnz_threshold = (IType)strtoull(optarg, nullptr, 10); break;
         case 't': target_mode = atoi(optarg); break;
         case 'e': epsilon = atof(optarg); break;
         case 'u': zero_based = 1; break;
@@ -182,7 +210,13 @@ int main(int argc, char **argv) {
         case 'a': streaming_mode = atoi(optarg); break;
         case 'n': nprtn = atoi(optarg); break;
         case 'P':
-            prec = !strcmp(optarg, "f32") ? ALTO_PREC_F32 : !strcmp(optarg, "bf16") ? ALTO_PREC_BF16 : ALTO_PREC_F64;
+            if (!strcmp(optarg, "f32")) prec = ALTO_PREC_F32;
+            else if (!strcmp(optarg, "bf16")) prec = ALTO_PREC_BF16;
+            else if (!strcmp(optarg, "f64")) prec = ALTO_PREC_F64;
+            else {
+                fprintf(stderr, "Invalid --prec: %s (f64 | f32 | bf16)\n", optarg);
+                return -1;
+            }
             break;
         default: Usage(argv[0]); return -1;
         }
@@ -226,9 +260,17 @@ int main(int argc, char **argv) {
             fprintf(stderr, "Need to provide -streaming_mode: %d\n", streaming_mode);
             return -1;
         }
+        if (prec == ALTO_PREC_BF16) {
+            fprintf(stderr, "CP-stream runs its MTTKRP in f64 or f32 (--prec bf16 is for -l als and -p)\n");
+            return -1;
+        }
+        if (timeslice_limit < 1) {
+            fprintf(stderr, "Invalid -timeslice-limit: %llu\n", timeslice_limit);
+            return -1;
+        }
         double w = now();
-        cpstream(X, (int)rank, max_iters, streaming_mode, epsilon, 0, nonneg, 0,
-                 prec == ALTO_PREC_F64 ? ALTO_PREC_F64 : ALTO_PREC_F32, model == 2);
+        cpstream_amd(X, (int)rank, max_iters, streaming_mode, epsilon, nonneg, model == 2, nnz_threshold,
+                     timeslice_limit, 0, prec);
         printf("%s (ALTO): %f s\n", model == 2 ? "spCPSTREAM" : "CPSTREAM", now() - w);
     } else {
         fprintf(stderr, "No -l model given (als | cpstream_alto | spcpstream_alto)\n");

@@ -90,6 +90,11 @@ int alto_hip_timer_stop(void *timer, void *stream, float *ms);
 int alto_hip_tensor_create(const alto_hip_coo *coo, int nprtn, int key_words,
                            void *stream, alto_hip_tensor *out);
 int alto_hip_tensor_destroy(alto_hip_tensor t);
+/* update_salto (include/alto.hpp:195-299): rebuilds t in place from a new
+ * COO tensor (new dims and nonzeros, same key-width rule as create with
+ * key_words 0), reusing its device buffers; the per-mode schedules are
+ * rebuilt on the next MTTKRP. */
+int alto_hip_tensor_update(alto_hip_tensor t, const alto_hip_coo *coo, int nprtn, void *stream);
 int alto_hip_tensor_info(alto_hip_tensor t, int *nmode, uint64_t *nnz,
                          int *key_words, int *nprtn, int *total_bits);
 int alto_hip_tensor_dims(alto_hip_tensor t, uint64_t *dims);
@@ -116,16 +121,19 @@ int alto_hip_tensor_device_arrays(alto_hip_tensor t, const uint64_t **keys,
  * out[target] += MTTKRP.  factors: host array of nmode DEVICE pointers,
  * row-major I_m x rank, element type by `prec` (double / float / bf16 bits).
  * out: device, dims[target] x rank, double when out_f64 != 0 (or prec is
- * F64), else float.  The output is accumulated, as the reference does, so
- * callers zero it first (cpd.hpp:110). */
+ * F64), else float -- never BF16, whatever the factor storage.  out_bytes:
+ * the capacity of out; below dims[target] * rank * 8 (FP64 output) or * 4
+ * (FP32 output) the call fails with ALTO_ERR_ARG and writes nothing.  The
+ * output is accumulated, as the reference does, so callers zero it first
+ * (cpd.hpp:110). */
 int alto_hip_mttkrp(alto_hip_tensor t, int target, int prec, int rank,
-                    const void *const *factors, void *out, int out_f64,
+                    const void *const *factors, void *out, int out_f64, size_t out_bytes,
                     void *stream);
 /* Same, selecting the kernel explicitly (for tests / the autotuner):
  * algo 0 = automatic, 1 = ALTO-order segments (atomic flush),
  * 2 = output-ordered segments (per-mode row order). */
 int alto_hip_mttkrp_algo(alto_hip_tensor t, int target, int prec, int rank,
-                         const void *const *factors, void *out, int out_f64,
+                         const void *const *factors, void *out, int out_f64, size_t out_bytes,
                          int algo, void *stream);
 /* Same, with the kernel's tuning knobs, which the defaults of the calls
  * above pick (autotuned once per shape bucket for >= 2^20 schedule entries,
@@ -135,11 +143,15 @@ int alto_hip_mttkrp_algo(alto_hip_tensor t, int target, int prec, int rank,
  * gathered product across equal consecutive keys, bit 3 = no decode-once
  * kernel, bit 4 = no fiber kernel, bit 5 = the output holds zeros on entry
  * (algo 2: rows owned by one range are stored instead of added; the caller
- * guarantees the zeros), bit 6 = fiber kernel also for BF16 factors
- * (-1 = defaults: bits 0 and 1).
+ * guarantees the zeros), bit 6 = fiber kernel also for BF16 factors,
+ * bit 7 = the hot-row kernel (algo 2; one workgroup per CU with the most
+ * popular factor rows of the gathered modes in LDS; ranks 16, 32, 64, 3 to
+ * 5 modes, 16-byte aligned output; its workgroup shape in bits 9-10:
+ * 0 = 1024 threads, 1 = 768, 2 = 512), bit 8 = the tuner leaves the hot-row
+ * kernel out (-1 = defaults: bits 0 and 1).
  * No reference counterpart (the reference fixes its launch shape). */
 int alto_hip_mttkrp_config(alto_hip_tensor t, int target, int prec, int rank,
-                           const void *const *factors, void *out, int out_f64,
+                           const void *const *factors, void *out, int out_f64, size_t out_bytes,
                            int algo, uint64_t segment, int flags, void *stream);
 /* Reference COO MTTKRP, mttkrp (src/mttkrp.cpp:122-156), on the GPU:
  * out += MTTKRP over a device COO tensor, FP64. */
@@ -171,6 +183,20 @@ int alto_hip_cpd_als(alto_hip_tensor t, const alto_hip_als_opts *opts,
                      double *const *factors, double *lambda, double *fits,
                      double *fits_exact, int *iters, double *timings,
                      void *stream);
+
+/* ---- cpd_fit_alto (include/cpd.hpp:193-275), on the GPU ----
+ * fit = sqrt(normX + ||M||^2 - 2 <X, M>) / sqrt(normX) (the unsquared value
+ * when the residual is negative, as cpd.hpp:264-268), with
+ * <X, M> = sum_r lambda_r sum_i U_last[i, r] * mttkrp_last[i, r] and ||M||^2
+ * from lambda and the nmode Gram matrices (grams: nmode consecutive R x R
+ * row-major; their upper triangles are read, as update_gram writes them).
+ * fit_mode ALTO_FIT_REFERENCE_DIAG keeps the diagonal terms of ||M||^2 (the
+ * reference's reading of never-written lower triangles as zeros),
+ * ALTO_FIT_EXACT all of them.  Host arrays: U_last and mttkrp_last rows x
+ * rank, lambda rank. */
+int alto_hip_cpd_fit(int nmode, int rank, uint64_t rows, const double *U_last,
+                     const double *mttkrp_last, const double *lambda, const double *grams,
+                     double normX, int fit_mode, double *fit, void *stream);
 
 /* ---- batched CP-ALS: cpd_alto (cpd.hpp:47-191) on many independent tensors ----
  * Runs alto_hip_cpd_als on ntensors distinct tensors concurrently: nstreams
@@ -210,21 +236,29 @@ int alto_hip_cpd_error(const alto_hip_coo *coo, int rank, const double *const *f
                        double *err, double *parts, void *stream);
 
 /* ---- CP-stream: cpstream + cpstream_alto_iter (src/streaming_cpd.cpp:26-366,
- * 1198-1519) with admm_cf_ (src/admm.cpp:216-429), CPSTREAM_ALTO model,
- * timeslice limit 1.  init_factors: host, nmode arrays of dims[m] x rank
- * rows used when the model grows (row order = first-appearance labels).
- * Outputs (host, may be NULL): final factors (dims[m] x rank each), final
- * dims, the time row s_t per slice, outer iterations per slice, ADMM
- * iterations per slice, final Grams (nmode*rank*rank), and the final error
- * of cpstream's closing cpd_error call (streaming_cpd.cpp:307-345; see
- * alto_hip_cpd_error) over the processed slices in the relabeled index
- * space, the time factor being the s_t rows.  timings (may be NULL): 4
- * doubles: slice-loop seconds, slice-build seconds, MTTKRP seconds, slices.
- * prec: ALTO_PREC_F64 or ALTO_PREC_F32 (MTTKRP arithmetic; the solve, ADMM
- * and Grams are FP64 either way).  rank <= 64.
- * Returns the number of slices processed, or a negative status. */
+ * 1198-1519) with admm_cf_ (src/admm.cpp:216-429), CPSTREAM_ALTO model.
+ * Time slices are grouped into batches as next_dynamic_batch does
+ * (src/streaming_sptensor.cpp:196-264): consecutive time indices, no
+ * further one once nnz_threshold nonzeros are taken (0 = no threshold, the
+ * reference's ULLONG_MAX default), at most timeslice_limit of them (0 or 1 =
+ * one per batch, the reference's default); a batch's stream mode has one row
+ * per time index.  init_factors: host, nmode arrays of dims[m] x rank rows
+ * used when the model grows (row order = first-appearance labels).
+ * Outputs (host, may be NULL): final factors (dims[m] x rank each; the
+ * stream mode holds the last batch's rows), final dims, the time rows (one
+ * per time index), outer iterations per batch, ADMM iterations per batch,
+ * final Grams (nmode*rank*rank), and the final error of cpstream's closing
+ * cpd_error call (streaming_cpd.cpp:307-345; see alto_hip_cpd_error) over the
+ * processed nonzeros in the relabeled index space, the time factor being the
+ * time rows.  timings (may be NULL): 4 doubles: slice-loop seconds,
+ * slice-build seconds, MTTKRP seconds, time indices processed.  max_slices >
+ * 0 caps the number of batches.  prec: ALTO_PREC_F64 or ALTO_PREC_F32
+ * (MTTKRP arithmetic; the solve, ADMM and Grams are FP64 either way).
+ * rank <= 64.  Returns the number of batches processed, or a negative
+ * status. */
 int alto_hip_cpstream(const alto_hip_coo *coo, int stream_mode, int rank,
                       int max_iters, double epsilon, int nonneg, int max_slices,
+                      uint64_t nnz_threshold, uint64_t timeslice_limit,
                       int prec, const double *const *init_factors, double *const *out_factors,
                       uint64_t *out_dims, double *out_time_rows,
                       int *out_outer_iters, double *out_admm_iters,
@@ -250,18 +284,19 @@ int alto_hip_stream_prepare(const alto_hip_coo *coo, int stream_mode, uint64_t *
 
 /* ---- spCP-stream: cpstream with use_spcpstream (model SPCPSTREAM_ALTO,
  * src/streaming_cpd.cpp:26-366, main.cpp:163, 397) driving
- * spcpstream_alto_iter (streaming_cpd.cpp:466-1085), timeslice limit 1.
- * Per slice only the rows that occur in it are updated inside the outer
- * loop (row-sparse MTTKRP over compact coordinates, update_salto_rowsparse,
- * alto.hpp:301-434); the other rows take prevU * (Q Phi^-1) at the end.
- * Constraints do not apply (the reference's spCP-stream solves
- * unconstrained).  Arguments and outputs as alto_hip_cpstream, without the
- * ADMM counts; out_grams holds the reference's grams array (non-stream
- * modes: U^T U at the start of the last slice; stream mode: G_s).
- * max_iters >= 1, rank <= 64.  Returns the number of slices processed, or a
- * negative status. */
+ * spcpstream_alto_iter (streaming_cpd.cpp:466-1085), batches as
+ * alto_hip_cpstream.  Per batch only the rows that occur in it are updated
+ * inside the outer loop (row-sparse MTTKRP over compact coordinates,
+ * update_salto_rowsparse, alto.hpp:301-434); the other rows take
+ * prevU * (Q Phi^-1) at the end.  Constraints do not apply (the reference's
+ * spCP-stream solves unconstrained).  Arguments and outputs as
+ * alto_hip_cpstream, without the ADMM counts; out_grams holds the
+ * reference's grams array (non-stream modes: U^T U at the start of the last
+ * batch; stream mode: G_s).  max_iters >= 1, rank <= 64.  Returns the number
+ * of batches processed, or a negative status. */
 int alto_hip_spcpstream(const alto_hip_coo *coo, int stream_mode, int rank,
-                        int max_iters, double epsilon, int max_slices, int prec,
+                        int max_iters, double epsilon, int max_slices,
+                        uint64_t nnz_threshold, uint64_t timeslice_limit, int prec,
                         const double *const *init_factors, double *const *out_factors,
                         uint64_t *out_dims, double *out_time_rows, int *out_outer_iters,
                         double *out_grams, double *out_final_error, double *timings,

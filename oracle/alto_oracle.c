@@ -715,28 +715,73 @@ static double frob(const double *a, size_t n)
     return sqrt(s);
 }
 
+/* next_dynamic_batch (streaming_sptensor.cpp:196-264) over the sorted and
+ * relabelled stream: from the first nonzero of time index *batch, take whole
+ * time slices while fewer than thr nonzeros are taken, stopping once lim
+ * slices are taken; the batch's stream index is the time index minus the
+ * batch's first (bc[s]), the other modes' indices are copied (bc[m]) and
+ * their dims are the running max of (index + 1).  thr 0 = no threshold,
+ * lim 0 counts as 1.  Returns the batch's nonzero count (0 at the end). */
+static uint64_t next_batch(int nmode, int s, uint64_t nnz, uint64_t *const *cidx, uint64_t thr,
+                           uint64_t lim, uint64_t *ptr, uint64_t *batch, uint64_t *prev_dim,
+                           uint64_t *const *bc, uint64_t *bd, uint64_t *start_out, uint64_t *t0)
+{
+    if (thr == 0) thr = UINT64_MAX;
+    if (lim == 0) lim = 1;
+    const uint64_t prev_batch = *batch;
+    if (*ptr >= nnz) return 0;
+    uint64_t start = *ptr;
+    while (start < nnz && cidx[s][start] < *batch) ++start;
+    uint64_t end = start;
+    while (end - start < thr && end < nnz) {
+        while (end < nnz && cidx[s][end] < *batch + 1) {
+            bc[s][end - start] = *batch - prev_batch;
+            ++end;
+        }
+        ++*batch;
+        if (*batch - prev_batch >= lim) break;
+    }
+    const uint64_t bn = end - start;
+    for (int m = 0; m < nmode; ++m) {
+        if (m == s) { bd[m] = *batch - prev_batch; continue; }
+        uint64_t mx = 0;
+        for (uint64_t i = 0; i < bn; ++i) {
+            bc[m][i] = cidx[m][start + i];
+            if (bc[m][i] > mx) mx = bc[m][i];
+        }
+        bd[m] = mx + 1 > prev_dim[m] ? mx + 1 : prev_dim[m];
+        prev_dim[m] = bd[m];
+    }
+    *ptr = end;
+    *start_out = start;
+    *t0 = prev_batch;
+    return bn;
+}
+
 /* streaming_cpd.cpp:26-366 cpstream with model CPSTREAM_ALTO,
- * timeslice_limit 1 and nnz_threshold unlimited (the reference defaults,
- * main.cpp:130-131), driving cpstream_alto_iter (streaming_cpd.cpp:1198-1519).
+ * driving cpstream_alto_iter (streaming_cpd.cpp:1198-1519).  Batches as
+ * next_batch (timeslice_limit / nnz_threshold; the reference defaults 1 and
+ * unlimited, main.cpp:130-131, are lim 1 and thr 0).
  *  - StreamingSparseTensor (streaming_sptensor.cpp:69-125): sort by
  *    (s, s+1, ...) mod N (sort.cpp:477-519; stable here), relabel every
  *    non-stream index by first appearance.
- *  - next_dynamic_batch (streaming_sptensor.cpp:196-264): one time index per
- *    slice; non-stream dims are the running max of (index + 1).
+ *  - next_dynamic_batch (streaming_sptensor.cpp:196-264): next_batch above.
  *  - Model growth takes rows [old, new) of init_factors[m] (replacing the
  *    reference's rand() fill, kruskal_model.cpp:101-133); prev_M grows with
  *    zeros.
  * Outputs: final factors (out_factors[m], out_dims[m] rows), the time-mode
- * row s_t of every slice, the outer-iteration count per slice, the ADMM
- * iteration total per slice, and the final Grams (nmode*R*R).  Returns the
- * number of slices processed. */
+ * rows (one per time index), the outer-iteration count per batch, the ADMM
+ * iteration total per batch, the final Grams (nmode*R*R) and the number of
+ * time indices processed (out_ntime).  Returns the number of batches
+ * processed (at most max_slices when > 0). */
 int orc_cpstream(int nmode, const uint64_t *dims, uint64_t nnz,
                  const uint64_t *const *cidx_in, const double *vals_in,
                  int s, int rank, int max_iters, double eps,
-                 int nonneg, int max_slices, double *const *init_factors,
+                 int nonneg, int max_slices, uint64_t nnz_threshold, uint64_t timeslice_limit,
+                 double *const *init_factors,
                  double *const *out_factors, uint64_t *out_dims,
                  double *out_time_rows, int *out_outer_iters,
-                 double *out_admm_iters, double *out_grams)
+                 double *out_admm_iters, double *out_grams, uint64_t *out_ntime)
 {
     int R = rank;
     int order[ORC_MAX_MODES];
@@ -768,8 +813,7 @@ int orc_cpstream(int nmode, const uint64_t *dims, uint64_t nnz,
 
     uint64_t cur[ORC_MAX_MODES] = {0}, prev_dim[ORC_MAX_MODES] = {0};
     uint64_t maxd[ORC_MAX_MODES];
-    for (int m = 0; m < nmode; ++m) maxd[m] = dims[m];
-    maxd[s] = 1;
+    for (int m = 0; m < nmode; ++m) maxd[m] = dims[m];  /* a batch holds at most dims[s] time rows */
     double *U[ORC_MAX_MODES], *P[ORC_MAX_MODES], *G[ORC_MAX_MODES];
     for (int m = 0; m < nmode; ++m) {
         U[m] = (double *)calloc(maxd[m] * R, sizeof(double));
@@ -797,27 +841,15 @@ int orc_cpstream(int nmode, const uint64_t *dims, uint64_t nnz,
     double *bv = (double *)malloc(sizeof(double) * (nnz ? nnz : 1));
     uint64_t *keys = (uint64_t *)malloc(sizeof(uint64_t) * 2 * (nnz ? nnz : 1));
 
+    /* init_salto's stream mode (streaming_cpd.cpp:62-65): the single-row
+     * stream MTTKRP variant only at timeslice limit 1 */
+    const int sdisp = timeslice_limit > 1 ? -1 : s;
     while (ptr < nnz && (max_slices <= 0 || slices < max_slices)) {
-        uint64_t start = ptr;
-        while (start < nnz && cidx[s][start] < batch) ++start;
-        uint64_t end = start;
-        while (end < nnz && cidx[s][end] < batch + 1) ++end;
-        ++batch;
-        uint64_t bn = end - start;
+        uint64_t bd[ORC_MAX_MODES], start = 0, t0 = 0;
+        const uint64_t bn = next_batch(nmode, s, nnz, cidx, nnz_threshold, timeslice_limit, &ptr, &batch,
+                                       prev_dim, bc, bd, &start, &t0);
         if (bn == 0) break;                    /* reference asserts nnz > 0 */
-        uint64_t bd[ORC_MAX_MODES];
-        for (int m = 0; m < nmode; ++m) {
-            if (m == s) { bd[m] = 1; for (uint64_t i = 0; i < bn; ++i) bc[m][i] = 0; continue; }
-            uint64_t mx = 0;
-            for (uint64_t i = 0; i < bn; ++i) {
-                bc[m][i] = cidx[m][start + i];
-                if (bc[m][i] > mx) mx = bc[m][i];
-            }
-            bd[m] = mx + 1 > prev_dim[m] ? mx + 1 : prev_dim[m];
-            prev_dim[m] = bd[m];
-        }
         for (uint64_t i = 0; i < bn; ++i) bv[i] = vals[start + i];
-        ptr = end;
 
         if (slices == 0) {
             for (int m = 0; m < nmode; ++m) {
@@ -825,7 +857,7 @@ int orc_cpstream(int nmode, const uint64_t *dims, uint64_t nnz,
                 memset(P[m], 0, sizeof(double) * maxd[m] * R);
             }
             for (int r = 0; r < R; ++r) lambda[r] = 1.0;
-            memset(U[s], 0, sizeof(double) * R);
+            memset(U[s], 0, sizeof(double) * bd[s] * R);
             for (int m = 0; m < nmode; ++m) orc_gram_upper(bd[m], R, U[m], G[m]);
             memset(G[s], 0, sizeof(double) * R * R);
         } else {
@@ -874,7 +906,7 @@ int orc_cpstream(int nmode, const uint64_t *dims, uint64_t nnz,
         for (int i = 0; i < max_iters; ++i) {
             delta = 0.0;
             memset(U[s], 0, sizeof(double) * R * bd[s]);
-            orc_mttkrp_alto(nmode, bd, bn, kw, keys, bv, masks, nprtn, pp, iv, s,
+            orc_mttkrp_alto(nmode, bd, bn, kw, keys, bv, masks, nprtn, pp, iv, sdisp,
                             U, R, s, U[s]);
             for (int q = 0; q < R * R; ++q) Phi[q] = 1.0;
             for (int m = 0; m < nmode; ++m)
@@ -893,7 +925,7 @@ int orc_cpstream(int nmode, const uint64_t *dims, uint64_t nnz,
                 if (j == s) continue;
                 memset(buf, 0, sizeof(double) * bd[j] * R);
                 orc_mttkrp_alto(nmode, bd, bn, kw, keys, bv, masks, nprtn, pp, iv,
-                                s, U, R, j, buf);
+                                sdisp, U, R, j, buf);
                 memcpy(ata, old_gram, sizeof(double) * R * R);
                 for (int m = 0; m < nmode; ++m) {
                     if (m == j || m == s) continue;
@@ -930,7 +962,7 @@ int orc_cpstream(int nmode, const uint64_t *dims, uint64_t nnz,
         for (int q = 0; q < R * R; ++q) G[s][q] *= 0.99;
         free(pp); free(iv);
 
-        if (out_time_rows) memcpy(out_time_rows + (size_t)slices * R, U[s], sizeof(double) * R);
+        if (out_time_rows) memcpy(out_time_rows + (size_t)t0 * R, U[s], sizeof(double) * bd[s] * R);
         if (out_outer_iters) out_outer_iters[slices] = outer;
         if (out_admm_iters) out_admm_iters[slices] = admm_total;
         for (int m = 0; m < nmode; ++m) memcpy(P[m], U[m], sizeof(double) * bd[m] * R);
@@ -941,6 +973,7 @@ int orc_cpstream(int nmode, const uint64_t *dims, uint64_t nnz,
         if (out_factors && out_factors[m]) memcpy(out_factors[m], U[m], sizeof(double) * cur[m] * R);
         if (out_grams) memcpy(out_grams + (size_t)m * R * R, G[m], sizeof(double) * R * R);
     }
+    if (out_ntime) *out_ntime = batch;
     for (int m = 0; m < nmode; ++m) {
         free(U[m]); free(P[m]); free(G[m]); free(duals[m]); free(cidx[m]); free(bc[m]);
     }
@@ -984,7 +1017,7 @@ static void hadamard_excl(int nmode, int R, double *const *mats, int mode, doubl
 }
 
 /* streaming_cpd.cpp:26-366 cpstream with model SPCPSTREAM_ALTO
- * (use_spcpstream, main.cpp:163, 397), timeslice_limit 1, driving
+ * (use_spcpstream, main.cpp:163, 397), batches as next_batch, driving
  * spcpstream_alto_iter (streaming_cpd.cpp:466-1085).  The stream handling,
  * model growth and outputs are those of orc_cpstream; constraints are not
  * used (spcpstream_alto_iter solves with pseudo_inverse only).  Per slice:
@@ -1012,13 +1045,15 @@ static void hadamard_excl(int nmode, int R, double *const *mats, int mode, doubl
  * matrix.cpp:226-250).  max_iters must be >= 1 (with 0 the reference reads
  * an uninitialised Q Phi^-1).  out_grams: the reference's grams array (the
  * non-stream modes' upper-triangle Grams from the start of the last slice,
- * and G_s).  Returns the number of slices, or -1 on bad arguments. */
+ * and G_s), the time rows one per time index, out_ntime the number of time
+ * indices.  Returns the number of batches, or -1 on bad arguments. */
 int orc_spcpstream(int nmode, const uint64_t *dims, uint64_t nnz,
                    const uint64_t *const *cidx_in, const double *vals_in,
                    int s, int rank, int max_iters, double eps, int max_slices,
+                   uint64_t nnz_threshold, uint64_t timeslice_limit,
                    double *const *init_factors, double *const *out_factors,
                    uint64_t *out_dims, double *out_time_rows, int *out_outer_iters,
-                   double *out_grams)
+                   double *out_grams, uint64_t *out_ntime)
 {
     if (max_iters < 1) return -1;
     int R = rank;
@@ -1051,8 +1086,7 @@ int orc_spcpstream(int nmode, const uint64_t *dims, uint64_t nnz,
     }
 
     uint64_t cur[ORC_MAX_MODES] = {0}, prev_dim[ORC_MAX_MODES] = {0}, maxd[ORC_MAX_MODES];
-    for (int m = 0; m < nmode; ++m) maxd[m] = dims[m];
-    maxd[s] = 1;
+    for (int m = 0; m < nmode; ++m) maxd[m] = dims[m];  /* a batch holds at most dims[s] time rows */
     double *U[ORC_MAX_MODES], *P[ORC_MAX_MODES], *G[ORC_MAX_MODES];
     double *c[ORC_MAX_MODES], *h[ORC_MAX_MODES], *cnz[ORC_MAX_MODES], *hnz[ORC_MAX_MODES];
     double *cz[ORC_MAX_MODES], *hz[ORC_MAX_MODES], *cprev[ORC_MAX_MODES];
@@ -1091,27 +1125,13 @@ int orc_spcpstream(int nmode, const uint64_t *dims, uint64_t nnz,
 
     uint64_t ptr = 0, batch = 0;
     int slices = 0;
+    const int sdisp = timeslice_limit > 1 ? -1 : s;  /* init_salto's stream mode, as orc_cpstream */
     while (ptr < nnz && (max_slices <= 0 || slices < max_slices)) {
-        uint64_t start = ptr;
-        while (start < nnz && cidx[s][start] < batch) ++start;
-        uint64_t end = start;
-        while (end < nnz && cidx[s][end] < batch + 1) ++end;
-        ++batch;
-        uint64_t bn = end - start;
+        uint64_t bd[ORC_MAX_MODES], start = 0, t0 = 0;
+        const uint64_t bn = next_batch(nmode, s, nnz, cidx, nnz_threshold, timeslice_limit, &ptr, &batch,
+                                       prev_dim, bc, bd, &start, &t0);
         if (bn == 0) break;
-        uint64_t bd[ORC_MAX_MODES];
-        for (int m = 0; m < nmode; ++m) {
-            if (m == s) { bd[m] = 1; for (uint64_t i = 0; i < bn; ++i) bc[m][i] = 0; continue; }
-            uint64_t mx = 0;
-            for (uint64_t i = 0; i < bn; ++i) {
-                bc[m][i] = cidx[m][start + i];
-                if (bc[m][i] > mx) mx = bc[m][i];
-            }
-            bd[m] = mx + 1 > prev_dim[m] ? mx + 1 : prev_dim[m];
-            prev_dim[m] = bd[m];
-        }
         for (uint64_t i = 0; i < bn; ++i) bv[i] = vals[start + i];
-        ptr = end;
 
         /* model creation / growth (streaming_cpd.cpp:106-128) */
         if (slices == 0) {
@@ -1119,7 +1139,7 @@ int orc_spcpstream(int nmode, const uint64_t *dims, uint64_t nnz,
                 memcpy(U[m], init_factors[m], sizeof(double) * bd[m] * R);
                 memset(P[m], 0, sizeof(double) * maxd[m] * R);
             }
-            memset(U[s], 0, sizeof(double) * R);
+            memset(U[s], 0, sizeof(double) * bd[s] * R);
             for (int m = 0; m < nmode; ++m) orc_gram_upper(bd[m], R, U[m], G[m]);
             memset(G[s], 0, sizeof(double) * RR);
         } else {
@@ -1191,7 +1211,7 @@ int orc_spcpstream(int nmode, const uint64_t *dims, uint64_t nnz,
             delta = 0.0;
             /* s_t (712-742) */
             memset(Anz[s], 0, sizeof(double) * nnzr[s] * R);
-            orc_mttkrp_alto(nmode, cdims, bn, kw, keys, sv, masks, nprtn, pp, iv, s, Anz, R, s,
+            orc_mttkrp_alto(nmode, cdims, bn, kw, keys, sv, masks, nprtn, pp, iv, sdisp, Anz, R, s,
                             Anz[s]);
             hadamard_excl(nmode, R, c, s, Phi);
             for (int q = 0; q < R; ++q) Phi[q * R + q] += 1e-12;
@@ -1213,7 +1233,7 @@ int orc_spcpstream(int nmode, const uint64_t *dims, uint64_t nnz,
                 hadamard_excl(nmode, R, c, m, Phi);
                 for (int q = 0; q < R; ++q) Phi[q * R + q] += 1e-12;
                 memset(Anz[m], 0, sizeof(double) * nnzr[m] * R);
-                orc_mttkrp_alto(nmode, cdims, bn, kw, keys, sv, masks, nprtn, pp, iv, s, Anz, R, m,
+                orc_mttkrp_alto(nmode, cdims, bn, kw, keys, sv, masks, nprtn, pp, iv, sdisp, Anz, R, m,
                                 Anz[m]);
                 for (uint64_t r = 0; r < nnzr[m]; ++r) {  /* rsp_mat_mul + rsp_mat_add */
                     for (int b = 0; b < R; ++b) {
@@ -1270,7 +1290,7 @@ int orc_spcpstream(int nmode, const uint64_t *dims, uint64_t nnz,
             memcpy(cnzprev[m], cnz[m], sizeof(double) * RR);
         }
         free(pp); free(iv); free(sv);
-        if (out_time_rows) memcpy(out_time_rows + (size_t)slices * R, U[s], sizeof(double) * R);
+        if (out_time_rows) memcpy(out_time_rows + (size_t)t0 * R, U[s], sizeof(double) * bd[s] * R);
         if (out_outer_iters) out_outer_iters[slices] = outer;
         for (int m = 0; m < nmode; ++m) memcpy(P[m], U[m], sizeof(double) * bd[m] * R);
         ++slices;
@@ -1280,6 +1300,7 @@ int orc_spcpstream(int nmode, const uint64_t *dims, uint64_t nnz,
         if (out_factors && out_factors[m]) memcpy(out_factors[m], U[m], sizeof(double) * cur[m] * R);
         if (out_grams) memcpy(out_grams + (size_t)m * RR, G[m], sizeof(double) * RR);
     }
+    if (out_ntime) *out_ntime = batch;
     for (int m = 0; m < nmode; ++m) {
         free(U[m]); free(P[m]); free(Anz[m]); free(Anzp[m]); free(ridx[m]); free(nzr[m]);
         free(cc[m]); free(bc[m]); free(G[m]); free(c[m]); free(h[m]); free(cnz[m]); free(hnz[m]);

@@ -77,16 +77,18 @@ double orc_admm_cf(uint64_t rows, int rank, int nmode, int mode,
 int orc_cpstream(int nmode, const uint64_t *dims, uint64_t nnz,
                  const uint64_t *const *cidx, const double *vals,
                  int stream_mode, int rank, int max_iters, double eps,
-                 int nonneg, int max_slices, double *const *init_factors,
+                 int nonneg, int max_slices, uint64_t nnz_threshold, uint64_t timeslice_limit,
+                 double *const *init_factors,
                  double *const *out_factors, uint64_t *out_dims,
                  double *out_time_rows, int *out_outer_iters,
-                 double *out_admm_iters, double *out_grams);
+                 double *out_admm_iters, double *out_grams, uint64_t *out_ntime);
 int orc_spcpstream(int nmode, const uint64_t *dims, uint64_t nnz,
                    const uint64_t *const *cidx, const double *vals,
                    int stream_mode, int rank, int max_iters, double eps, int max_slices,
+                   uint64_t nnz_threshold, uint64_t timeslice_limit,
                    double *const *init_factors, double *const *out_factors,
                    uint64_t *out_dims, double *out_time_rows, int *out_outer_iters,
-                   double *out_grams);
+                   double *out_grams, uint64_t *out_ntime);
 
 #ifdef __cplusplus
 }

@@ -290,31 +290,37 @@ def admm_cf(grams, mode, primal, dual, mttkrp, nonneg=True):
 
 
 def cpstream(dims, cidx, vals, stream_mode, rank, max_iters, eps, nonneg, init_factors,
-             max_slices=0):
+             max_slices=0, nnz_threshold=0, timeslice_limit=1):
+    """orc_cpstream: cpstream (src/streaming_cpd.cpp:26-366) with batches as
+    next_dynamic_batch (streaming_sptensor.cpp:196-264).  outer_iters and
+    admm_iters have one entry per batch, time_rows one row per time index."""
     dims = np.ascontiguousarray(dims, dtype=np.uint64)
     N = len(dims)
     cidx = [np.ascontiguousarray(c, dtype=np.uint64) for c in cidx]
     vals = np.ascontiguousarray(vals, dtype=np.float64)
     init = [np.ascontiguousarray(f, dtype=np.float64) for f in init_factors]
     nslices_max = int(dims[stream_mode])
-    outs = [np.zeros((int(dims[m]) if m != stream_mode else 1, rank)) for m in range(N)]
+    outs = [np.zeros((int(dims[m]), rank)) for m in range(N)]
     out_dims = np.zeros(N, dtype=np.uint64)
     trows = np.zeros((nslices_max, rank))
     outer = np.zeros(nslices_max, dtype=np.int32)
     admm = np.zeros(nslices_max)
     grams = np.zeros((N, rank, rank))
+    nt = ctypes.c_uint64()
     n = lib().orc_cpstream(
         ctypes.c_int(N), _p64(dims), ctypes.c_uint64(len(vals)), _ptr_array(cidx, "u"), _pf(vals),
         ctypes.c_int(stream_mode), ctypes.c_int(rank), ctypes.c_int(max_iters), ctypes.c_double(eps),
-        ctypes.c_int(1 if nonneg else 0), ctypes.c_int(max_slices), _ptr_array(init, "f"),
+        ctypes.c_int(1 if nonneg else 0), ctypes.c_int(max_slices), ctypes.c_uint64(nnz_threshold),
+        ctypes.c_uint64(timeslice_limit), _ptr_array(init, "f"),
         _ptr_array(outs, "f"), _p64(out_dims), _pf(trows),
-        outer.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _pf(admm), _pf(grams))
+        outer.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _pf(admm), _pf(grams), ctypes.byref(nt))
     outs = [o[: int(out_dims[m])] for m, o in enumerate(outs)]
-    return dict(factors=outs, dims=out_dims, time_rows=trows[:n], outer_iters=outer[:n],
-                admm_iters=admm[:n], grams=grams, slices=n)
+    return dict(factors=outs, dims=out_dims, time_rows=trows[: nt.value], outer_iters=outer[:n],
+                admm_iters=admm[:n], grams=grams, slices=n, time_slices=int(nt.value))
 
 
-def spcpstream(dims, cidx, vals, stream_mode, rank, max_iters, eps, init_factors, max_slices=0):
+def spcpstream(dims, cidx, vals, stream_mode, rank, max_iters, eps, init_factors, max_slices=0,
+               nnz_threshold=0, timeslice_limit=1):
     """orc_spcpstream: cpstream with use_spcpstream (src/streaming_cpd.cpp:26-366)
     driving spcpstream_alto_iter (streaming_cpd.cpp:466-1085).  Same outputs as
     cpstream, without the ADMM counts."""
@@ -324,21 +330,23 @@ def spcpstream(dims, cidx, vals, stream_mode, rank, max_iters, eps, init_factors
     vals = np.ascontiguousarray(vals, dtype=np.float64)
     init = [np.ascontiguousarray(f, dtype=np.float64) for f in init_factors]
     nslices_max = int(dims[stream_mode])
-    outs = [np.zeros((int(dims[m]) if m != stream_mode else 1, rank)) for m in range(N)]
+    outs = [np.zeros((int(dims[m]), rank)) for m in range(N)]
     out_dims = np.zeros(N, dtype=np.uint64)
     trows = np.zeros((nslices_max, rank))
     outer = np.zeros(nslices_max, dtype=np.int32)
     grams = np.zeros((N, rank, rank))
+    nt = ctypes.c_uint64()
     n = lib().orc_spcpstream(
         ctypes.c_int(N), _p64(dims), ctypes.c_uint64(len(vals)), _ptr_array(cidx, "u"), _pf(vals),
         ctypes.c_int(stream_mode), ctypes.c_int(rank), ctypes.c_int(max_iters), ctypes.c_double(eps),
-        ctypes.c_int(max_slices), _ptr_array(init, "f"), _ptr_array(outs, "f"), _p64(out_dims), _pf(trows),
-        outer.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _pf(grams))
+        ctypes.c_int(max_slices), ctypes.c_uint64(nnz_threshold), ctypes.c_uint64(timeslice_limit),
+        _ptr_array(init, "f"), _ptr_array(outs, "f"), _p64(out_dims), _pf(trows),
+        outer.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _pf(grams), ctypes.byref(nt))
     if n < 0:
         raise ValueError("orc_spcpstream: bad arguments")
     outs = [o[: int(out_dims[m])] for m, o in enumerate(outs)]
-    return dict(factors=outs, dims=out_dims, time_rows=trows[:n], outer_iters=outer[:n], grams=grams,
-                slices=n)
+    return dict(factors=outs, dims=out_dims, time_rows=trows[: nt.value], outer_iters=outer[:n], grams=grams,
+                slices=n, time_slices=int(nt.value))
 
 
 def cpd_error(cidx, vals, factors):

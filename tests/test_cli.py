@@ -35,6 +35,8 @@ def test_cli_without_device_fails_loudly():
 
 @pytest.mark.gpu
 def test_cli_check_all_modes(gpu):
+    """-c compares the GPU ALTO MTTKRP with the sequential COO MTTKRP on the
+    host (mttkrp.cpp:122-156), element by element at 1e-6 (main.cpp:632)."""
     rc, out, err = run(["-c", "-t", "-1", "-r", "16", "-x", "3", "-i", golden("med4.tns")])
     assert rc == 0, err
     assert out.count("Results of mttkrp_alto is correct") == 4, out
@@ -98,7 +100,60 @@ def test_cli_cpstream(gpu, tmp_path):
     rc, out, err = run(["-l", "cpstream_alto", "-a", "0", "-z", "nonneg", "-r", "4", "-m", "3", "-e", "1e-6",
                         "-i", str(p)])
     assert rc == 0, err
-    assert "Total CPSTREAM time" in out and "(4 slices" in out
+    assert "Total CPSTREAM time" in out and "(4 time slices in 4 batches" in out
+
+
+def _write_stream(p, seed, T=4, nnz=3000):
+    rng = np.random.default_rng(seed)
+    t = rng.integers(0, T, size=nnz)
+    t[:T] = np.arange(T)
+    i = rng.integers(0, 80, size=nnz)
+    j = rng.integers(0, 60, size=nnz)
+    v = rng.poisson(3, size=nnz) + 1
+    with open(p, "w") as f:
+        for a, b, c, d in zip(t, i, j, v):
+            f.write("%d %d %d %d\n" % (a + 1, b + 1, c + 1, d))
+
+
+@pytest.mark.gpu
+def test_cli_cpstream_timeslice_limit(gpu, tmp_path):
+    """-g 2 groups the 4 time slices into 2 batches (next_dynamic_batch)."""
+    p = tmp_path / "s.tns"
+    _write_stream(p, 7)
+    rc, out, err = run(["-l", "cpstream_alto", "-a", "0", "-z", "nonneg", "-r", "4", "-m", "3", "-e", "1e-6",
+                        "-g", "2", "-i", str(p)])
+    assert rc == 0, err
+    assert "timeslice_limit: 2" in out and "(4 time slices in 2 batches" in out
+
+
+@pytest.mark.parametrize("flag", [["-d", "10,10,10"], ["-s", "0.1"], ["-f", "1"]])
+def test_cli_refuses_generator_flags(flag):
+    """The synthetic generator is not built: its flags are errors, not ignored."""
+    rc, out, err = run(flag + ["-i", golden("small.tns"), "-l", "als"])
+    assert rc != 0 and "generator" in err
+
+
+def test_cli_refuses_unknown_precision():
+    rc, out, err = run(["--prec", "f16", "-i", golden("small.tns"), "-l", "als"])
+    assert rc != 0 and "Invalid --prec" in err
+
+
+@pytest.mark.gpu
+def test_cli_refuses_bf16_stream(gpu, tmp_path):
+    p = tmp_path / "s.tns"
+    _write_stream(p, 8)
+    rc, out, err = run(["-l", "cpstream_alto", "-a", "0", "--prec", "bf16", "-i", str(p)])
+    assert rc != 0 and "bf16" in err
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["f32", "bf16"])
+def test_cli_bench_reduced_precision(gpu, prec):
+    """-p with FP32 or BF16 factors writes the MTTKRP in the factors' own
+    storage (BF16 through an FP32 accumulation copy): no output overrun."""
+    rc, out, err = run(["-p", "-t", "-1", "-m", "3", "-r", "16", "--prec", prec, "-i", golden("med4.tns")])
+    assert rc == 0, err
+    assert re.search(r"ALTO runtime:\s+[0-9.]+", out)
 
 
 @pytest.mark.gpu
@@ -119,5 +174,5 @@ def test_cli_spcpstream(gpu, tmp_path):
     rc, out, err = run(["-l", "spcpstream_alto", "-a", "0", "-z", "nonneg", "-r", "4", "-m", "3", "-e", "1e-6",
                         "-i", str(p)])
     assert rc == 0, err
-    assert "spCPSTREAM" in err and "Total CPSTREAM time" in out and "(4 slices" in out
+    assert "spCPSTREAM" in err and "Total CPSTREAM time" in out and "(4 time slices in 4 batches" in out
     assert "final fit error" in out

@@ -90,3 +90,26 @@ def test_cpstream_empty_slice_is_an_error(gpu):
     cidx[0] = np.where(cidx[0] == 2, 3, cidx[0]).astype(np.uint64)  # slice 2 empty
     with pytest.raises(gpu.AltoError, match="empty time slice"):
         gpu.cpstream(dims, cidx, vals, 0, 4, 2, 1e-5, True, _init(dims, 4, 97))
+
+
+@pytest.mark.parametrize("thr,lim", [(0, 2), (2500, 10), (0, 7)])
+@pytest.mark.parametrize("nonneg", [True, False])
+def test_cpstream_dynamic_batches_match_oracle(gpu, thr, lim, nonneg):
+    """Batches of several time slices (next_dynamic_batch with -g lim and
+    -k thr): the same batches, dims and outer / ADMM iteration counts as the
+    oracle, time rows (one per time index) and factors within 1e-8."""
+    dims, cidx, vals = _stream_tensor(7, [300, 200], 7000, seed=98)
+    R = 8
+    init = _init(dims, R, 99)
+    print("thr", thr, "lim", lim, "nonneg", nonneg, flush=True)
+    ref = O.cpstream(dims, cidx, vals, 0, R, 5, 1e-6, nonneg, init, nnz_threshold=thr, timeslice_limit=lim)
+    got = gpu.cpstream(dims, cidx, vals, 0, R, 5, 1e-6, nonneg, init, nnz_threshold=thr, timeslice_limit=lim)
+    assert got["slices"] == ref["slices"] and got["time_slices"] == ref["time_slices"] == 7
+    assert ref["slices"] < 7
+    np.testing.assert_array_equal(got["dims"][1:], ref["dims"][1:])
+    np.testing.assert_array_equal(got["outer_iters"], ref["outer_iters"])
+    if nonneg:
+        np.testing.assert_array_equal(got["admm_iters"], ref["admm_iters"])
+    np.testing.assert_allclose(got["time_rows"], ref["time_rows"], rtol=1e-8, atol=1e-10)
+    for a, b in zip(got["factors"][1:], ref["factors"][1:]):
+        np.testing.assert_allclose(a, b, rtol=1e-8, atol=1e-10)

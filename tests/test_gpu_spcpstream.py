@@ -115,3 +115,25 @@ def test_spcpstream_argument_errors(gpu):
     bad[0] = np.where(bad[0] == 2, 3, bad[0]).astype(np.uint64)  # slice 2 empty
     with pytest.raises(gpu.AltoError, match="empty time slice"):
         gpu.spcpstream(dims, bad, vals, 0, 4, 2, 0.0, init)
+
+
+@pytest.mark.parametrize("thr,lim", [(0, 2), (2500, 10)])
+def test_spcpstream_dynamic_batches_match_oracle(gpu, thr, lim):
+    """spCP-stream over batches of several time slices equals the oracle
+    (same batches and outer iterations; time rows and factors within 1e-8)."""
+    rng = np.random.default_rng(126)
+    T, nnz = 7, 7000
+    t = np.sort(rng.integers(0, T, size=nnz)).astype(np.uint64)
+    t[:T] = np.arange(T)
+    cidx = [t, rng.integers(0, 300, nnz).astype(np.uint64), rng.integers(0, 200, nnz).astype(np.uint64)]
+    vals = rng.poisson(3, size=nnz) + 1.0
+    dims = [T, 300, 200]
+    init = _init(dims, 8, 127)
+    print("thr", thr, "lim", lim, flush=True)
+    ref = O.spcpstream(dims, cidx, vals, 0, 8, 4, 1e-6, init, nnz_threshold=thr, timeslice_limit=lim)
+    got = gpu.spcpstream(dims, cidx, vals, 0, 8, 4, 1e-6, init, nnz_threshold=thr, timeslice_limit=lim)
+    assert got["slices"] == ref["slices"] < T and got["time_slices"] == ref["time_slices"] == T
+    np.testing.assert_array_equal(got["outer_iters"], ref["outer_iters"])
+    np.testing.assert_allclose(got["time_rows"], ref["time_rows"], rtol=1e-8, atol=1e-10)
+    for a, b in zip(got["factors"][1:], ref["factors"][1:]):
+        np.testing.assert_allclose(a, b, rtol=1e-8, atol=1e-10)

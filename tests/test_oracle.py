@@ -372,31 +372,35 @@ def _stream_case(seed, T=5, I=14, J=11, nnz=260, R=3, distinct=True):
     return [T, I, J], cidx, vals, init, R
 
 
-def _np_cpstream_unconstrained(dims, cidx, vals, R, iters, init):
+def _np_cpstream_unconstrained(dims, cidx, vals, R, iters, init, batches=None):
     """Dense numpy restatement of CP-stream without constraints, stream mode 0
     (cpstream, streaming_cpd.cpp:26-366; cpstream_alto_iter 1198-1519): per
-    slice s_t = MTTKRP_t (Had_{m!=t} G_m + 1e-12 I)^-1, G_t += s_t^T s_t, then
-    per mode U_j = (MTTKRP_j + P_j (G_t,old Had_{m!=j,t} P_m^T U_m))
-    (Had_{m!=j} G_m + 1e-12 I)^-1; G_t *= 0.99 after the slice."""
+    batch of time slices (default: one slice each) the time rows
+    S = MTTKRP_t (Had_{m!=t} G_m + 1e-12 I)^-1, G_t += S^T S, then per mode
+    U_j = (MTTKRP_j + P_j (G_t,old Had_{m!=j,t} P_m^T U_m))
+    (Had_{m!=j} G_m + 1e-12 I)^-1; G_t *= 0.99 once per batch.
+    batches: list of (first time index, number of time indices)."""
     c2, v2 = O.stream_relabel(cidx, vals, 0)
     T = int(c2[0].max()) + 1
-    I, J = int(c2[1].max()) + 1, int(c2[2].max()) + 1
+    if batches is None:
+        batches = [(t, 1) for t in range(T)]
     U = [None] * 3
     P = [None] * 3
     G = [None] * 3
     prev = [1, 0, 0]
     trows = []
     eye = np.eye(R)
-    for t in range(T):
-        sel = c2[0] == t
+    for b, (t0, tb) in enumerate(batches):
+        sel = (c2[0] >= t0) & (c2[0] < t0 + tb)
+        bt = (c2[0][sel] - t0).astype(np.int64)
         bi, bj, bv = c2[1][sel].astype(np.int64), c2[2][sel].astype(np.int64), v2[sel]
-        bd = [1, max(int(bi.max()) + 1, prev[1]), max(int(bj.max()) + 1, prev[2])]
+        bd = [tb, max(int(bi.max()) + 1, prev[1]), max(int(bj.max()) + 1, prev[2])]
         prev = bd
-        X = np.zeros((bd[1], bd[2]))
-        np.add.at(X, (bi, bj), bv)
-        if t == 0:
-            U = [np.zeros((1, R)), init[1][: bd[1]].copy(), init[2][: bd[2]].copy()]
-            P = [np.zeros((1, R)), np.zeros((bd[1], R)), np.zeros((bd[2], R))]
+        X = np.zeros((tb, bd[1], bd[2]))
+        np.add.at(X, (bt, bi, bj), bv)
+        if b == 0:
+            U = [np.zeros((tb, R)), init[1][: bd[1]].copy(), init[2][: bd[2]].copy()]
+            P = [np.zeros((tb, R)), np.zeros((bd[1], R)), np.zeros((bd[2], R))]
             for m in (1, 2):
                 lam = np.sqrt(np.maximum((U[m] ** 2).sum(0), 1e-12))
                 U[m] /= lam
@@ -409,22 +413,76 @@ def _np_cpstream_unconstrained(dims, cidx, vals, R, iters, init):
                     P[m] = np.vstack([P[m], np.zeros((bd[m] - old, R))])
                 G[m] = U[m].T @ U[m]
         for _ in range(iters):
-            st = (U[1] * (X @ U[2])).sum(0)
-            st = np.linalg.solve(G[1] * G[2] + 1e-12 * eye, st)
-            U[0] = st[None, :]
+            S = np.einsum("tij,ir,jr->tr", X, U[1], U[2])
+            S = np.linalg.solve(G[1] * G[2] + 1e-12 * eye, S.T).T
+            U[0] = S
             Gold = G[0].copy()
-            G[0] = G[0] + np.outer(st, st)
+            G[0] = G[0] + S.T @ S
             for j in (1, 2):
                 k = 3 - j
-                mt = (X @ U[2]) * st if j == 1 else (X.T @ U[1]) * st
+                if j == 1:
+                    mt = np.einsum("tij,tr,jr->ir", X, S, U[2])
+                else:
+                    mt = np.einsum("tij,tr,ir->jr", X, S, U[1])
                 ata = Gold * (P[k].T @ U[k])
                 rhs = mt + P[j] @ ata
                 U[j] = np.linalg.solve((G[0] * G[k] + 1e-12 * eye).T, rhs.T).T
                 G[j] = U[j].T @ U[j]
         G[0] *= 0.99
         P = [u.copy() for u in U]
-        trows.append(st.copy())
+        trows.extend(S.copy())
     return U, np.array(trows)
+
+
+def _stream_counts_case(counts, seed, I=12, J=9, R=3):
+    """A stream whose time slice t has exactly counts[t] nonzeros."""
+    rng = np.random.default_rng(seed)
+    t = np.repeat(np.arange(len(counts)), counts).astype(np.uint64)
+    n = len(t)
+    i = rng.integers(0, I, n).astype(np.uint64)
+    j = rng.integers(0, J, n).astype(np.uint64)
+    vals = rng.uniform(0.5, 2.0, n)
+    init = [rng.random((len(counts), R)), rng.random((I, R)), rng.random((J, R))]
+    return [len(counts), I, J], [t, i, j], vals, init, R
+
+
+@pytest.mark.parametrize("thr,lim,expect", [
+    (0, 1, [1, 1, 1, 1, 1]),      # the reference defaults: one time index per batch
+    (10, 3, [3, 2]),              # 5+3 < 10 -> take the third (3 slices); 2+7 < 10, stream ends
+    (6, 10, [2, 1, 2]),           # 5 < 6 -> +3; 8 >= 6 alone; 2 < 6 -> +7
+    (0, 2, [2, 2, 1]),            # pairs
+    (1, 5, [1, 1, 1, 1, 1]),      # any slice reaches a threshold of 1
+])
+def test_cpstream_dynamic_batches_known_answers(thr, lim, expect):
+    """next_dynamic_batch (streaming_sptensor.cpp:196-264) on time slices of
+    5, 3, 8, 2, 7 nonzeros: hand-derived batch sizes."""
+    dims, cidx, vals, init, R = _stream_counts_case([5, 3, 8, 2, 7], 41)
+    res = O.cpstream(dims, cidx, vals, 0, R, 2, 0.0, False, init, nnz_threshold=thr, timeslice_limit=lim)
+    assert res["slices"] == len(expect)
+    assert res["time_slices"] == 5 and res["time_rows"].shape == (5, R)
+    sp = O.spcpstream(dims, cidx, vals, 0, R, 2, 0.0, init, nnz_threshold=thr, timeslice_limit=lim)
+    assert sp["slices"] == len(expect) and sp["time_slices"] == 5
+
+
+@pytest.mark.parametrize("thr,lim,batches", [(0, 2, [(0, 2), (2, 2), (4, 1)]), (6, 10, [(0, 2), (2, 1), (3, 2)])])
+def test_cpstream_batched_oracle_matches_dense_restatement(thr, lim, batches):
+    """Batches of several time indices: the oracle's unconstrained CP-stream
+    equals the dense restatement over the same batches (a stream mode with
+    one row per time index, one Gram update and one forgetting step per
+    batch)."""
+    dims, cidx, vals, init, R = _stream_counts_case([5, 3, 8, 2, 7], 42)
+    iters = 3
+    res = O.cpstream(dims, cidx, vals, 0, R, iters, 0.0, False, init, nnz_threshold=thr, timeslice_limit=lim)
+    U, trows = _np_cpstream_unconstrained(dims, cidx, vals, R, iters, init, batches=batches)
+    assert res["slices"] == len(batches)
+    np.testing.assert_allclose(res["time_rows"], trows, rtol=1e-9, atol=1e-12)
+    for m in (1, 2):
+        np.testing.assert_allclose(res["factors"][m], U[m], rtol=1e-9, atol=1e-11)
+    # spCP-stream reformulates the same update over batches as well
+    sp = O.spcpstream(dims, cidx, vals, 0, R, iters, 0.0, init, nnz_threshold=thr, timeslice_limit=lim)
+    np.testing.assert_allclose(sp["time_rows"], trows, rtol=1e-8, atol=1e-11)
+    for m in (1, 2):
+        np.testing.assert_allclose(sp["factors"][m], U[m], rtol=1e-8, atol=1e-10)
 
 
 def test_cpstream_unconstrained_oracle_matches_dense_restatement():
