@@ -495,6 +495,16 @@ def cpd_error(dims, cidx, vals, factors, stream=None):
     return err.value, tuple(parts)
 
 
+def cpd_error_noalloc(dims, cidx_dev, vals_dev, factors_dev, out_dev, stream=None):
+    """alto_hip_cpd_error_noalloc: device COO and factors in; out_dev (device,
+    4 doubles) receives err, ||X||^2, ||M||^2, <X, M> once the stream reaches
+    the call.  Allocates nothing on the device."""
+    c, keep = _coo(dims, cidx_dev, vals_dev, True)
+    R = int(factors_dev[0].shape[1])
+    check(lib().alto_hip_cpd_error_noalloc(ctypes.byref(c), ctypes.c_int(R), _ptrs(factors_dev),
+                                           _vp(out_dev.ptr), _vp(stream)), "cpd_error_noalloc")
+
+
 def gather_probe(rows, gathers=1 << 25, dist=0, zipf_s=1.1, layout=0, seed=1, stream=None):
     """ms per launch of the 128-byte row gather probe over a table of `rows`
     rows: dist 0 uniform, 1 Zipf(zipf_s); layout 0 = 8 lanes x 16 B, 1 = 4

@@ -9,6 +9,9 @@
 // COO mttkrp_par); the sums are fixed-order two-level reductions.
 #include <math.h>
 
+#include <algorithm>
+#include <mutex>
+
 #include "dense.hpp"
 #include "tensor.hpp"
 
@@ -183,5 +186,164 @@ extern "C" int alto_hip_cpd_error(const alto_hip_coo *coo, int rank, const doubl
         parts[1] = out[2];
         parts[2] = out[3];
     }
+    return ALTO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// alto_hip_cpd_error_noalloc: the same quantities with no device allocation,
+// no host synchronisation and no host read, for callers that cannot allocate
+// (graph capture, tight serving loops).  <X, M> is summed per nonzero,
+// val * sum_r prod_m U_m[i_m, r] (the same sum as the MTTKRP dot product,
+// in another order), so no MTTKRP output is needed; the scratch holds the
+// reduction partials, the Grams and a coordinate-check flag.
+
+namespace alto {
+
+constexpr int kNaMaxRank = 64;
+constexpr int kNaGramBlocks = 256;
+constexpr size_t kNaFlag = 0;                                 // int flag in the first slot
+constexpr size_t kNaDot = 32;                                 // kDotBlocks partials
+constexpr size_t kNaSq = kNaDot + kDotBlocks;                 // kSumsqBlocks partials
+constexpr size_t kNaXX = kNaSq + kSumsqBlocks;                // ||X||^2
+constexpr size_t kNaG = kNaXX + 32;                           // kMaxModes R x R Grams
+constexpr size_t kNaGpart = kNaG + (size_t)kMaxModes * kNaMaxRank * kNaMaxRank;
+constexpr size_t kNaDoubles = kNaGpart + (size_t)kNaGramBlocks * kNaMaxRank * kNaMaxRank;
+
+// The scratch array (about 8.4 MB per GPU, held until the process ends).  In
+// host code its name is only a handle: kernels get the device address that
+// hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) double scratch[kNaDoubles];
+
+constexpr int kNaMaxDevices = 64;
+static std::mutex g_na_mu;
+static double *g_na_ptr[kNaMaxDevices] = {};
+
+// Device address of `scratch` on the current device, resolved once per device.
+static int na_scratch(double **out) {
+    int dev = 0;
+    ALTO_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNaMaxDevices) return ALTO_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> g(g_na_mu);
+    if (!g_na_ptr[dev]) {
+        void *p = nullptr;
+        ALTO_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_na_ptr[dev] = (double *)p;
+    }
+    *out = g_na_ptr[dev];
+    return ALTO_OK;
+}
+
+__global__ void k_na_clear_flag(int *flag) {
+    if (threadIdx.x == 0) *flag = 0;
+}
+
+struct FacArgs {
+    const double *p[kMaxModes];
+};
+
+// part[b] = sum over block b's nonzeros of val * sum_r prod_m U_m[i_m, r];
+// a coordinate not below its mode's rows sets the flag and is skipped.
+__global__ __launch_bounds__(256) void k_na_inner(IdxArgs ci, const double *__restrict__ vals, uint64_t nnz,
+                                                  FacArgs F, Rows rows, int N, int R, double *__restrict__ part,
+                                                  int *__restrict__ flag) {
+    __shared__ double sh[4];
+    double acc = 0.0;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < nnz; i += (uint64_t)gridDim.x * 256) {
+        bool ok = true;
+        for (int m = 0; m < N; ++m) ok = ok && ci.p[m][i] < rows.r[m];
+        if (!ok) {
+            *flag = 1;
+            continue;
+        }
+        double srow = 0.0;
+        for (int r = 0; r < R; ++r) {
+            double v = 1.0;
+            for (int m = 0; m < N; ++m) v *= F.p[m][ci.p[m][i] * R + r];
+            srow += v;
+        }
+        acc += vals[i] * srow;
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// As k_error_final, into the caller's device output; NaN when the flag is set.
+__global__ void k_na_final(const double *__restrict__ dot_part, const double *__restrict__ xx,
+                           const double *__restrict__ grams, int nmode, int R, const int *__restrict__ flag,
+                           double *__restrict__ out) {
+    const int lane = threadIdx.x;
+    double inner = wave_reduce_entry(dot_part, kDotBlocks, 1, 0, 0);
+    double z = 0.0;
+    const int RR = R * R;
+    for (int e = lane; e < RR; e += 64) {
+        const int i = e / R, j = e % R;
+        if (j < i) continue;
+        double v = 1.0;
+        for (int m = 0; m < nmode; ++m) v *= grams[(size_t)m * RR + e];
+        z += (i == j) ? v : 2.0 * v;
+    }
+    for (int o = 32; o > 0; o >>= 1) z += __shfl_down(z, o, 64);
+    if (lane == 0) {
+        if (*flag) {
+            // NaN made at run time (0 / 0): as a constant, this compiler
+            // materialises it with a 64-bit scalar move whose encoding keeps
+            // only the low 32 bits of the literal, which stores 0
+            const double q = (double)*flag - 1.0;
+            const double nan_v = q / q;
+            for (int k = 0; k < 4; ++k) out[k] = nan_v;
+            return;
+        }
+        const double X = *xx, Z = fabs(z);
+        out[0] = sqrt(X + Z - 2.0 * inner) / sqrt(X);
+        out[1] = X;
+        out[2] = Z;
+        out[3] = inner;
+    }
+}
+
+}  // namespace alto
+
+extern "C" int alto_hip_cpd_error_noalloc(const alto_hip_coo *coo, int rank, const double *const *factors,
+                                          double *out, void *stream) {
+    if (!coo || !coo->dims || !coo->cidx || !coo->vals || !factors || !out) return ALTO_ERR_ARG;
+    if (!coo->on_device) {
+        set_error("cpd_error_noalloc takes device arrays (coo->on_device = 1)");
+        return ALTO_ERR_ARG;
+    }
+    const int N = coo->nmode;
+    if (N < 1 || N > kMaxModes || rank < 1) return ALTO_ERR_ARG;
+    if (rank > kNaMaxRank) {
+        set_error("cpd_error_noalloc: rank %d exceeds its scratch limit of %d", rank, kNaMaxRank);
+        return ALTO_ERR_UNSUPPORTED;
+    }
+    for (int m = 0; m < N; ++m)
+        if (!factors[m] || coo->dims[m] == 0 || (coo->nnz && !coo->cidx[m])) return ALTO_ERR_ARG;
+    ALTO_TRY(ensure_init());
+    hipStream_t s = to_stream(stream);
+    double *sc = nullptr;
+    ALTO_TRY(na_scratch(&sc));
+    int *flag = (int *)(sc + kNaFlag);
+    IdxArgs ci{};
+    Rows rr{};
+    FacArgs F{};
+    for (int m = 0; m < N; ++m) {
+        ci.p[m] = coo->cidx[m];
+        rr.r[m] = coo->dims[m];
+        F.p[m] = factors[m];
+    }
+    k_na_clear_flag<<<1, 64, 0, s>>>(flag);
+    k_na_inner<<<kDotBlocks, 256, 0, s>>>(ci, coo->vals, coo->nnz, F, rr, N, rank, sc + kNaDot, flag);
+    ALTO_LAUNCH_CHECK();
+    ALTO_TRY(launch_sumsq(coo->vals, coo->nnz, sc + kNaXX, sc + kNaSq, s));
+    const size_t RR = (size_t)rank * rank;
+    for (int m = 0; m < N; ++m) {
+        const int nb = std::min(row_blocks(coo->dims[m]), kNaGramBlocks);
+        ALTO_TRY(launch_gram(factors[m], ALTO_PREC_F64, coo->dims[m], rank, sc + kNaGpart, nb, s));
+        ALTO_TRY(launch_gram_reduce(sc + kNaGpart, nb, rank, sc + kNaG + (size_t)m * RR, s));
+    }
+    k_na_final<<<1, 64, 0, s>>>(sc + kNaDot, sc + kNaXX, sc + kNaG, N, rank, flag, out);
+    ALTO_LAUNCH_CHECK();
     return ALTO_OK;
 }

@@ -235,6 +235,21 @@ int alto_hip_pseudo_inverse(int nmode, int rank, const double *grams, int mode, 
 int alto_hip_cpd_error(const alto_hip_coo *coo, int rank, const double *const *factors,
                        double *err, double *parts, void *stream);
 
+/* ---- alto_hip_cpd_error without allocation ----
+ * No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops).  The quantities of
+ * alto_hip_cpd_error, enqueued on `stream` with no device allocation, no
+ * host synchronisation and no host read: out (DEVICE, 4 doubles) receives
+ * err, ||X||^2, ||M||^2 and <X, M> when the stream reaches the call.  Device
+ * inputs only (coo->on_device = 1, factors[m] device FP64, dims[m] x rank).
+ * rank <= 64, nmode <= 8 (its scratch, a static device array of about
+ * 8.4 MB per GPU, is sized for that; larger ranks are ALTO_ERR_UNSUPPORTED).
+ * A coordinate not below its mode's size sets all four outputs to NaN
+ * (alto_hip_cpd_error returns ALTO_ERR_ARG).  Calls on one GPU share that
+ * scratch: calls that overlap on one GPU (other streams or threads) race. */
+int alto_hip_cpd_error_noalloc(const alto_hip_coo *coo, int rank, const double *const *factors,
+                               double *out, void *stream);
+
 /* ---- CP-stream: cpstream + cpstream_alto_iter (src/streaming_cpd.cpp:26-366,
  * 1198-1519) with admm_cf_ (src/admm.cpp:216-429), CPSTREAM_ALTO model.
  * Time slices are grouped into batches as next_dynamic_batch does
