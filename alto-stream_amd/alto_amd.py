@@ -306,6 +306,22 @@ def cpd_alto(at, factors, rank, max_iters, eps=1e-30, prec=PREC_F64, fit_mode=FI
     return fs, lam, fits[:n], fits_exact[:n], tim
 
 
+def cpd_fit(last_factor, last_mttkrp, lam, grams, normX, fit_mode=FIT_REFERENCE_DIAG, stream=None):
+    """cpd_fit_alto (cpd.hpp:193-275) on the GPU from host arrays: the last
+    mode's factor and MTTKRP (rows x R), lambda, the nmode Gram matrices
+    (their upper triangles are read) and ||X||^2."""
+    U = np.ascontiguousarray(last_factor, dtype=np.float64)
+    MT = np.ascontiguousarray(last_mttkrp, dtype=np.float64)
+    lam = np.ascontiguousarray(lam, dtype=np.float64)
+    g = np.ascontiguousarray(np.stack([np.asarray(x, dtype=np.float64) for x in grams]))
+    fit = ctypes.c_double()
+    check(lib().alto_hip_cpd_fit(ctypes.c_int(g.shape[0]), ctypes.c_int(U.shape[1]), ctypes.c_uint64(U.shape[0]),
+                                 U.ctypes.data_as(_f64p), MT.ctypes.data_as(_f64p), lam.ctypes.data_as(_f64p),
+                                 g.ctypes.data_as(_f64p), ctypes.c_double(normX), ctypes.c_int(fit_mode),
+                                 ctypes.byref(fit), _vp(stream)), "cpd_fit")
+    return fit.value
+
+
 def pseudo_inverse(grams, mode, U, stream=None):
     """pseudo_inverse (cpd.hpp:377-480) on the GPU: returns (U V^-1, fallback)."""
     g = np.ascontiguousarray(np.stack([np.asarray(x, dtype=np.float64) for x in grams]))

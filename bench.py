@@ -10,9 +10,11 @@ BenchmarkAlto does (src/main.cpp:498-512).
 
 Prints one JSON line.  `value` = algorithmic HBM bytes of the step
 (sum over modes of nnz*(k+v) + sum_{m!=n} I_m*R*f + I_n*R*o) / step time.
-Also reported: CP-ALS s/iter on the same tensor for the FP32 and the plain
-FP64 GPU paths with their fits, and the oracle (the CPU restatement of the
-reference's OpenMP algorithm) timed on the host cores on a bounded sample.
+Also reported: CP-ALS s/iter on the same tensor in FP32, BF16 storage and
+FP64 with the exact fit of each final model (FP64, duplicates merged) and
+their lambda/factor errors against FP64; the other workloads; and the
+oracle (the CPU restatement of the reference's OpenMP algorithm) timed over
+the whole tensor on the process's CPU share, pinned one thread per core.
 """
 import argparse
 import json
@@ -20,12 +22,23 @@ import os
 import sys
 import time
 
+# The CPU legs (the oracle, the only OpenMP code here) run pinned, one thread
+# per core, threads spinning between regions; libgomp reads these when it
+# loads, and the CPU legs run after every GPU leg so that the binding of the
+# main thread touches no GPU work.
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+os.environ.setdefault("OMP_WAIT_POLICY", "active")
+
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "alto-stream_amd"))
 
 HBM_PEAK_GBS = 8000.0
+# |fit_exact(prec) - fit_exact(FP64)| within which a reduced-precision CP-ALS
+# counts as reaching the FP64 fit
+EQUAL_FIT_TOL = 1e-5
 
 
 def log(*a):
@@ -65,10 +78,11 @@ def main():
     ap.add_argument("--als-iters", type=int, default=10)
     ap.add_argument("--no-als", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=25_000_000)
+    ap.add_argument("--cpu-sample", type=int, default=0,
+                    help="CPU baseline over the first N nonzeros only (0: the whole tensor)")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the other workloads (plain CP-ALS, 5-mode BF16, CP-stream sample)")
-    ap.add_argument("--stream-slices", type=int, default=24, help="CP-stream sample: number of time slices")
+    ap.add_argument("--stream-slices", type=int, default=1000, help="CP-stream: number of time slices (of 1000)")
     ap.add_argument("--batch-tensors", type=int, default=512, help="batched CP-ALS: number of tensors")
     ap.add_argument("--probe", action="store_true",
                     help="random 128-byte row gather rate vs table size (the MTTKRP's miss-service bound)")
@@ -80,12 +94,13 @@ def main():
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    cpu_threads()
 
     import alto_amd as A
     A.init(0)
     dev = A.device_name()
     if args.leg:
-        legs = {"plain_als": lambda: leg_plain_als(A, not args.no_cpu), "bf16_5mode": lambda: leg_bf16_5mode(A),
+        legs = {"plain_als": lambda: leg_plain_als(A), "bf16_5mode": lambda: leg_bf16_5mode(A),
                 "cp_stream": lambda: leg_cp_stream(A, args.stream_slices),
                 "batched_als": lambda: leg_batched_als(A, args.batch_tensors)}
         print(json.dumps(legs[args.leg]()), flush=True)
@@ -205,32 +220,114 @@ def main():
     for d in outs:
         d.close()
 
+    # host copy of the ALTO-ordered nonzeros: the merged-duplicate norm and the
+    # CPU legs read it (the tensor itself is released before the other legs)
+    t0 = time.time()
+    hkeys, hvals = host_nonzeros(A, at, at.nnz if args.cpu_sample <= 0 else min(args.cpu_sample, at.nnz))
+    normX2_merged = merged_norm2(hkeys, hvals) if hkeys.shape[0] == at.nnz else None
+    log("host copy of %d nonzeros and merged norm in %.1f s" % (hkeys.shape[0], time.time() - t0))
+
     if not args.no_als:
         als = {}
+        models = {}
         for name, p in (("f32", A.PREC_F32), ("bf16", A.PREC_BF16), ("f64", A.PREC_F64)):
             A.cpd_alto(at, F64, R, 2, eps=0.0, prec=p)  # warm-up (one-time runtime set-up)
             t0 = time.time()
-            _, lam, fits, fe, tim = A.cpd_alto(at, F64, R, args.als_iters, eps=1e-30, prec=p)
-            als[name] = dict(s_per_iter=round(float(tim[7]), 5),
-                             fit=float(fits[-1]), fit_exact=float(fe[-1]), iters=len(fits))
-            log("CP-ALS %s: %.4f s/iter, fit %.8f (wall %.1f s)" % (name, tim[7], fits[-1], time.time() - t0))
-        als["speedup_f32_vs_f64"] = round(als["f64"]["s_per_iter"] / als["f32"]["s_per_iter"], 3)
-        als["dfit"] = abs(als["f32"]["fit"] - als["f64"]["fit"])
-        als["dfit_rel"] = als["dfit"] / max(abs(als["f64"]["fit"]), 1e-300)
-        als["speedup_bf16_vs_f64"] = round(als["f64"]["s_per_iter"] / als["bf16"]["s_per_iter"], 3)
-        als["dfit_bf16"] = abs(als["bf16"]["fit"] - als["f64"]["fit"])
-        als["dfit_rel_bf16"] = als["dfit_bf16"] / max(abs(als["f64"]["fit"]), 1e-300)
+            fs, lam, fits, fe, tim = A.cpd_alto(at, F64, R, args.als_iters, eps=1e-30, prec=p)
+            models[name] = (fs, lam)
+            als[name] = dict(s_per_iter=round(float(tim[7]), 5), fit_reference=float(fits[-1]),
+                             fit_exact_unmerged_norm=float(fe[-1]), iters=len(fits))
+            if normX2_merged is not None:
+                als[name]["fit_exact"] = exact_fit(A, at, fs, lam, normX2_merged)
+            log("CP-ALS %s: %.4f s/iter, reference fit %.8f, exact fit %s (wall %.1f s)" %
+                (name, tim[7], fits[-1], als[name].get("fit_exact"), time.time() - t0))
+        als["fit_note"] = ("fit_exact: 1 - ||X - M|| / ||X|| in FP64 over the tensor with duplicate coordinates "
+                           "merged (%.1f%% of the stored nonzeros repeat a coordinate); fit_reference: the "
+                           "reference's printed fit, whose ||X||^2 is the sum of squared stored values "
+                           "(cpd.hpp:80-83) and so is not a fit on this tensor" %
+                           (100.0 * (1.0 - distinct_count(hkeys) / hkeys.shape[0]) if normX2_merged is not None
+                            else float("nan")))
+        for name in ("f32", "bf16"):
+            fs, lam = models[name]
+            fs64, lam64 = models["f64"]
+            als["speedup_%s_vs_f64" % name] = round(als["f64"]["s_per_iter"] / als[name]["s_per_iter"], 3)
+            if normX2_merged is not None:
+                als["dfit_exact_%s" % name] = abs(als[name]["fit_exact"] - als["f64"]["fit_exact"])
+                als["%s_at_equal_fit" % name] = als["dfit_exact_%s" % name] <= EQUAL_FIT_TOL
+            als["lambda_rel_err_%s" % name] = float(np.linalg.norm(lam - lam64) / np.linalg.norm(lam64))
+            als["factor_rel_err_%s" % name] = float(max(np.linalg.norm(a - b) / np.linalg.norm(b)
+                                                        for a, b in zip(fs, fs64)))
+            als["dfit_reference_%s" % name] = abs(als[name]["fit_reference"] - als["f64"]["fit_reference"])
+        als["equal_fit_tolerance"] = EQUAL_FIT_TOL
         result["cp_als"] = als
-
-    if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(A, at, dims, R, F64, args.cpu_sample)
+    kw = at.kw
     at.close()
     if not args.no_extra:
-        result["other_workloads"] = {"plain_cp_als_1000x800x600": leg_plain_als(A, not args.no_cpu),
+        result["other_workloads"] = {"plain_cp_als_1000x800x600": leg_plain_als(A),
                                      "bf16_5mode_5e7": leg_bf16_5mode(A),
                                      "cp_stream_200000x100000": leg_cp_stream(A, args.stream_slices),
                                      "batched_cp_als_512": leg_batched_als(A, args.batch_tensors)}
+    # CPU legs last: the pinned OpenMP threads must not share cores with GPU work
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(hkeys, hvals, dims, R, F64, kw, nnz)
+        if not args.no_extra:
+            result["other_workloads"]["plain_cp_als_1000x800x600"].update(plain_als_cpu())
     print(json.dumps(result), flush=True)
+
+
+def host_nonzeros(A, at, count):
+    """The first `count` nonzeros of the ALTO order (keys, FP64 values) on the host."""
+    import ctypes
+    keys_d, vals_d = ctypes.c_void_p(), ctypes.c_void_p()
+    A.check(A.lib().alto_hip_tensor_device_arrays(ctypes.c_void_p(at.handle), ctypes.byref(keys_d),
+                                                   ctypes.byref(vals_d)), "device_arrays")
+    keys = np.empty((count, at.kw), dtype=np.uint64)
+    vals = np.empty(count, dtype=np.float64)
+    A.check(A.lib().alto_hip_memcpy_d2h(keys.ctypes.data_as(ctypes.c_void_p), keys_d,
+                                        ctypes.c_size_t(keys.nbytes), None), "d2h")
+    A.check(A.lib().alto_hip_memcpy_d2h(vals.ctypes.data_as(ctypes.c_void_p), vals_d,
+                                        ctypes.c_size_t(vals.nbytes), None), "d2h")
+    return keys, vals
+
+
+def _run_starts(keys):
+    """Start of each run of equal keys (the ALTO order is sorted by key, so a
+    coordinate's duplicates are adjacent)."""
+    diff = np.any(keys[1:] != keys[:-1], axis=1)
+    return np.concatenate(([0], np.flatnonzero(diff) + 1))
+
+
+def distinct_count(keys):
+    return int(_run_starts(keys).shape[0])
+
+
+def merged_norm2(keys, vals):
+    """||X||^2 of the tensor whose duplicate coordinates are summed."""
+    sums = np.add.reduceat(vals, _run_starts(keys))
+    return float(np.dot(sums, sums))
+
+
+def exact_fit(A, at, fs, lam, normX2):
+    """1 - ||X - M|| / ||X|| in FP64 for the model M = sum_r lam_r u_1r o ... o u_Nr:
+    ||M||^2 from the Gram matrices, <X, M> from an FP64 MTTKRP of the last
+    mode with the model's factors (the MTTKRP sums every stored nonzero, so
+    duplicates count as their merged sum)."""
+    N = len(fs)
+    R = fs[0].shape[1]
+    G = np.ones((R, R))
+    for f in fs:
+        G *= f.T @ f
+    nM2 = float(lam @ G @ lam)
+    dF = [A.to_prec(f, A.PREC_F64) for f in fs]
+    o = A.DeviceArray.zeros((fs[-1].shape[0], R), np.float64)
+    A.mttkrp_alto_par(N - 1, dF, R, at, o, prec=A.PREC_F64)
+    K = o.numpy()
+    for d in dF:
+        d.close()
+    o.close()
+    ip = float(np.sum((K * fs[-1]) @ lam))
+    res2 = max(normX2 - 2.0 * ip + nM2, 0.0)
+    return 1.0 - np.sqrt(res2) / np.sqrt(normX2)
 
 
 def measured_traffic(workload, prec):
@@ -250,23 +347,68 @@ def measured_traffic(workload, prec):
     return None
 
 
-def cpu_baseline(A, at, dims, R, F64, sample):
+def cpu_info():
+    """CPU model, the host's physical cores and logical CPUs, and this
+    process's CPU share: its affinity set, bounded by a cgroup quota."""
+    model, cores = None, set()
+    phys = core = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name" and model is None:
+                model = v
+            elif k == "physical id":
+                phys = v
+            elif k == "core id":
+                core = v
+            elif not k:
+                if core is not None:
+                    cores.add((phys, core))
+                phys = core = None
+        if core is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    logical = os.cpu_count() or 1
+    affinity = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()
+        if q != "max":
+            quota = int(q) / int(per)
+    except (OSError, ValueError):
+        pass
+    share = affinity if quota is None else max(1, min(affinity, int(quota)))
+    return dict(cpu_model=model, host_physical_cores=len(cores) or logical, host_logical_cpus=logical,
+                process_affinity_cpus=affinity, cgroup_cpu_quota=quota, process_cpu_share=share)
+
+
+# at most 16 threads: the CPU share allotted to one GPU's job on the measuring host
+CPU_THREADS_CAP = 16
+
+
+_CPU = None
+
+
+def cpu_threads():
+    """Threads for the CPU legs and the CPU description, read once: after the
+    first pinned OpenMP region the main thread's affinity is one core."""
+    global _CPU
+    if _CPU is None:
+        info = cpu_info()
+        _CPU = (min(info["process_cpu_share"], CPU_THREADS_CAP), info)
+    return _CPU
+
+
+def cpu_baseline(keys, vals, dims, R, F64, kw, nnz_total):
     """The oracle's OpenMP restatement of the reference MTTKRP (da_mem_pull /
-    atomic dispatch, FP64) on the first `sample` nonzeros of the ALTO order."""
+    atomic dispatch, FP64) over the ALTO-ordered nonzeros copied from the GPU
+    tensor: one untimed all-mode pass, then one timed."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    import ctypes
-    threads = min(16, os.cpu_count() or 1)
-    S = min(sample, at.nnz)
-    keys_d, vals_d = ctypes.c_void_p(), ctypes.c_void_p()
-    A.check(A.lib().alto_hip_tensor_device_arrays(ctypes.c_void_p(at.handle), ctypes.byref(keys_d),
-                                                   ctypes.byref(vals_d)), "device_arrays")
-    keys = np.empty((S, at.kw), dtype=np.uint64)
-    vals = np.empty(S, dtype=np.float64)
-    A.check(A.lib().alto_hip_memcpy_d2h(keys.ctypes.data_as(ctypes.c_void_p), keys_d,
-                                        ctypes.c_size_t(keys.nbytes), None), "d2h")
-    A.check(A.lib().alto_hip_memcpy_d2h(vals.ctypes.data_as(ctypes.c_void_p), vals_d,
-                                        ctypes.c_size_t(vals.nbytes), None), "d2h")
+    threads, info = cpu_threads()
+    S = keys.shape[0]
     O.set_threads(threads)
 
     class T:  # oracle-side ALTO tensor over the host copy
@@ -274,29 +416,36 @@ def cpu_baseline(A, at, dims, R, F64, sample):
     t = T()
     t.dims = np.array(dims, dtype=np.uint64)
     t.nmode = len(dims)
-    t.masks, _ = O.alto_masks(dims, at.kw)
-    t.kw, t.keys, t.vals, t.nnz, t.nprtn = at.kw, keys, vals, S, threads
+    t.masks, _ = O.alto_masks(dims, kw)
+    t.kw, t.keys, t.vals, t.nnz, t.nprtn = kw, keys, vals, S, threads
+    t0 = time.time()
     t.prtn_ptr, t.intervals = O.prtn_alto(t.dims, keys, t.masks, threads)
+    t_prtn = time.time() - t0
     outs = [np.zeros((d, R)) for d in dims]
+    for n in range(len(dims)):
+        O.AltoTensor.mttkrp(t, F64, n, out=outs[n])
     t0 = time.time()
     for n in range(len(dims)):
         O.AltoTensor.mttkrp(t, F64, n, out=outs[n])
     dt = time.time() - t0
-    tot, _ = alg_bytes(dims, S, R, 8 * at.kw, 8, 8, 8)
-    log("cpu baseline: %.2f s for one all-mode FP64 MTTKRP over %d nnz on %d threads" % (dt, S, threads))
-    return {"value": round(tot / dt / 1e9, 3), "unit": "GB/s", "cores": threads, "kind": "port",
-            "sample": "one all-mode FP64 MTTKRP (oracle OpenMP restatement of the reference dispatch) over "
-                      "the first %d nonzeros in ALTO order: %.3f s; extrapolated %.1f ms per all-mode step "
-                      "at 1e8 nnz" % (S, dt, dt * 1e3 * 1e8 / S)}
+    tot, _ = alg_bytes(dims, S, R, 8 * kw, 8, 8, 8)
+    log("cpu baseline: %.2f s for one all-mode FP64 MTTKRP over %d nnz on %d threads (%s)" %
+        (dt, S, threads, info["cpu_model"]))
+    full = S == nnz_total
+    out = {"value": round(tot / dt / 1e9, 3), "unit": "GB/s", "cores": threads, "kind": "port",
+           "sample": "one all-mode FP64 MTTKRP (oracle OpenMP restatement of the reference dispatch) over %s "
+                     "(%d nonzeros in ALTO order), after one untimed pass: %.3f s per all-mode step" %
+                     ("the whole tensor" if full else "a prefix of the tensor", S, dt),
+           "ms_per_step": round(dt * 1e3, 1), "partition_s": round(t_prtn, 3),
+           "omp": {k: os.environ.get(k) for k in ("OMP_PROC_BIND", "OMP_PLACES", "OMP_WAIT_POLICY")}}
+    out.update(info)
+    return out
 
 
-
-def leg_plain_als(A, with_cpu):
-    """Plain CP-ALS as the reference runs it: 3-mode 1000x800x600, 1e6
-    distinct uniform coordinates, U(0,1) values, rank 16, 20 iterations;
-    plus 100 mode-0 MTTKRPs (the reference's -p benchmark)."""
+def plain_als_data():
+    """3-mode 1000x800x600, 1e6 distinct uniform coordinates, U(0,1) values,
+    rank 16: coordinates, values and the initial model."""
     dims, nnz, R = [1000, 800, 600], 1_000_000, 16
-    log("plain CP-ALS, 1000x800x600, 1e6 nnz, rank 16")
     rng = np.random.default_rng(0xA1700000)
     total = dims[0] * dims[1] * dims[2]
     lin = np.unique(rng.integers(0, total, size=int(nnz * 1.05)))
@@ -305,6 +454,15 @@ def leg_plain_als(A, with_cpu):
             (lin % dims[2]).astype(np.uint64)]
     vals = rng.random(nnz)
     F = [rng.random((d, R)) for d in dims]
+    return dims, cidx, vals, F, R
+
+
+def leg_plain_als(A):
+    """Plain CP-ALS as the reference runs it: 3-mode 1000x800x600, 1e6
+    distinct uniform coordinates, U(0,1) values, rank 16, 20 iterations;
+    plus 100 mode-0 MTTKRPs (the reference's -p benchmark)."""
+    log("plain CP-ALS, 1000x800x600, 1e6 nnz, rank 16")
+    dims, cidx, vals, F, R = plain_als_data()
     at = A.create_alto(dims, cidx, vals, nprtn=256)
     out = {"workload": "3-mode 1000x800x600, 1e6 distinct nnz, U(0,1), rank 16"}
     for name, p in (("f64", A.PREC_F64), ("f32", A.PREC_F32)):
@@ -326,21 +484,24 @@ def leg_plain_als(A, with_cpu):
         d.close()
     o.close()
     at.close()
-    if with_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O
-        threads = min(16, os.cpu_count() or 1)
-        O.set_threads(threads)
-        ref = O.AltoTensor(dims, cidx, vals, nprtn=threads)
-        t0 = time.time()
-        _, _, rfits, _ = ref.cpd_als(F, R, 5, eps=1e-30)
-        dt = (time.time() - t0) / 5
-        out["cpu_port_cp_als_s_per_iter"] = round(dt, 5)
-        out["cpu_cores"] = threads
-        out["cpu_fit_after_5"] = float(rfits[-1])
     log("plain CP-ALS: f64 %.5f s/iter, f32 %.5f s/iter, MTTKRP mode 0 %.4f ms" %
         (out["cp_als_f64_s_per_iter"], out["cp_als_f32_s_per_iter"], out["mttkrp_mode0_f64_ms"]))
     return out
+
+
+def plain_als_cpu():
+    """The oracle's CP-ALS on the plain workload, 5 iterations, on the CPU share."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    dims, cidx, vals, F, R = plain_als_data()
+    threads, info = cpu_threads()
+    O.set_threads(threads)
+    ref = O.AltoTensor(dims, cidx, vals, nprtn=threads)
+    t0 = time.time()
+    _, _, rfits, _ = ref.cpd_als(F, R, 5, eps=1e-30)
+    dt = (time.time() - t0) / 5
+    log("plain CP-ALS on the CPU: %.4f s/iter on %d threads" % (dt, threads))
+    return {"cpu_port_cp_als_s_per_iter": round(dt, 5), "cpu_cores": threads, "cpu_fit_after_5": float(rfits[-1])}
 
 
 def leg_bf16_5mode(A):

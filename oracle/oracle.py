@@ -252,6 +252,20 @@ class AltoTensor:
         return factors, lam, fits[:n], fits_exact[:n]
 
 
+def cpd_fit(dims, factors, lam, grams, last_mttkrp, normX, fit_mode=FIT_REFERENCE_DIAG):
+    """orc_cpd_fit: cpd_fit_alto (cpd.hpp:193-275) from the factors, lambda,
+    the Gram matrices as given (the reference's are upper-triangle only) and
+    the last mode's MTTKRP."""
+    dims = np.ascontiguousarray(dims, dtype=np.uint64)
+    factors = [np.ascontiguousarray(f, dtype=np.float64) for f in factors]
+    grams = [np.ascontiguousarray(g, dtype=np.float64) for g in grams]
+    lam = np.ascontiguousarray(lam, dtype=np.float64)
+    mt = np.ascontiguousarray(last_mttkrp, dtype=np.float64)
+    return lib().orc_cpd_fit(ctypes.c_int(len(dims)), _p64(dims), ctypes.c_int(factors[0].shape[1]),
+                             _ptr_array(factors, "f"), _pf(lam), _ptr_array(grams, "f"), _pf(mt),
+                             ctypes.c_double(normX), ctypes.c_int(fit_mode))
+
+
 def gram_upper(U):
     U = np.ascontiguousarray(U, dtype=np.float64)
     R = U.shape[1]

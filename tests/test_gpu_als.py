@@ -143,3 +143,29 @@ def test_als_graph_replay_matches_checked_loop(gpu, prec, tol):
     for a, b in zip(gf, sf):
         np.testing.assert_allclose(a, b, rtol=10 * tol, atol=10 * tol)
     assert tim[0] > 0 and tim[7] > 0
+
+
+@pytest.mark.parametrize("R", [4, 8, 32])
+def test_cpd_fit_entry_matches_closed_forms(gpu, R):
+    """alto_hip_cpd_fit (cpd_fit_alto, cpd.hpp:193-275): the reference's
+    diagonal convention equals the sum of the dense rank-1 terms' squared
+    norms, the exact one ||X - M|| / ||X||, and both equal the oracle's."""
+    dims, cidx, vals = rand_tensor([9, 7, 6], 150, seed=97, distinct=True)
+    F = rand_factors(dims, R, seed=98)
+    lam = np.random.default_rng(99).uniform(0.5, 2.0, R)
+    X = np.zeros(dims)
+    np.add.at(X, tuple(c.astype(np.int64) for c in cidx), vals)
+    terms = [lam[r] * np.einsum("i,j,k->ijk", F[0][:, r], F[1][:, r], F[2][:, r]) for r in range(R)]
+    M = sum(terms)
+    nx = float((X ** 2).sum())
+    mt = np.einsum("ijk,ir,jr->kr", X, F[0], F[1])
+    grams = [O.gram_upper(f) for f in F]
+    inner = float((X * M).sum())
+    r = nx + sum(float((t ** 2).sum()) for t in terms) - 2 * inner
+    want_diag = (np.sqrt(r) if r > 0 else r) / np.sqrt(nx)
+    want_exact = np.linalg.norm(X - M) / np.sqrt(nx)
+    got_diag = gpu.cpd_fit(F[2], mt, lam, grams, nx, gpu.FIT_REFERENCE_DIAG)
+    got_exact = gpu.cpd_fit(F[2], mt, lam, grams, nx, gpu.FIT_EXACT)
+    assert abs(got_diag - want_diag) < 1e-10 * max(1.0, abs(want_diag))
+    assert abs(got_exact - want_exact) < 1e-10
+    assert abs(got_diag - O.cpd_fit(dims, F, lam, grams, mt, nx, O.FIT_REFERENCE_DIAG)) < 1e-12 * max(1.0, abs(want_diag))

@@ -290,6 +290,34 @@ def test_cpd_fit_exact_is_relative_residual():
     assert abs(fd[-1] - np.linalg.norm(X - M1) / np.linalg.norm(X)) < 1e-10
 
 
+@pytest.mark.parametrize("R", [4, 8])
+def test_cpd_fit_reference_diag_is_sum_of_rank_one_norms(R):
+    """At rank > 1 the reference's fit (cpd.hpp:228-268 reading Gram lower
+    triangles that update_gram never writes, gram.cpp:35-41) keeps only the
+    diagonal of lambda lambda^T (*) Had_m G_m, which is the sum over r of the
+    squared norms of the dense rank-1 terms lambda_r u_1r o u_2r o u_3r; the
+    exact convention is ||X - M|| / ||X||.  Non-orthogonal factors, so the
+    two differ.  <X, M> and the norms come from dense numpy tensors."""
+    dims, cidx, vals = rand_tensor([7, 6, 5], 90, seed=19, distinct=True)
+    F = rand_factors(dims, R, seed=20)
+    lam = np.random.default_rng(21).uniform(0.5, 2.0, R)
+    X = _dense(dims, cidx, vals)
+    terms = [lam[r] * np.einsum("i,j,k->ijk", F[0][:, r], F[1][:, r], F[2][:, r]) for r in range(R)]
+    M = sum(terms)
+    diag_norm = sum(float((t ** 2).sum()) for t in terms)
+    inner = float((X * M).sum())
+    nx = float((X ** 2).sum())
+    mt = np.einsum("ijk,ir,jr->kr", X, F[0], F[1])  # MTTKRP of the last mode
+    grams = [O.gram_upper(f) for f in F]            # upper triangles, zero below
+    r = nx + diag_norm - 2 * inner
+    want_diag = (np.sqrt(r) if r > 0 else r) / np.sqrt(nx)
+    got_diag = O.cpd_fit(dims, F, lam, grams, mt, nx, O.FIT_REFERENCE_DIAG)
+    assert abs(got_diag - want_diag) < 1e-10 * max(1.0, abs(want_diag))
+    got_exact = O.cpd_fit(dims, F, lam, grams, mt, nx, O.FIT_EXACT)
+    assert abs(got_exact - np.linalg.norm(X - M) / np.sqrt(nx)) < 1e-10
+    assert abs(got_diag - got_exact) > 1e-3  # the conventions do differ here
+
+
 def test_cpd_als_decreases_residual_and_stops_on_epsilon():
     dims, cidx, vals = rand_tensor([20, 15, 10], 800, seed=17)
     at = O.AltoTensor(dims, cidx, vals, nprtn=4)
