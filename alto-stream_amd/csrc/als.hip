@@ -236,8 +236,11 @@ static int als_run(TensorImpl *t, const alto_hip_als_opts *o, double *const *fac
         for (int it = 0; it < o->max_iters; ++it) {
             if (it == 0) {
                 ALTO_TRY(enqueue(true));
+            } else if (!graphs_enabled()) {
+                ALTO_TRY(enqueue(false));
             } else {
                 if (!gx.ge) {
+                    CaptureScope cs;  // no device allocation or release meanwhile
                     hipGraph_t g = nullptr;
                     ALTO_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
                     const int st = enqueue(false);

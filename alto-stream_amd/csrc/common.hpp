@@ -4,6 +4,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stdio.h>
+
+#include <mutex>
+#include <shared_mutex>
 #include <string>
 
 #include "../../include/alto_hip.h"
@@ -40,6 +43,30 @@ hipStream_t default_stream();
 inline hipStream_t to_stream(void *s) { return s ? (hipStream_t)s : default_stream(); }
 int ensure_init();
 
+// Graph capture and device memory.  A thread that captures a stream into a
+// graph holds capture_mutex() shared from the start of the capture to the
+// end of instantiation, with its capture_flag() set; device allocation and
+// release take the mutex exclusively.  So no hipMalloc or hipFree (which
+// waits for the whole device) runs while any thread of the process is
+// capturing, and an allocation requested inside a capture (which would not
+// be part of the graph) is refused with an error.
+std::shared_mutex &capture_mutex();
+bool &capture_flag();
+
+// Graph replay of repeated launch sequences (CP-ALS and CP-stream
+// iterations); ALTO_HIP_GRAPHS=0 in the environment turns it off (the same
+// launches are then issued directly every time, e.g. for tools that do not
+// follow graph launches).
+bool graphs_enabled();
+
+struct CaptureScope {
+    std::shared_lock<std::shared_mutex> lk;
+    CaptureScope() : lk(capture_mutex()) { capture_flag() = true; }
+    ~CaptureScope() { capture_flag() = false; }
+    CaptureScope(const CaptureScope &) = delete;
+    CaptureScope &operator=(const CaptureScope &) = delete;
+};
+
 // Device buffer owned by RAII.
 struct DevBuf {
     void *p = nullptr;
@@ -49,14 +76,26 @@ struct DevBuf {
     DevBuf &operator=(const DevBuf &) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            if (capture_flag()) {
+                (void)hipFree(p);
+            } else {
+                std::unique_lock<std::shared_mutex> lk(capture_mutex());
+                (void)hipFree(p);
+            }
+        }
         p = nullptr;
         bytes = 0;
     }
     int alloc(size_t n) {
         if (n <= bytes && p) return ALTO_OK;
+        if (capture_flag()) {
+            set_error("device allocation of %zu bytes requested inside a graph capture", n);
+            return ALTO_ERR_HIP;
+        }
         release();
         if (n == 0) n = 16;
+        std::unique_lock<std::shared_mutex> lk(capture_mutex());
         hipError_t e = hipMalloc(&p, n);
         if (e != hipSuccess) {
             p = nullptr;

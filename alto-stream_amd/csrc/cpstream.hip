@@ -126,89 +126,182 @@ __global__ __launch_bounds__(256) void k_admm_first(double *__restrict__ mt,
     if (t == 0) part[blockIdx.x] = tot;
 }
 
-// One AO-ADMM iteration (admm.cpp:331-405) on the state (aux, dual):
-// x = max(aux - d, 0) is the current primal; y = x - aux; d += y;
-// aux = (mt + rho (x + d)) * vinv.  Per-block partials of p_norm, d_norm,
-// p_res.  Skips everything once the convergence flag is set.
-__global__ __launch_bounds__(256) void k_admm_iter(const double *__restrict__ mt,
+// Cross-workgroup traffic of the ADMM loop goes only through device-scope
+// atomic accesses (stores and loads carry the device coherence scope, so a
+// store is acknowledged once it is visible device-wide); the wait for that
+// acknowledgement orders it before the arrival count.  No cache writeback or
+// invalidation is needed: each workgroup's rows are its own.
+__device__ __forceinline__ void store_dev(double *p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_dev(unsigned *p, unsigned v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double load_dev(const double *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned load_dev(const unsigned *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void wait_stores() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// Sums over workgroups of the three device-scope partials (wave 0; at most
+// 1024 workgroups): all loads are issued before any is used.
+__device__ void sum_partials(const double *part, unsigned nblk, double *out) {
+    const int lane = threadIdx.x & 63;
+    double v[16][3];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const unsigned b = lane + 64u * j;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) v[j][k] = b < nblk ? load_dev(part + (size_t)b * 3 + k) : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double a = 0.0;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) a += v[j][k];
+        for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
+        out[k] = a;
+    }
+}
+
+// The AO-ADMM loop (admm.cpp:331-413) of one factor in one launch: at most
+// max_iter iterations on the state (aux, dual), each
+//   x = max(aux - d, 0) (the current primal); y = x - aux; d += y;
+//   aux = (mt + rho (x + d)) * vinv,
+// with p_norm, d_norm and p_res summed per workgroup.  The convergence test
+// (float tolerance 1e-6f; d_res is the first-pass value at iteration 0 and 0
+// afterwards: the reference sets init = primal in every later prox step) is
+// a grid-wide decision: every workgroup publishes its partials and arrives
+// at bar[0]; the last to arrive sums the partials in workgroup order,
+// counts the iteration, stores the decision in bar[2] and advances the
+// generation bar[1], which the others wait for.  Then U = max(aux - dual,
+// 0) (the new primal).  The grid must be resident at once (the launch sizes
+// it from the occupancy); a wait longer than `limit` wall-clock ticks sets
+// *err and ends the kernel (its results are then not used).
+__global__ __launch_bounds__(256) void k_admm_loop(const double *__restrict__ mt,
                                                    double *__restrict__ aux,
                                                    double *__restrict__ dual,
                                                    const double *__restrict__ vinv,
                                                    const double *__restrict__ rho_p,
-                                                   const int *__restrict__ flag, uint64_t rows,
-                                                   int R, int RP, double *__restrict__ part) {
-    if (*flag) return;
+                                                   double *__restrict__ U, uint64_t rows, int R, int RP,
+                                                   double *part, const double *__restrict__ part0,
+                                                   unsigned *bar, int *err, long long limit,
+                                                   int max_iter, double *iters) {
     extern __shared__ double lds[];
     const int lanes = 256 / RP, TR = lanes * 4;
     double *Wv = lds, *T2 = lds + R * R;
     __shared__ double red[4];
+    __shared__ int conv, last;
     const int t = threadIdx.x, c = t % RP, rl = t / RP;
+    const unsigned nblk = gridDim.x;
     for (int e = t; e < R * R; e += 256) Wv[e] = vinv[e];
     const double rho = *rho_p;
-    double pn = 0.0, dn = 0.0, pr = 0.0;
-    uint64_t ntiles = (rows + TR - 1) / TR;
-    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const uint64_t ntiles = (rows + TR - 1) / TR;
+    for (int it = 0; it < max_iter; ++it) {
+        double pn = 0.0, dn = 0.0, pr = 0.0;
+        for (uint64_t tile = blockIdx.x; tile < ntiles; tile += nblk) {
+            uint64_t r0 = tile * TR;
+            int nr = (int)((rows - r0) < (uint64_t)TR ? (rows - r0) : TR);
+            __syncthreads();
+            double di_keep[4];
+            if (c < R)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    int rr = rl + q * lanes;
+                    if (rr < nr) {
+                        uint64_t g = (r0 + rr) * R + c;
+                        double a = aux[g], d = dual[g];
+                        double x = a - d;
+                        x = x < 0.0 ? 0.0 : x;
+                        pn += x * x;
+                        double y = x - a;
+                        double di = d + y;
+                        dn += di * di;
+                        pr += y * y;
+                        di_keep[q] = di;
+                        T2[rr * R + c] = mt[g] + rho * (x + di);
+                    }
+                }
+            __syncthreads();
+            if (c < R)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    int rr = rl + q * lanes;
+                    if (rr < nr) {
+                        uint64_t g = (r0 + rr) * R + c;
+                        double a = 0.0;
+                        for (int k = 0; k < R; ++k) a += T2[rr * R + k] * Wv[k * R + c];
+                        aux[g] = a;
+                        dual[g] = di_keep[q];
+                    }
+                }
+        }
+        double s0 = block_sum(pn, red);
+        double s1 = block_sum(dn, red);
+        double s2 = block_sum(pr, red);
+        unsigned gen = 0;
+        if (t == 0) {
+            store_dev(part + blockIdx.x * 3 + 0, s0);
+            store_dev(part + blockIdx.x * 3 + 1, s1);
+            store_dev(part + blockIdx.x * 3 + 2, s2);
+            gen = load_dev(bar + 1);
+            wait_stores();
+            last = atomicAdd(bar, 1u) == nblk - 1;
+        }
+        __syncthreads();
+        if (last) {
+            if (t < 64) {
+                double sums[3];
+                sum_partials(part, nblk, sums);
+                const double spn = sums[0], sdn = sums[1], spr = sums[2];
+                const double sdr = it == 0 ? wave_reduce_entry(part0, nblk, 1, 0, 0) : 0.0;
+                if (t == 0) {
+                    const double tol = (double)1e-6f;
+                    conv = spr <= tol * spn && sdr <= tol * sdn;
+                    atomicAdd(iters, 1.0);
+                    store_dev(bar + 2, (unsigned)conv);
+                    store_dev(bar, 0u);
+                    wait_stores();
+                    atomicAdd(bar + 1, 1u);
+                }
+            }
+        } else if (t == 0) {
+            const long long t0 = wall_clock64();
+            int ok = 1;
+            while (load_dev(bar + 1) == gen) {
+                if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+                    wall_clock64() - t0 > limit) {
+                    atomicOr(err, 1);
+                    ok = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+            conv = ok ? (int)load_dev(bar + 2) : -1;
+        }
+        __syncthreads();
+        if (conv < 0) return;
+        if (conv) break;
+    }
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += nblk) {
         uint64_t r0 = tile * TR;
         int nr = (int)((rows - r0) < (uint64_t)TR ? (rows - r0) : TR);
-        __syncthreads();
-        double di_keep[4];
         if (c < R)
-#pragma unroll
             for (int q = 0; q < 4; ++q) {
                 int rr = rl + q * lanes;
                 if (rr < nr) {
                     uint64_t g = (r0 + rr) * R + c;
-                    double a = aux[g], d = dual[g];
-                    double x = a - d;
-                    x = x < 0.0 ? 0.0 : x;
-                    pn += x * x;
-                    double y = x - a;
-                    double di = d + y;
-                    dn += di * di;
-                    pr += y * y;
-                    di_keep[q] = di;
-                    T2[rr * R + c] = mt[g] + rho * (x + di);
-                }
-            }
-        __syncthreads();
-        if (c < R)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                int rr = rl + q * lanes;
-                if (rr < nr) {
-                    uint64_t g = (r0 + rr) * R + c;
-                    double a = 0.0;
-                    for (int k = 0; k < R; ++k) a += T2[rr * R + k] * Wv[k * R + c];
-                    aux[g] = a;
-                    dual[g] = di_keep[q];
+                    double x = aux[g] - dual[g];
+                    U[g] = x > 0.0 ? x : 0.0;
                 }
             }
     }
-    double s0 = block_sum(pn, red);
-    double s1 = block_sum(dn, red);
-    double s2 = block_sum(pr, red);
-    if (t == 0) {
-        part[blockIdx.x * 3 + 0] = s0;
-        part[blockIdx.x * 3 + 1] = s1;
-        part[blockIdx.x * 3 + 2] = s2;
-    }
-}
-
-// Convergence test after iteration `it` (admm.cpp:409-413), float tolerance
-// 1e-6f.  d_res is the first-pass value at it == 0 and 0 afterwards (the
-// reference sets init = primal in every later prox step).
-__global__ void k_admm_check(const double *__restrict__ part, int nblk,
-                             const double *__restrict__ part0, int it, int *__restrict__ flag,
-                             double *__restrict__ iters) {
-    if (*flag) return;
-    double pn = wave_reduce_entry(part, nblk, 3, 0, 0);
-    double dn = wave_reduce_entry(part, nblk, 3, 1, 0);
-    double pr = wave_reduce_entry(part, nblk, 3, 2, 0);
-    double dr = it == 0 ? wave_reduce_entry(part0, nblk, 1, 0, 0) : 0.0;
-    if (threadIdx.x != 0) return;
-    *iters += 1.0;
-    const double tol = (double)1e-6f;
-    if (pr <= tol * pn && dr <= tol * dn) *flag = 1;
 }
 
 __global__ void k_admm_final(const double *__restrict__ aux, const double *__restrict__ dual,
@@ -272,10 +365,6 @@ __global__ void k_delta_add(const double *__restrict__ part, int nblk, double *_
     if (threadIdx.x == 0) *delta += sqrt(a) / (sqrt(b) + 1e-12);
 }
 
-__global__ void k_zero_flag(int *p) {
-    if (threadIdx.x == 0) p[0] = 0;
-}
-
 // Stream-mode index of a batch's nonzeros: time index - the batch's first.
 __global__ void k_local_time(uint64_t *__restrict__ out, const uint64_t *__restrict__ t, uint64_t n,
                              uint64_t t0) {
@@ -284,6 +373,14 @@ __global__ void k_local_time(uint64_t *__restrict__ out, const uint64_t *__restr
 }
 
 static constexpr int kAdmmMaxIter = 25;
+
+// Workgroups of the ADMM kernels: one per row tile, at most `resident` (the
+// ADMM loop's grid barrier needs the whole grid resident at once; at most
+// 4 per CU and 1024 in all, the halves of the partials buffer).
+static int admm_blocks(uint64_t rows, const TileShape &ts, int resident) {
+    uint64_t n = (rows + ts.TR - 1) / ts.TR;
+    return (int)std::min<uint64_t>(std::max<uint64_t>(n, 1), (uint64_t)resident);
+}
 
 }  // namespace alto
 
@@ -327,7 +424,7 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
     for (int k = 0; k < nsl; ++k) maxslice = std::max(maxslice, slices[k].end - slices[k].start);
     uint64_t used = nsl ? slices[nsl - 1].end : 0;
     DevBuf zidx, U[kMaxModes], P[kMaxModes], Uf[kMaxModes], duals[kMaxModes];
-    DevBuf G, oldG, ata, cross, gpart, vinv, flag, rho, buf, aux, part0, part, dpart, delta,
+    DevBuf G, oldG, ata, cross, gpart, vinv, flag, bar, rho, buf, aux, part0, part, dpart, delta,
         admm_it, colp, lam;
     DevBuf *dci = sin.cid;
     DevBuf &dvals = sin.vals;
@@ -346,6 +443,9 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
     ALTO_TRY(gpart.alloc(sizeof(double) * kMaxBlocks * RR));
     ALTO_TRY(vinv.alloc(sizeof(double) * RR));
     ALTO_TRY(flag.alloc(sizeof(int) * 4));
+    ALTO_TRY(bar.alloc(sizeof(unsigned) * 4));
+    ALTO_HIP_TRY(hipMemsetAsync(flag.p, 0, sizeof(int) * 4, st));
+    ALTO_HIP_TRY(hipMemsetAsync(bar.p, 0, sizeof(unsigned) * 4, st));
     ALTO_TRY(rho.alloc(sizeof(double)));
     ALTO_TRY(buf.alloc(sizeof(double) * maxall * R));
     ALTO_TRY(aux.alloc(sizeof(double) * maxall * R));
@@ -373,9 +473,26 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
         std::call_once(attr, [] {
             (void)hipFuncSetAttribute((const void *)k_admm_first,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 256);
-            (void)hipFuncSetAttribute((const void *)k_admm_iter,
+            (void)hipFuncSetAttribute((const void *)k_admm_loop,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 256);
         });
+    }
+    // ADMM loop grid: resident at once (occupancy x CUs, at most 4 per CU and
+    // 1024 in all); its barrier gives up after 50 ms of wall clock
+    int admm_resident = 1;
+    long long bar_limit = 0;
+    {
+        int dev = 0, cus = 0, occ = 0, khz = 0;
+        ALTO_HIP_TRY(hipGetDevice(&dev));
+        ALTO_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        ALTO_HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+        ALTO_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_admm_loop, 256, lds_iter));
+        if (occ < 1 || cus < 1 || khz < 1) {
+            set_error("ADMM loop kernel: no resident workgroup (occupancy %d, %d CUs)", occ, cus);
+            return ALTO_ERR_UNSUPPORTED;
+        }
+        admm_resident = std::min(cus * std::min(occ, 4), 1024);
+        bar_limit = (long long)khz * 50;
     }
     EventSet evs;
     hipEvent_t ev[4];
@@ -498,39 +615,21 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
                     ALTO_TRY(launch_chol_inv(G.as<double>(), N, j, R, 1e-12, false, vinv.as<double>(),
                                              flag.as<int>() + 1, nullptr, st, 1, nullptr));
                 }
-                k_admm_first<<<nb, 256, lds_first, st>>>(buf.as<double>(), P[j].as<double>(),
+                const int nba = admm_blocks(I, ts, admm_resident);
+                k_admm_first<<<nba, 256, lds_first, st>>>(buf.as<double>(), P[j].as<double>(),
                                                          ata.as<double>(), U[j].as<double>(),
                                                          duals[j].as<double>(), vinv.as<double>(),
                                                          rho.as<double>(), I, R, ts.RP, nonneg,
                                                          aux.as<double>(), part0.as<double>());
                 ALTO_LAUNCH_CHECK();
                 if (nonneg) {
-                    // At most 25 iterations; once k_admm_check sets the flag the
-                    // remaining launches would do nothing, so they are issued in
-                    // short chunks and the flag is read between chunks (the
-                    // results are those of issuing all 25; the ADMM typically
-                    // converges in 2-3 iterations).
-                    k_zero_flag<<<1, 64, 0, st>>>(flag.as<int>());
-                    int it = 0;
-                    while (it < kAdmmMaxIter) {
-                        const int chunk = it == 0 ? 3 : 4;
-                        for (int c = 0; c < chunk && it < kAdmmMaxIter; ++c, ++it) {
-                            k_admm_iter<<<nb, 256, lds_iter, st>>>(buf.as<double>(), aux.as<double>(),
-                                                                   duals[j].as<double>(), vinv.as<double>(),
-                                                                   rho.as<double>(), flag.as<int>(), I, R,
-                                                                   ts.RP, part.as<double>());
-                            k_admm_check<<<1, 64, 0, st>>>(part.as<double>(), nb, part0.as<double>(), it,
-                                                           flag.as<int>(), admm_it.as<double>());
-                        }
-                        ALTO_LAUNCH_CHECK();
-                        if (it >= kAdmmMaxIter) break;
-                        int conv = 0;
-                        ALTO_HIP_TRY(hipMemcpyAsync(&conv, flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
-                        ALTO_HIP_TRY(hipStreamSynchronize(st));
-                        if (conv) break;
-                    }
-                    k_admm_final<<<cdiv(I * R, 256), 256, 0, st>>>(aux.as<double>(), duals[j].as<double>(),
-                                                                   U[j].as<double>(), I * R);
+                    k_admm_loop<<<nba, 256, lds_iter, st>>>(buf.as<double>(), aux.as<double>(),
+                                                            duals[j].as<double>(), vinv.as<double>(),
+                                                            rho.as<double>(), U[j].as<double>(), I, R, ts.RP,
+                                                            part.as<double>(), part0.as<double>(),
+                                                            bar.as<unsigned>(), flag.as<int>() + 3,
+                                                            bar_limit, kAdmmMaxIter, admm_it.as<double>());
+                    ALTO_LAUNCH_CHECK();
                 }
                 ALTO_TRY(gram(j, I));
                 ALTO_TRY(refresh_f32(j, I));
@@ -541,20 +640,25 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
             }
             return ALTO_OK;
         };
-        // Without the constraint the iterations after the first replay one
-        // captured graph (kept across slices, updated in place); with it they
-        // run directly (the ADMM reads its flag).
+        // The iterations after the first replay one captured graph (kept
+        // across batches, updated in place); the host reads delta and the
+        // Cholesky flags once per outer iteration.
         for (int i = 0; i < max_iters; ++i) {
-            if (nonneg) ALTO_TRY(outer_iter());
-            else ALTO_TRY(ig.run(i, outer_iter));
+            ALTO_TRY(ig.run(i, outer_iter));
             double hd[2];
             int hf[4];
             ALTO_HIP_TRY(hipMemcpyAsync(hd, delta.p, sizeof(double), hipMemcpyDeviceToHost, st));
-            ALTO_HIP_TRY(hipMemcpyAsync(hf, flag.p, sizeof(int) * 3, hipMemcpyDeviceToHost, st));
+            ALTO_HIP_TRY(hipMemcpyAsync(hf, flag.p, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
             ALTO_HIP_TRY(hipStreamSynchronize(st));
             if (nonneg && hf[2]) {
                 set_error("Cholesky factorisation failed in ADMM (a mode's Gram Hadamard, batch %d)", k);
                 return ALTO_ERR_NUMERIC;
+            }
+            if (nonneg && hf[3]) {
+                set_error("ADMM loop: its workgroups were not all resident at once (the device is shared "
+                          "with other work), batch %d",
+                          k);
+                return ALTO_ERR_HIP;
             }
             float bms = 0;
             (void)hipEventElapsedTime(&bms, ev[2], ev[3]);

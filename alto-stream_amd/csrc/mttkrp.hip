@@ -953,6 +953,10 @@ static int autotune(TensorImpl *t, MttkrpArgs &a, hipStream_t s) {
             return ALTO_OK;
         }
     }
+    if (capture_flag()) {  // timing launches cannot run inside a capture
+        set_error("MTTKRP launch tuning requested inside a graph capture");
+        return ALTO_ERR_HIP;
+    }
     const uint64_t rows = t->dims[a.target];
     const size_t osz = a.out64 ? 8 : 4;
     DevBuf scratch;

@@ -1,6 +1,7 @@
 // Runtime part of the C ABI: device selection, memory, streams, timers,
 // precision conversion, error reporting.
 #include <stdarg.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include "common.hpp"
@@ -18,6 +19,22 @@ void set_error(const char *fmt, ...) {
     va_end(ap);
 }
 const char *last_error() { return g_err; }
+
+std::shared_mutex &capture_mutex() {
+    static std::shared_mutex m;
+    return m;
+}
+bool &capture_flag() {
+    static thread_local bool f = false;
+    return f;
+}
+bool graphs_enabled() {
+    static const bool on = [] {
+        const char *e = getenv("ALTO_HIP_GRAPHS");
+        return !(e && e[0] == '0' && e[1] == 0);
+    }();
+    return on;
+}
 
 int ensure_init() {
     if (g_device >= 0) return ALTO_OK;
@@ -106,6 +123,7 @@ int alto_hip_device_name(char *buf, size_t len) {
 int alto_hip_malloc(void **ptr, size_t bytes) {
     if (!ptr) return ALTO_ERR_ARG;
     ALTO_TRY(ensure_init());
+    std::unique_lock<std::shared_mutex> lk(capture_mutex());
     hipError_t e = hipMalloc(ptr, bytes ? bytes : 16);
     if (e != hipSuccess) {
         set_error("hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
@@ -115,6 +133,7 @@ int alto_hip_malloc(void **ptr, size_t bytes) {
 }
 
 int alto_hip_free(void *ptr) {
+    std::unique_lock<std::shared_mutex> lk(capture_mutex());
     if (ptr) ALTO_HIP_TRY(hipFree(ptr));
     return ALTO_OK;
 }

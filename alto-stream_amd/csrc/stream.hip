@@ -360,6 +360,27 @@ namespace alto {
 // next_dynamic_batch (src/streaming_sptensor.cpp:196-264): from the first
 // time index not yet taken, take whole time slices while fewer than
 // nnz_threshold nonzeros are taken, stopping after timeslice_limit of them.
+int graph_signature(hipGraph_t g, GraphSig *sig) {
+    size_t n = 0;
+    ALTO_HIP_TRY(hipGraphGetNodes(g, nullptr, &n));
+    std::vector<hipGraphNode_t> nodes(n);
+    if (n) ALTO_HIP_TRY(hipGraphGetNodes(g, nodes.data(), &n));
+    sig->nodes.clear();
+    sig->nodes.reserve(n);
+    for (size_t i = 0; i < n; ++i) {
+        hipGraphNodeType ty;
+        ALTO_HIP_TRY(hipGraphNodeGetType(nodes[i], &ty));
+        const void *fn = nullptr;
+        if (ty == hipGraphNodeTypeKernel) {
+            hipKernelNodeParams kp{};
+            ALTO_HIP_TRY(hipGraphKernelNodeGetParams(nodes[i], &kp));
+            fn = kp.func;
+        }
+        sig->nodes.emplace_back((int)ty, fn);
+    }
+    return ALTO_OK;
+}
+
 std::vector<Slice> stream_batches(const std::vector<Slice> &slices, int s, uint64_t nnz_threshold,
                                   uint64_t timeslice_limit) {
     const uint64_t thr = nnz_threshold ? nnz_threshold : ~0ULL;

@@ -2,6 +2,7 @@
 #pragma once
 #include <stdint.h>
 
+#include <utility>
 #include <vector>
 
 #include "tensor.hpp"
@@ -55,6 +56,14 @@ struct EventSet {
 // the reference asserts).
 int stream_prepare(const alto_hip_coo *coo, int s, StreamInput *out, hipStream_t st);
 
+// The nodes of a captured graph in creation order: type and, for kernel
+// nodes, the kernel function.
+struct GraphSig {
+    std::vector<std::pair<int, const void *>> nodes;
+    bool operator==(const GraphSig &o) const { return nodes == o.nodes; }
+};
+int graph_signature(hipGraph_t g, GraphSig *sig);
+
 // Runs the enqueue of one outer iteration directly the first time and, from
 // the second time on, as one captured graph replayed on the same stream (the
 // iterations of a slice are identical launches over the same buffers).  The
@@ -62,6 +71,7 @@ int stream_prepare(const alto_hip_coo *coo, int s, StreamInput *out, hipStream_t
 struct IterGraph {
     hipStream_t s = nullptr;
     hipGraphExec_t ge = nullptr;
+    GraphSig sig;          // nodes of the capture ge was made from
     bool current = false;  // ge holds this sequence's capture
     explicit IterGraph(hipStream_t st) : s(st) {}
     IterGraph(const IterGraph &) = delete;
@@ -74,16 +84,20 @@ struct IterGraph {
     }
     // Iteration 0 of a sequence runs directly; iteration 1 is captured and
     // iterations >= 1 replay the capture.  A sequence after the first (the
-    // next slice) updates the executable graph in place when the capture has
-    // the same shape (hipGraphExecUpdate, cheaper than instantiating), and
-    // instantiates a new one otherwise.
+    // next slice) updates the executable graph in place
+    // (hipGraphExecUpdate, cheaper than instantiating) when the new capture
+    // has the same nodes calling the same kernels in the same order (only
+    // arguments, grids and copy extents differ: the update's contract), and
+    // instantiates a new one otherwise (the MTTKRP's kernel variant can
+    // change with a slice's shape).  The capture runs under CaptureScope.
     template <typename F>
     int run(int iter, F enqueue) {
-        if (iter == 0) {
+        if (iter == 0 || !graphs_enabled()) {
             current = false;
             return enqueue();
         }
         if (!current) {
+            CaptureScope cs;
             hipGraph_t g = nullptr;
             ALTO_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             const int st = enqueue();
@@ -93,12 +107,19 @@ struct IterGraph {
                 if (st != ALTO_OK) return st;
                 ALTO_HIP_TRY(ce);
             }
+            GraphSig ns;
+            const int gs = graph_signature(g, &ns);
+            if (gs != ALTO_OK) {
+                (void)hipGraphDestroy(g);
+                return gs;
+            }
             if (ge) {
                 // no launch of the previous capture may still be running
                 hipError_t ue = hipStreamSynchronize(s);
                 hipGraphNode_t bad = nullptr;
                 hipGraphExecUpdateResult res = hipGraphExecUpdateSuccess;
-                if (ue == hipSuccess) ue = hipGraphExecUpdate(ge, g, &bad, &res);
+                if (ue == hipSuccess && ns == sig) ue = hipGraphExecUpdate(ge, g, &bad, &res);
+                else if (ue == hipSuccess) res = hipGraphExecUpdateErrorTopologyChanged;
                 if (ue != hipSuccess || res != hipGraphExecUpdateSuccess) {
                     (void)hipGetLastError();
                     (void)hipGraphExecDestroy(ge);
@@ -109,6 +130,7 @@ struct IterGraph {
             if (!ge) ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
             ALTO_HIP_TRY(ie);
+            sig = std::move(ns);
             current = true;
         }
         ALTO_HIP_TRY(hipGraphLaunch(ge, s));
