@@ -15,61 +15,26 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <string>
 #include <thread>
 #include <vector>
 
 #include "dense.hpp"
+#include "fit.hpp"
 #include "tensor.hpp"
 
 namespace alto {
 
-// inner = sum over blocks/columns of u*M; ||M||^2 exact and diagonal-only.
-// Writes (fit, exact fit) to fitd[2k], fitd[2k+1] with k = *cnt (then
-// incremented) when cnt != nullptr, else k = slot.
 __global__ __launch_bounds__(256) void k_fit(const double *__restrict__ grams, int nmode, int R,
                                              const double *__restrict__ lambda,
                                              const double *__restrict__ partials, int nblk,
                                              const double *__restrict__ normX, int fit_mode,
                                              double *__restrict__ fitd, int *__restrict__ cnt,
                                              int slot) {
-    extern __shared__ double sh[];  // R*R + R
-    double *tg = sh, *ipc = sh + R * R;
-    const int t = threadIdx.x, RR = R * R;
-    for (int e = t; e < RR; e += blockDim.x) {
-        int i = e / R, j = e % R;
-        double v = lambda[i] * lambda[j];
-        for (int m = 0; m < nmode; ++m) v *= grams[(size_t)m * RR + e];
-        tg[e] = v;
-    }
-    for (int c = t >> 6; c < R; c += blockDim.x >> 6) {
-        double s = 0.0;
-        const int lane = t & 63;
-        for (int b = lane; b < nblk; b += 64) s += partials[(size_t)b * 3 * R + 2 * R + c];
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
-        if (lane == 0) ipc[c] = s;
-    }
-    __syncthreads();
-    if (t == 0) {
-        double inner = 0.0;
-        for (int c = 0; c < R; ++c) inner += ipc[c];
-        double nx = *normX;
-        double ue = 0.0, ud = 0.0;
-        for (int e = 0; e < RR; ++e) ue += tg[e];
-        for (int i = 0; i < R; ++i) ud += tg[i * R + i];
-        double f[2];
-        double nu[2] = {fabs(ue), fabs(ud)};
-        for (int k = 0; k < 2; ++k) {
-            double r = nx + nu[k] - 2.0 * inner;
-            if (r > 0.0) r = sqrt(r);
-            f[k] = r / sqrt(nx);
-        }
-        const int k = cnt ? *cnt : slot;
-        fitd[2 * k] = fit_mode == ALTO_FIT_EXACT ? f[0] : f[1];
-        fitd[2 * k + 1] = f[0];
-        if (cnt) *cnt = k + 1;
-    }
+    k_fit_body(grams, nmode, R, lambda, partials, nblk, normX, fit_mode, fitd, cnt, slot);
 }
 
 static size_t prec_size(int prec) { return prec == ALTO_PREC_F64 ? 8 : prec == ALTO_PREC_F32 ? 4 : 2; }
@@ -430,6 +395,24 @@ extern "C" int alto_hip_cpd_als_batch(int ntensors, const alto_hip_tensor *tenso
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
         return tensors[a]->impl.nnz * opts[a].max_iters > tensors[b]->impl.nnz * opts[b].max_iters;
     });
+    // Tensors of a shape class the grouped CP-ALS takes run class by class,
+    // one launch per ALS step and class; the others on the worker threads.
+    std::vector<int> rest;
+    if (grouped_als_enabled()) {
+        std::map<std::tuple<int, int, int, int>, std::vector<int>> cls;
+        for (int k : order) {
+            const TensorImpl *t = &tensors[k]->impl;
+            if (als_group_eligible(t, &opts[k]))
+                cls[std::make_tuple(t->nmode, opts[k].rank, opts[k].prec, opts[k].fit_mode)].push_back(k);
+            else
+                rest.push_back(k);
+        }
+        std::vector<std::vector<int>> classes;
+        for (auto &kv : cls) classes.push_back(kv.second);
+        ALTO_TRY(als_groups_run(classes, tensors, opts, factors, lambdas, fits, iters));
+    } else {
+        rest = order;
+    }
     std::atomic<int> next{0};
     std::atomic<int> status{ALTO_OK};
     std::mutex err_mu;
@@ -446,8 +429,8 @@ extern "C" int alto_hip_cpd_als_batch(int ntensors, const alto_hip_tensor *tenso
         }
         {
             AlsWork w;
-            for (int i = next++; i < ntensors && status == ALTO_OK; i = next++) {
-                const int k = order[i];
+            for (int i = next++; i < (int)rest.size() && status == ALTO_OK; i = next++) {
+                const int k = rest[i];
                 int st = als_run(&tensors[k]->impl, &opts[k], factors[k], lambdas ? lambdas[k] : nullptr,
                                  fits ? fits[k] : nullptr, nullptr, iters ? iters + k : nullptr, nullptr, w, s);
                 if (st != ALTO_OK) {
@@ -462,7 +445,8 @@ extern "C" int alto_hip_cpd_als_batch(int ntensors, const alto_hip_tensor *tenso
         }
         (void)hipStreamDestroy(s);
     };
-    const int nw = nstreams < ntensors ? nstreams : (ntensors > 0 ? ntensors : 1);
+    const int nrest = (int)rest.size();
+    const int nw = nrest == 0 ? 0 : nstreams < nrest ? nstreams : nrest;
     std::vector<std::thread> pool;
     for (int i = 0; i < nw; ++i) pool.emplace_back(worker);
     for (auto &th : pool) th.join();

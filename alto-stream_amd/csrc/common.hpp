@@ -148,4 +148,16 @@ __host__ __device__ inline uint16_t f32_to_bf16(float f) {
 
 inline unsigned cdiv(uint64_t a, uint64_t b) { return (unsigned)((a + b - 1) / b); }
 
+// Grouped launches: the job owning block b, first[j] <= b < first[j + 1]
+// (first: njobs + 1 prefix sums of the jobs' block counts, first[0] = 0).
+__device__ __forceinline__ int find_job(const unsigned *__restrict__ first, int njobs, unsigned b) {
+    int lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (first[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 }  // namespace alto

@@ -298,7 +298,7 @@ __device__ __forceinline__ double readlane_f64(double x, int l) {
 // for the same rows (v_readlane, no LDS).  Fallbacks as k_chol_inv_w; the
 // result is symmetrised.
 template <int R>
-__global__ __launch_bounds__(256) void k_inv_gj(const double *__restrict__ grams, int nmode, int mode,
+__device__ __forceinline__ void k_inv_gj_body(const double *__restrict__ grams, int nmode, int mode,
                                                 double diag_add, double *__restrict__ vinv,
                                                 int *__restrict__ flag, double *__restrict__ phi_out,
                                                 int fail_mode, double *__restrict__ rho_out) {
@@ -441,6 +441,14 @@ __global__ __launch_bounds__(256) void k_inv_gj(const double *__restrict__ grams
     if (t == 0 && flag) *flag = 0;
 }
 
+template <int R>
+__global__ __launch_bounds__(256) void k_inv_gj(const double *__restrict__ grams, int nmode, int mode,
+                                               double diag_add, double *__restrict__ vinv,
+                                               int *__restrict__ flag, double *__restrict__ phi_out,
+                                               int fail_mode, double *__restrict__ rho_out) {
+    k_inv_gj_body<R>(grams, nmode, mode, diag_add, vinv, flag, phi_out, fail_mode, rho_out);
+}
+
 // The same for R in {8, 16, 32} in one wave, with no barrier: lane l owns
 // column l % R for the rows i = l / R (mod C), C = 64 / R.  Step k takes row
 // k (for the lane's column) and column k (for its rows) from the lanes that
@@ -448,7 +456,7 @@ __global__ __launch_bounds__(256) void k_inv_gj(const double *__restrict__ grams
 // every register index is static.  Same arithmetic, fallbacks and output as
 // k_inv_gj.
 template <int R>
-__global__ __launch_bounds__(64) void k_inv_gj1(const double *__restrict__ grams, int nmode, int mode,
+__device__ __forceinline__ void k_inv_gj1_body(const double *__restrict__ grams, int nmode, int mode,
                                                double diag_add, double *__restrict__ vinv,
                                                int *__restrict__ flag, double *__restrict__ phi_out,
                                                int fail_mode, double *__restrict__ rho_out) {
@@ -553,6 +561,14 @@ __global__ __launch_bounds__(64) void k_inv_gj1(const double *__restrict__ grams
     if (t == 0 && flag) *flag = 0;
 }
 
+template <int R>
+__global__ __launch_bounds__(64) void k_inv_gj1(const double *__restrict__ grams, int nmode, int mode,
+                                               double diag_add, double *__restrict__ vinv,
+                                               int *__restrict__ flag, double *__restrict__ phi_out,
+                                               int fail_mode, double *__restrict__ rho_out) {
+    k_inv_gj1_body<R>(grams, nmode, mode, diag_add, vinv, flag, phi_out, fail_mode, rho_out);
+}
+
 int launch_chol_inv(const double *grams, int nmode, int mode, int R, double diag_add,
                     bool copy_first, double *vinv, int *flag, double *phi_out, hipStream_t s,
                     int fail_mode, double *rho_out) {
@@ -652,7 +668,7 @@ __global__ __launch_bounds__(256) void k_solve(double *__restrict__ M, uint64_t 
 // registers, so the inner product reads only the (broadcast) tile row from
 // LDS.  Same tiles, arithmetic order and partials as k_solve.
 template <int R, typename MT>
-__global__ __launch_bounds__(256) void k_solve_r(const MT *M, double *U, uint64_t rows,
+__device__ __forceinline__ void k_solve_r_body(unsigned bid_, unsigned nblk_, const MT *M, double *U, uint64_t rows,
                                                  const double *__restrict__ vinv,
                                                  double *__restrict__ partials) {
     constexpr int lanes = 256 / R, TR = lanes * 4;
@@ -673,15 +689,15 @@ __global__ __launch_bounds__(256) void k_solve_r(const MT *M, double *U, uint64_
 #pragma unroll
         for (int q = 0; q < 4; ++q) pre[q] = t + 256 * q < n ? (double)M[r0 * R + t + 256 * q] : 0.0;
     };
-    if (blockIdx.x < ntiles) fetch(blockIdx.x);
-    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    if (bid_ < ntiles) fetch(bid_);
+    for (uint64_t tile = bid_; tile < ntiles; tile += nblk_) {
         const uint64_t r0 = tile * TR;
         const int nr = (int)((rows - r0) < (uint64_t)TR ? (rows - r0) : TR);
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < 4; ++q) T[t + 256 * q] = pre[q];
         __syncthreads();
-        if (tile + gridDim.x < ntiles) fetch(tile + gridDim.x);
+        if (tile + nblk_ < ntiles) fetch(tile + nblk_);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int rr = rl + q * lanes;
@@ -710,11 +726,18 @@ __global__ __launch_bounds__(256) void k_solve_r(const MT *M, double *U, uint64_
             mx = m2 > mx ? m2 : mx;
             ip += red[512 + l * R + c];
         }
-        double *p = partials + (size_t)blockIdx.x * 3 * R;
+        double *p = partials + (size_t)bid_ * 3 * R;
         p[c] = ss;
         p[R + c] = mx;
         p[2 * R + c] = ip;
     }
+}
+
+template <int R, typename MT>
+__global__ __launch_bounds__(256) void k_solve_r(const MT *M, double *U, uint64_t rows,
+                                                 const double *__restrict__ vinv,
+                                                 double *__restrict__ partials) {
+    k_solve_r_body<R, MT>(blockIdx.x, gridDim.x, M, U, rows, vinv, partials);
 }
 
 // u = M * vinv in place on the FP64 matrix cores, R in {16, 32, 64}, with
@@ -725,7 +748,7 @@ __global__ __launch_bounds__(256) void k_solve_r(const MT *M, double *U, uint64_
 // + l%16]) over the R/4 k-steps; vinv sits in LDS.  D item q of lane l is row
 // l/16 + 4q, column 16J + l%16 of the block.  Dynamic LDS: R^2 + 64 R doubles.
 template <int R>
-__global__ __launch_bounds__(256) void k_solve_mfma(double *M, uint64_t rows, const double *__restrict__ vinv,
+__device__ __forceinline__ void k_solve_mfma_body(unsigned bid_, unsigned nblk_, double *M, uint64_t rows, const double *__restrict__ vinv,
                                                     double *__restrict__ partials) {
     static_assert(R % 16 == 0 && R <= 64, "16-column blocks");
     constexpr int NB = R / 16, KS = R / 4, PL = R / 4;  // column blocks, k-steps, tile doubles per lane
@@ -743,7 +766,7 @@ __global__ __launch_bounds__(256) void k_solve_mfma(double *M, uint64_t rows, co
         mx[J] = -INFINITY;
         ip[J] = 0.0;
     }
-    const uint64_t ntiles = (rows + 15) / 16, stride = (uint64_t)gridDim.x * 4, lim = rows * R;
+    const uint64_t ntiles = (rows + 15) / 16, stride = (uint64_t)nblk_ * 4, lim = rows * R;
     double nx[PL];
     auto load = [&](uint64_t tile) {
         const uint64_t base = tile * 16 * R;
@@ -753,7 +776,7 @@ __global__ __launch_bounds__(256) void k_solve_mfma(double *M, uint64_t rows, co
             nx[p] = e < lim ? M[e] : 0.0;
         }
     };
-    uint64_t tile = (uint64_t)blockIdx.x * 4 + wave;
+    uint64_t tile = (uint64_t)bid_ * 4 + wave;
     if (tile < ntiles) load(tile);
     for (; tile < ntiles; tile += stride) {
         // the wave's LDS operations run in order: these stores land before
@@ -820,11 +843,17 @@ __global__ __launch_bounds__(256) void k_solve_mfma(double *M, uint64_t rows, co
             m = mw > m ? mw : m;
             p += red[(w * 3 + 2) * R + t];
         }
-        double *out = partials + (size_t)blockIdx.x * 3 * R;
+        double *out = partials + (size_t)bid_ * 3 * R;
         out[t] = a;
         out[R + t] = m;
         out[2 * R + t] = p;
     }
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void k_solve_mfma(double *M, uint64_t rows, const double *__restrict__ vinv,
+                                                    double *__restrict__ partials) {
+    k_solve_mfma_body<R>(blockIdx.x, gridDim.x, M, rows, vinv, partials);
 }
 
 int launch_solve(double *M, uint64_t rows, int R, const double *vinv, double *partials, int nblk,
@@ -852,9 +881,9 @@ int launch_solve(double *M, uint64_t rows, int R, const double *vinv, double *pa
     return ALTO_OK;
 }
 
-__global__ __launch_bounds__(256) void k_lambda(const double *__restrict__ partials, int nblk,
+__device__ __forceinline__ void k_lambda_body(unsigned bid_, unsigned nblk_, const double *__restrict__ partials, int nblk,
                                                 int R, int which, double *__restrict__ lambda) {
-    const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int c = bid_ * 4 + (threadIdx.x >> 6);
     if (c >= R) return;
     if (which == 0) {
         double s = wave_reduce_entry(partials, nblk, 3 * (size_t)R, c, 0);
@@ -865,6 +894,11 @@ __global__ __launch_bounds__(256) void k_lambda(const double *__restrict__ parti
     }
 }
 
+__global__ __launch_bounds__(256) void k_lambda(const double *__restrict__ partials, int nblk,
+                                                int R, int which, double *__restrict__ lambda) {
+    k_lambda_body(blockIdx.x, gridDim.x, partials, nblk, R, which, lambda);
+}
+
 int launch_lambda(const double *partials, int nblk, int R, int which, double *lambda,
                   hipStream_t s) {
     k_lambda<<<cdiv(R, 4), 256, 0, s>>>(partials, nblk, R, which, lambda);
@@ -873,7 +907,7 @@ int launch_lambda(const double *partials, int nblk, int R, int which, double *la
 }
 
 template <int MODE, int NQ>  // MODE 0: scale f64 u by 1/lambda and store; 1: read stored factor
-__global__ __launch_bounds__(256) void k_scale_gram(const double *u,
+__device__ __forceinline__ void k_scale_gram_body(unsigned bid_, unsigned nblk_, const double *u,
                                                     const double *__restrict__ lambda,
                                                     const void *__restrict__ Uin,
                                                     void *__restrict__ Uout, int prec,
@@ -888,7 +922,7 @@ __global__ __launch_bounds__(256) void k_scale_gram(const double *u,
 #pragma unroll
     for (int q = 0; q < NQ; ++q) g[q] = 0.0;
     uint64_t ntiles = (rows + TR - 1) / TR;
-    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    for (uint64_t tile = bid_; tile < ntiles; tile += nblk_) {
         uint64_t r0 = tile * TR;
         int nr = (int)((rows - r0) < (uint64_t)TR ? (rows - r0) : TR);
         __syncthreads();
@@ -941,14 +975,22 @@ __global__ __launch_bounds__(256) void k_scale_gram(const double *u,
         }
     }
     if (zero)  // beyond this factor's rows: nobody reads them here
-        for (uint64_t e = rows * R + (uint64_t)blockIdx.x * 256 + t; e < zero_n; e += (uint64_t)gridDim.x * 256)
+        for (uint64_t e = rows * R + (uint64_t)bid_ * 256 + t; e < zero_n; e += (uint64_t)nblk_ * 256)
             zero[e] = 0.0;
-    double *p = gpart + (size_t)blockIdx.x * RR;
+    double *p = gpart + (size_t)bid_ * RR;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
         const int e = t * NQ + q;
         if (e < RR) p[e] = g[q];
     }
+}
+
+template <int MODE, int NQ>
+__global__ __launch_bounds__(256) void k_scale_gram(const double *u, const double *__restrict__ lambda,
+                                                    const void *__restrict__ Uin, void *__restrict__ Uout,
+                                                    int prec, uint64_t rows, int R, double *__restrict__ gpart,
+                                                    double *zero, uint64_t zero_n) {
+    k_scale_gram_body<MODE, NQ>(blockIdx.x, gridDim.x, u, lambda, Uin, Uout, prec, rows, R, gpart, zero, zero_n);
 }
 
 // The same on the FP64 matrix cores, R a multiple of 16.  A wave takes four
@@ -966,7 +1008,7 @@ __global__ __launch_bounds__(256) void k_scale_gram(const double *u,
 // written by one lane, the read first.
 
 template <int MODE, int R, typename PT>
-__global__ __launch_bounds__(256) void k_scale_gram_mfma(const double *u, const double *__restrict__ lambda,
+__device__ __forceinline__ void k_scale_gram_mfma_body(unsigned bid_, unsigned nblk_, const double *u, const double *__restrict__ lambda,
                                                          const PT *__restrict__ Uin, PT *Uout, uint64_t rows,
                                                          double *__restrict__ gpart, double *zero, uint64_t zero_n) {
     static_assert(R % 16 == 0 && R <= 64, "16-column blocks");
@@ -981,7 +1023,7 @@ __global__ __launch_bounds__(256) void k_scale_gram_mfma(const double *u, const 
     double lam[NB];
 #pragma unroll
     for (int I = 0; I < NB; ++I) lam[I] = (MODE == 0 && lambda) ? lambda[16 * I + c16] : 1.0;
-    const uint64_t nchunks = (rows + 3) / 4, stride = (uint64_t)gridDim.x * 4;
+    const uint64_t nchunks = (rows + 3) / 4, stride = (uint64_t)nblk_ * 4;
     auto load = [&](uint64_t c, RawT *raw) {
         const uint64_t row = c * 4 + kr;
 #pragma unroll
@@ -991,7 +1033,7 @@ __global__ __launch_bounds__(256) void k_scale_gram_mfma(const double *u, const 
             else raw[I] = row < rows ? Uin[gi] : (PT)0;
         }
     };
-    uint64_t c = (uint64_t)blockIdx.x * 4 + wave;
+    uint64_t c = (uint64_t)bid_ * 4 + wave;
     RawT nxt[NB];
     if (c < nchunks) load(c, nxt);
     for (; c < nchunks; c += stride) {
@@ -1033,8 +1075,8 @@ __global__ __launch_bounds__(256) void k_scale_gram_mfma(const double *u, const 
             for (int J = I; J < NB; ++J, ++p) acc[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[I], v[J], acc[p], 0, 0, 0);
     }
     if (zero)  // beyond this factor's rows: nobody reads them here
-        for (uint64_t e = rows * R + (uint64_t)blockIdx.x * 256 + threadIdx.x; e < zero_n;
-             e += (uint64_t)gridDim.x * 256)
+        for (uint64_t e = rows * R + (uint64_t)bid_ * 256 + threadIdx.x; e < zero_n;
+             e += (uint64_t)nblk_ * 256)
             zero[e] = 0.0;
     // D item q of lane l is element (l / 16 + 4 q, l % 16) of the block
     for (int w = 0; w < 4; ++w) {
@@ -1061,8 +1103,15 @@ __global__ __launch_bounds__(256) void k_scale_gram_mfma(const double *u, const 
         }
         __syncthreads();
     }
-    double *out = gpart + (size_t)blockIdx.x * RR;
+    double *out = gpart + (size_t)bid_ * RR;
     for (int e = threadIdx.x; e < RR; e += 256) out[e] = G[e];
+}
+
+template <int MODE, int R, typename PT>
+__global__ __launch_bounds__(256) void k_scale_gram_mfma(const double *u, const double *__restrict__ lambda,
+                                                         const PT *__restrict__ Uin, PT *Uout, uint64_t rows,
+                                                         double *__restrict__ gpart, double *zero, uint64_t zero_n) {
+    k_scale_gram_mfma_body<MODE, R, PT>(blockIdx.x, gridDim.x, u, lambda, Uin, Uout, rows, gpart, zero, zero_n);
 }
 
 template <int MODE, int R>
@@ -1210,12 +1259,17 @@ int launch_colsumsq(const double *U, uint64_t rows, int R, double *partials, int
     return ALTO_OK;
 }
 
-__global__ __launch_bounds__(256) void k_gram_reduce(const double *__restrict__ gpart, int nblk,
+__device__ __forceinline__ void k_gram_reduce_body(unsigned bid_, unsigned nblk_, const double *__restrict__ gpart, int nblk,
                                                      int RR, double *__restrict__ G) {
-    const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int e = bid_ * 4 + (threadIdx.x >> 6);
     if (e >= RR) return;
     double s = wave_reduce_entry(gpart, nblk, RR, e, 0);
     if ((threadIdx.x & 63) == 0) G[e] = s;
+}
+
+__global__ __launch_bounds__(256) void k_gram_reduce(const double *__restrict__ gpart, int nblk,
+                                                     int RR, double *__restrict__ G) {
+    k_gram_reduce_body(blockIdx.x, gridDim.x, gpart, nblk, RR, G);
 }
 
 int launch_gram_reduce(const double *gpart, int nblk, int R, double *G, hipStream_t s) {
@@ -1248,6 +1302,179 @@ __global__ void k_sum_final(const double *__restrict__ part, int n, double *__re
 int launch_sumsq(const double *v, uint64_t n, double *out, double *part, hipStream_t s) {
     k_sumsq_part<<<kSumsqBlocks, 256, 0, s>>>(v, n, part);
     k_sum_final<<<1, 64, 0, s>>>(part, kSumsqBlocks, out);
+    ALTO_LAUNCH_CHECK();
+    return ALTO_OK;
+}
+
+// ---- grouped launches ----
+
+bool group_rank_supported(int R) { return R == 8 || R == 16 || R == 32 || R == 64; }
+
+template <int R>
+__global__ __launch_bounds__(R == 64 ? 256 : 64) void k_inv_group(const BatchJob *__restrict__ jobs, int nmode,
+                                                                   int mode, double diag_add) {
+    const BatchJob &J = jobs[blockIdx.x];
+    if (!*J.active) return;
+    if constexpr (R == 64) k_inv_gj_body<64>(J.grams, nmode, mode, diag_add, J.vinv, J.flag, nullptr, 0, nullptr);
+    else k_inv_gj1_body<R>(J.grams, nmode, mode, diag_add, J.vinv, J.flag, nullptr, 0, nullptr);
+}
+
+int launch_inv_group(const BatchJob *jobs, int njobs, int nmode, int mode, int R, double diag_add,
+                     hipStream_t s) {
+    if (njobs <= 0) return ALTO_OK;
+    switch (R) {
+    case 8: k_inv_group<8><<<njobs, 64, 0, s>>>(jobs, nmode, mode, diag_add); break;
+    case 16: k_inv_group<16><<<njobs, 64, 0, s>>>(jobs, nmode, mode, diag_add); break;
+    case 32: k_inv_group<32><<<njobs, 64, 0, s>>>(jobs, nmode, mode, diag_add); break;
+    case 64: k_inv_group<64><<<njobs, 256, 0, s>>>(jobs, nmode, mode, diag_add); break;
+    default: return ALTO_ERR_UNSUPPORTED;
+    }
+    ALTO_LAUNCH_CHECK();
+    return ALTO_OK;
+}
+
+// launch_solve's kernel choice per rank: the register kernel at rank 8, the
+// matrix-core kernel above
+template <int R>
+__global__ __launch_bounds__(256) void k_solve_group(const BatchJob *__restrict__ jobs,
+                                                     const unsigned *__restrict__ first, int njobs) {
+    const int j = find_job(first, njobs, blockIdx.x);
+    const BatchJob &J = jobs[j];
+    if (!*J.active) return;
+    const unsigned b = blockIdx.x - first[j], nb = first[j + 1] - first[j];
+    if constexpr (R == 8) k_solve_r_body<8, double>(b, nb, J.mt, J.mt, J.rows, J.vinv, J.part);
+    else k_solve_mfma_body<R>(b, nb, J.mt, J.rows, J.vinv, J.part);
+}
+
+int launch_solve_group(const BatchJob *jobs, const unsigned *first, int njobs, unsigned nblocks, int R,
+                       hipStream_t s) {
+    if (njobs <= 0 || nblocks == 0) return ALTO_OK;
+    static std::once_flag attr;
+    std::call_once(attr, [] {
+        (void)hipFuncSetAttribute((const void *)k_solve_group<64>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024 - 256);
+    });
+    const size_t lds_mfma = sizeof(double) * ((size_t)R * R + 64 * (size_t)R);
+    switch (R) {
+    case 8: k_solve_group<8><<<nblocks, 256, 0, s>>>(jobs, first, njobs); break;
+    case 16: k_solve_group<16><<<nblocks, 256, lds_mfma, s>>>(jobs, first, njobs); break;
+    case 32: k_solve_group<32><<<nblocks, 256, lds_mfma, s>>>(jobs, first, njobs); break;
+    case 64: k_solve_group<64><<<nblocks, 256, lds_mfma, s>>>(jobs, first, njobs); break;
+    default: return ALTO_ERR_UNSUPPORTED;
+    }
+    ALTO_LAUNCH_CHECK();
+    return ALTO_OK;
+}
+
+__global__ __launch_bounds__(256) void k_lambda_group(const BatchJob *__restrict__ jobs, int per_job, int R,
+                                                      int which) {
+    const BatchJob &J = jobs[blockIdx.x / per_job];
+    if (!*J.active) return;
+    k_lambda_body(blockIdx.x % per_job, per_job, J.part, J.nb_solve, R, which, J.lam);
+}
+
+int launch_lambda_group(const BatchJob *jobs, int njobs, int R, int which, hipStream_t s) {
+    if (njobs <= 0) return ALTO_OK;
+    const int per = (int)cdiv(R, 4);
+    k_lambda_group<<<njobs * per, 256, 0, s>>>(jobs, per, R, which);
+    ALTO_LAUNCH_CHECK();
+    return ALTO_OK;
+}
+
+// launch_sg<0>'s kernel choice per rank: the LDS kernel at rank 8, the
+// matrix-core kernel above
+template <int R, typename PT>
+__global__ __launch_bounds__(256) void k_scale_gram_group(const BatchJob *__restrict__ jobs,
+                                                          const unsigned *__restrict__ first, int njobs,
+                                                          int prec, int lambda_on) {
+    const int j = find_job(first, njobs, blockIdx.x);
+    const BatchJob &J = jobs[j];
+    if (!*J.active) return;
+    const unsigned b = blockIdx.x - first[j], nb = first[j + 1] - first[j];
+    const double *lam = lambda_on ? J.lam : nullptr;
+    double *zero = lambda_on ? J.zero : nullptr;
+    const uint64_t zn = lambda_on ? J.zero_n : 0;
+    if constexpr (R == 8)
+        k_scale_gram_body<0, 1>(b, nb, J.u, lam, nullptr, J.U, prec, J.rows, 8, J.gpart, zero, zn);
+    else
+        k_scale_gram_mfma_body<0, R, PT>(b, nb, J.u, lam, nullptr, (PT *)J.U, J.rows, J.gpart, zero, zn);
+}
+
+template <int R>
+static void launch_sgg(const BatchJob *jobs, const unsigned *first, int njobs, unsigned nblocks, int prec,
+                       int lambda_on, hipStream_t s) {
+    const size_t lds = R == 8 ? sizeof(double) * 32 * R : 0;
+    if (prec == ALTO_PREC_F64)
+        k_scale_gram_group<R, double><<<nblocks, 256, lds, s>>>(jobs, first, njobs, prec, lambda_on);
+    else if (prec == ALTO_PREC_F32)
+        k_scale_gram_group<R, float><<<nblocks, 256, lds, s>>>(jobs, first, njobs, prec, lambda_on);
+    else
+        k_scale_gram_group<R, uint16_t><<<nblocks, 256, lds, s>>>(jobs, first, njobs, prec, lambda_on);
+}
+
+int launch_scale_gram_group(const BatchJob *jobs, const unsigned *first, int njobs, unsigned nblocks, int R,
+                            int prec, int lambda_on, hipStream_t s) {
+    if (njobs <= 0 || nblocks == 0) return ALTO_OK;
+    switch (R) {
+    case 8: launch_sgg<8>(jobs, first, njobs, nblocks, prec, lambda_on, s); break;
+    case 16: launch_sgg<16>(jobs, first, njobs, nblocks, prec, lambda_on, s); break;
+    case 32: launch_sgg<32>(jobs, first, njobs, nblocks, prec, lambda_on, s); break;
+    case 64: launch_sgg<64>(jobs, first, njobs, nblocks, prec, lambda_on, s); break;
+    default: return ALTO_ERR_UNSUPPORTED;
+    }
+    ALTO_LAUNCH_CHECK();
+    return ALTO_OK;
+}
+
+// Gram partials of the stored factors (launch_gram's kernels): u unused
+template <int R, typename PT>
+__global__ __launch_bounds__(256) void k_gram_group(const BatchJob *__restrict__ jobs,
+                                                    const unsigned *__restrict__ first, int njobs, int prec) {
+    const int j = find_job(first, njobs, blockIdx.x);
+    const BatchJob &J = jobs[j];
+    if (!*J.active) return;
+    const unsigned b = blockIdx.x - first[j], nb = first[j + 1] - first[j];
+    if constexpr (R == 8)
+        k_scale_gram_body<1, 1>(b, nb, nullptr, nullptr, J.U, nullptr, prec, J.rows, 8, J.gpart, nullptr, 0);
+    else
+        k_scale_gram_mfma_body<1, R, PT>(b, nb, nullptr, nullptr, (const PT *)J.U, nullptr, J.rows, J.gpart, nullptr,
+                                         0);
+}
+
+template <int R>
+static void launch_gg(const BatchJob *jobs, const unsigned *first, int njobs, unsigned nblocks, int prec,
+                      hipStream_t s) {
+    const size_t lds = R == 8 ? sizeof(double) * 32 * R : 0;
+    if (prec == ALTO_PREC_F64) k_gram_group<R, double><<<nblocks, 256, lds, s>>>(jobs, first, njobs, prec);
+    else if (prec == ALTO_PREC_F32) k_gram_group<R, float><<<nblocks, 256, lds, s>>>(jobs, first, njobs, prec);
+    else k_gram_group<R, uint16_t><<<nblocks, 256, lds, s>>>(jobs, first, njobs, prec);
+}
+
+int launch_gram_group(const BatchJob *jobs, const unsigned *first, int njobs, unsigned nblocks, int R, int prec,
+                      hipStream_t s) {
+    if (njobs <= 0 || nblocks == 0) return ALTO_OK;
+    switch (R) {
+    case 8: launch_gg<8>(jobs, first, njobs, nblocks, prec, s); break;
+    case 16: launch_gg<16>(jobs, first, njobs, nblocks, prec, s); break;
+    case 32: launch_gg<32>(jobs, first, njobs, nblocks, prec, s); break;
+    case 64: launch_gg<64>(jobs, first, njobs, nblocks, prec, s); break;
+    default: return ALTO_ERR_UNSUPPORTED;
+    }
+    ALTO_LAUNCH_CHECK();
+    return ALTO_OK;
+}
+
+__global__ __launch_bounds__(256) void k_gram_reduce_group(const BatchJob *__restrict__ jobs, int per_job,
+                                                           int RR) {
+    const BatchJob &J = jobs[blockIdx.x / per_job];
+    if (!*J.active) return;
+    k_gram_reduce_body(blockIdx.x % per_job, per_job, J.gpart, J.nb_gram, RR, J.G);
+}
+
+int launch_gram_reduce_group(const BatchJob *jobs, int njobs, int R, hipStream_t s) {
+    if (njobs <= 0) return ALTO_OK;
+    const int RR = R * R, per = (int)cdiv(RR, 4);
+    k_gram_reduce_group<<<njobs * per, 256, 0, s>>>(jobs, per, RR);
     ALTO_LAUNCH_CHECK();
     return ALTO_OK;
 }

@@ -96,6 +96,55 @@ int launch_scale_gram(const double *u, const double *lambda, uint64_t rows, int 
 int launch_gram(const void *U, int prec, uint64_t rows, int R, double *gpart, int nblk,
                 hipStream_t s);
 int launch_gram_reduce(const double *gpart, int nblk, int R, double *G, hipStream_t s);
+// ---- grouped launches: one launch runs a step for many CP-ALS instances ----
+// One instance's buffers for the dense steps of one mode update.
+struct BatchJob {
+    const double *u;      // rows x R FP64 input of the scale/Gram step (mt, or the initial factor)
+    const double *grams;  // the instance's nmode Gram matrices
+    double *G;            // Gram output: grams + mode R x R
+    double *vinv;         // R x R
+    int *flag;
+    double *mt;           // MTTKRP output, solved in place, rows x R FP64
+    uint64_t rows;
+    double *part;         // solve partials, nb_solve x 3R
+    double *lam;          // R
+    void *U;              // factor storage (the instance's precision)
+    double *gpart;        // Gram partials, nb_gram x R x R
+    double *zero;         // zeroed in [0, zero_n) by the scale step (the next MTTKRP's output)
+    uint64_t zero_n;
+    const int *active;    // 0: the instance has stopped; every grouped kernel skips it
+    int nb_solve, nb_gram;
+};
+
+// One instance's fit and stopping state in a grouped CP-ALS.
+struct FitJob {
+    const double *grams, *lam, *part, *normX;
+    double *fitd;  // (fit, exact fit) per iteration
+    int *cnt;      // iterations done
+    int *active;
+    int nblk;      // solve partials of the last mode
+    int max_iters;
+    double epsilon;
+};
+
+// Grouped forms of launch_chol_inv (diag_add, no phi/rho), launch_solve,
+// launch_lambda, launch_scale_gram (lambda_on 0: no scaling, no zeroing; u
+// from the job) and launch_gram_reduce, for R in {8, 16, 32, 64}.  jobs and
+// first are device arrays; first has njobs + 1 entries (prefix sums of
+// nb_solve for the solve, of nb_gram for the scale step).
+int launch_inv_group(const BatchJob *jobs, int njobs, int nmode, int mode, int R, double diag_add,
+                     hipStream_t s);
+int launch_solve_group(const BatchJob *jobs, const unsigned *first, int njobs, unsigned nblocks, int R,
+                       hipStream_t s);
+int launch_lambda_group(const BatchJob *jobs, int njobs, int R, int which, hipStream_t s);
+int launch_scale_gram_group(const BatchJob *jobs, const unsigned *first, int njobs, unsigned nblocks, int R,
+                            int prec, int lambda_on, hipStream_t s);
+int launch_gram_reduce_group(const BatchJob *jobs, int njobs, int R, hipStream_t s);
+// Gram partials of the stored factors U (launch_gram), first: prefix sums of nb_gram.
+int launch_gram_group(const BatchJob *jobs, const unsigned *first, int njobs, unsigned nblocks, int R, int prec,
+                      hipStream_t s);
+bool group_rank_supported(int R);
+
 constexpr int kSumsqBlocks = 1024;
 // out = sum of v^2 (fixed order); part: kSumsqBlocks doubles of scratch.
 int launch_sumsq(const double *v, uint64_t n, double *out, double *part, hipStream_t s);

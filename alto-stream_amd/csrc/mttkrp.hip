@@ -370,11 +370,11 @@ __device__ __forceinline__ void block_combine(const uint32_t (*s_row)[G], const 
 }
 
 template <typename VT, typename FT, typename OT, int R, int PKW, int NM1, int VPL, bool ZEROED>
-__global__ __launch_bounds__(256) void k_mttkrp_dec(const uint64_t *__restrict__ pkeys,
-                                                    const VT *__restrict__ vals, uint64_t nnz,
-                                                    FPtrs<FT> F, OT *__restrict__ out,
-                                                    PackInfo pk, int target, uint64_t per_group,
-                                                    int flags) {
+__device__ __forceinline__ void k_mttkrp_dec_body(unsigned lb, const uint64_t *__restrict__ pkeys,
+                                                  const VT *__restrict__ vals, uint64_t nnz,
+                                                  const FPtrs<FT> &F, OT *__restrict__ out,
+                                                  const PackInfo &pk, int target, uint64_t per_group,
+                                                  int flags) {
     using CT = typename FVec<FT>::CT;
     constexpr int L = R / VPL;
     constexpr int G = 256 / L;
@@ -387,7 +387,6 @@ __global__ __launch_bounds__(256) void k_mttkrp_dec(const uint64_t *__restrict__
     __shared__ uint32_t s_row[2][G];
     constexpr uint32_t NONE = 0xFFFFFFFFu;
 
-    const unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
     const uint64_t gtid = (uint64_t)lb * blockDim.x + threadIdx.x;
     const uint64_t group = gtid / L;
     const int gib = threadIdx.x / L;
@@ -546,10 +545,16 @@ __global__ __launch_bounds__(256) void k_mttkrp_dec(const uint64_t *__restrict__
 // ends, so gathers per nonzero drop from NM1 to (#fibers of all levels) /
 // nnz.  Range boundaries end every fiber.  Row flushing and the boundary
 // combine are those of k_mttkrp_dec in exclusive mode.
-struct FibInfo {
-    int mode[4];                  // mode of level q + 1, q = 0 .. NM1 - 1
-    uint64_t mlo[5], mhi[5];      // packed-key bits of level 0 (target) .. NM1
-};
+
+template <typename VT, typename FT, typename OT, int R, int PKW, int NM1, int VPL, bool ZEROED>
+__global__ __launch_bounds__(256) void k_mttkrp_dec(const uint64_t *__restrict__ pkeys,
+                                                    const VT *__restrict__ vals, uint64_t nnz,
+                                                    FPtrs<FT> F, OT *__restrict__ out,
+                                                    PackInfo pk, int target, uint64_t per_group,
+                                                    int flags) {
+    k_mttkrp_dec_body<VT, FT, OT, R, PKW, NM1, VPL, ZEROED>(xcd_remap(blockIdx.x, gridDim.x), pkeys, vals, nnz, F,
+                                                             out, pk, target, per_group, flags);
+}
 
 template <int PKW, int NM1>
 __device__ __forceinline__ uint32_t fib_depth(uint64_t alo, uint64_t ahi, uint64_t blo,
@@ -566,11 +571,11 @@ __device__ __forceinline__ uint32_t fib_depth(uint64_t alo, uint64_t ahi, uint64
 }
 
 template <typename VT, typename FT, typename OT, int R, int PKW, int NM1, int VPL, int CB, bool ZEROED>
-__global__ __launch_bounds__(256) void k_mttkrp_fib(const uint64_t *__restrict__ pkeys,
-                                                    const VT *__restrict__ vals, uint64_t nnz,
-                                                    FPtrs<FT> F, OT *__restrict__ out,
-                                                    PackInfo pk, int target, FibInfo fi,
-                                                    uint64_t per_group, int flags) {
+__device__ __forceinline__ void k_mttkrp_fib_body(unsigned lb, const uint64_t *__restrict__ pkeys,
+                                                  const VT *__restrict__ vals, uint64_t nnz,
+                                                  const FPtrs<FT> &F, OT *__restrict__ out,
+                                                  const PackInfo &pk, int target, const FibInfo &fi,
+                                                  uint64_t per_group, int flags) {
     using CT = typename FVec<FT>::CT;
     constexpr int L = R / VPL;
     constexpr int G = 256 / L;
@@ -582,7 +587,6 @@ __global__ __launch_bounds__(256) void k_mttkrp_fib(const uint64_t *__restrict__
     __shared__ uint32_t s_row[2][G];
     constexpr uint32_t NONE = 0xFFFFFFFFu;
 
-    const unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
     const uint64_t gtid = (uint64_t)lb * blockDim.x + threadIdx.x;
     const uint64_t group = gtid / L;
     const int gib = threadIdx.x / L;
@@ -720,6 +724,16 @@ __global__ __launch_bounds__(256) void k_mttkrp_fib(const uint64_t *__restrict__
     }
     __syncthreads();
     block_combine<OT, CT, R, G>(s_row, s_part, out, zeroed);
+}
+
+template <typename VT, typename FT, typename OT, int R, int PKW, int NM1, int VPL, int CB, bool ZEROED>
+__global__ __launch_bounds__(256) void k_mttkrp_fib(const uint64_t *__restrict__ pkeys,
+                                                    const VT *__restrict__ vals, uint64_t nnz,
+                                                    FPtrs<FT> F, OT *__restrict__ out,
+                                                    PackInfo pk, int target, FibInfo fi,
+                                                    uint64_t per_group, int flags) {
+    k_mttkrp_fib_body<VT, FT, OT, R, PKW, NM1, VPL, CB, ZEROED>(xcd_remap(blockIdx.x, gridDim.x), pkeys, vals,
+                                                                 nnz, F, out, pk, target, fi, per_group, flags);
 }
 
 // elements per lane of the decode kernel: 8 FP32 / BF16, 4 FP64 (<= R)
@@ -1045,18 +1059,14 @@ int prepare_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *co
     return run_mttkrp(t, target, prec, rank, factors, nullptr, out_f64, 2, 0, -1, s, true, true);
 }
 
-int run_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const *factors,
-               void *out, int out_f64, int algo, uint64_t per_group, int flags, hipStream_t s,
-               bool prepare_only, bool out_zeroed) {
-    if (target < 0 || target >= t->nmode || rank < 1 || rank > 4096 || !factors ||
-        (!out && !prepare_only))
-        return ALTO_ERR_ARG;
-    if (prec != ALTO_PREC_F64 && prec != ALTO_PREC_F32 && prec != ALTO_PREC_BF16) return ALTO_ERR_ARG;
-    for (int m = 0; m < t->nmode; ++m)
-        if (m != target && !factors[m]) return ALTO_ERR_ARG;
-    if (t->nnz == 0) return ALTO_OK;
+// The launch arguments of an MTTKRP (schedule built on demand); *empty when
+// there is nothing to compute; *defaults when per_group and flags are the
+// defaults (the tuner may then replace them).
+static int make_args(TensorImpl *t, int target, int prec, int rank, const void *const *factors, void *out,
+                     int out_f64, int algo, uint64_t per_group, int flags, bool out_zeroed, MttkrpArgs &a,
+                     bool *defaults_out, bool *empty, hipStream_t s) {
+    *empty = false;
     if (algo == 0) algo = 2;
-    MttkrpArgs a{};
     a.nnz = t->nnz;
     a.factors = factors;
     a.out = out;
@@ -1089,7 +1099,10 @@ int run_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const 
         ALTO_TRY(ensure_schedule(t, target, prec == ALTO_PREC_F64, s));
         const ModeSchedule &sc = t->sched[target];
         a.nnz = sc.n;
-        if (a.nnz == 0) return ALTO_OK;
+        if (a.nnz == 0) {
+            *empty = true;
+            return ALTO_OK;
+        }
         a.pkeys = sc.pkeys.as<uint64_t>();
         a.vals = prec == ALTO_PREC_F64 ? sc.vals64.p : sc.vals32.p;
         // fiber kernel: FP64 and FP32 factors by default, BF16 only on request
@@ -1109,7 +1122,7 @@ int run_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const 
     } else {
         return ALTO_ERR_ARG;
     }
-    const bool defaults = a.per_group == 0 && flags < 0;
+    *defaults_out = a.per_group == 0 && flags < 0;
     if (a.per_group == 0) {
         // 64 entries per lane group, fewer when that would leave the device
         // short of about 2^18 threads (small tensors are latency bound: a
@@ -1120,6 +1133,25 @@ int run_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const 
         uint64_t pg = a.nnz * lanes / (1ULL << 18);
         a.per_group = pg < 4 ? 4 : pg > 64 ? 64 : pg;
     }
+    return ALTO_OK;
+}
+
+int run_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const *factors,
+               void *out, int out_f64, int algo, uint64_t per_group, int flags, hipStream_t s,
+               bool prepare_only, bool out_zeroed) {
+    if (target < 0 || target >= t->nmode || rank < 1 || rank > 4096 || !factors ||
+        (!out && !prepare_only))
+        return ALTO_ERR_ARG;
+    if (prec != ALTO_PREC_F64 && prec != ALTO_PREC_F32 && prec != ALTO_PREC_BF16) return ALTO_ERR_ARG;
+    for (int m = 0; m < t->nmode; ++m)
+        if (m != target && !factors[m]) return ALTO_ERR_ARG;
+    if (t->nnz == 0) return ALTO_OK;
+    if (algo == 0) algo = 2;
+    MttkrpArgs a{};
+    bool defaults = false, empty = false;
+    ALTO_TRY(make_args(t, target, prec, rank, factors, out, out_f64, algo, per_group, flags, out_zeroed, a,
+                       &defaults, &empty, s));
+    if (empty) return ALTO_OK;
     // flags bit 7: the hot-row kernel, workgroup shape in bits 9-10
     if (algo == 2 && (a.flags & 128)) {
         if (!hot_supported(t, prec, rank)) {
@@ -1140,6 +1172,118 @@ int run_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *const 
         return ALTO_OK;
     }
     int st = launch_cfg(t, a, s);
+    ALTO_TRY(st);
+    ALTO_LAUNCH_CHECK();
+    return ALTO_OK;
+}
+
+// ---- grouped MTTKRP: one launch over many tensors' output-ordered schedules ----
+
+// The fiber kernel of MttkrpJob j runs on grouped blocks [first[j], first[j+1]).
+// DEC: the decode-once kernel, else the fiber kernel.
+template <typename VT, typename FT, int R, int NM1, bool DEC>
+__global__ __launch_bounds__(256) void k_mttkrp_group(const MttkrpJob *__restrict__ jobs,
+                                                      const unsigned *__restrict__ first, int njobs) {
+    constexpr int VPL = dec_vpl<FT, R>();
+    const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
+    const int j = find_job(first, njobs, b);
+    // The job stays in global memory: the factor pointers and the key
+    // fields are read once at the start, by mode index; the fiber levels are
+    // copied to registers because the kernel reads them for every entry
+    // (through a global-memory reference it would reload them after every
+    // output store it cannot prove unrelated).
+    const MttkrpJob &J = jobs[j];
+    if (!*J.active) return;
+    const FPtrs<FT> &F = *reinterpret_cast<const FPtrs<FT> *>(J.F);
+    if constexpr (DEC) {
+        k_mttkrp_dec_body<VT, FT, double, R, 1, NM1, VPL, true>(b - first[j], J.pkeys, (const VT *)J.vals, J.nnz, F,
+                                                                (double *)J.out, J.pk, J.target, J.per_group,
+                                                                J.flags);
+    } else {
+        const FibInfo fi = J.fi;
+        k_mttkrp_fib_body<VT, FT, double, R, 1, NM1, VPL, 1, true>(b - first[j], J.pkeys, (const VT *)J.vals,
+                                                                   J.nnz, F, (double *)J.out, J.pk, J.target, fi,
+                                                                   J.per_group, J.flags);
+    }
+}
+
+bool mttkrp_group_supported(const TensorImpl *t, int prec, int rank) {
+    if (t->pkw != 1 || (t->nmode != 3 && t->nmode != 4)) return false;
+    if (prec != ALTO_PREC_F64 && prec != ALTO_PREC_F32) return false;
+    if (rank != 8 && rank != 16 && rank != 32 && rank != 64) return false;
+    for (int m = 0; m < t->nmode; ++m)
+        if (t->dims[m] * (uint64_t)rank * (prec == ALTO_PREC_F64 ? 8 : 4) >= (1ULL << 32)) return false;
+    return true;
+}
+
+int mttkrp_group_job(TensorImpl *t, int target, int prec, int rank, const void *const *factors, void *out,
+                     const int *active, MttkrpJob *job, hipStream_t s) {
+    if (!mttkrp_group_supported(t, prec, rank)) return ALTO_ERR_UNSUPPORTED;
+    MttkrpArgs a{};
+    bool defaults = false, empty = false;
+    ALTO_TRY(make_args(t, target, prec, rank, factors, out, 1, 2, 0, -1, true, a, &defaults, &empty, s));
+    *job = MttkrpJob{};
+    job->active = active;
+    job->out = out;
+    job->nblocks = 0;
+    if (empty || t->nnz == 0) return ALTO_OK;
+    if (!a.decode_once) return ALTO_ERR_UNSUPPORTED;
+    // fiber kernel when a (target, first level) fiber holds about two or more
+    // entries on average (uniform estimate), the decode-once kernel otherwise
+    const double cells = (double)t->dims[target] * (double)t->dims[a.fi.mode[0]];
+    job->dec = (double)a.nnz < 2.0 * cells ? 1 : 0;
+    job->pkeys = a.pkeys;
+    job->vals = a.vals;
+    job->nnz = a.nnz;
+    for (int m = 0; m < t->nmode; ++m) job->F[m] = factors[m];
+    job->pk = a.pk;
+    job->target = target;
+    job->fi = a.fi;
+    // In a grouped launch the tensors together fill the device, so each takes
+    // the segment length that is fastest per entry (8 entries per lane group
+    // measured best from rank 8 to 32 at 5e5 to 5e6 entries), not the one
+    // that fills the device with one tensor.
+    job->per_group = 8;
+    job->flags = a.flags;
+    const int vpl = prec == ALTO_PREC_F64 ? (rank < 4 ? rank : 4) : (rank < 8 ? rank : 8);
+    const uint64_t groups = (a.nnz + job->per_group - 1) / job->per_group;
+    job->nblocks = cdiv(groups * (uint64_t)(rank / vpl), 256);
+    return ALTO_OK;
+}
+
+template <typename VT, typename FT, int R, bool DEC>
+static void launch_group_r(const MttkrpJob *jobs, const unsigned *first, int njobs, unsigned nblocks, int nmode,
+                           hipStream_t s) {
+    if (nmode == 3) k_mttkrp_group<VT, FT, R, 2, DEC><<<nblocks, 256, 0, s>>>(jobs, first, njobs);
+    else k_mttkrp_group<VT, FT, R, 3, DEC><<<nblocks, 256, 0, s>>>(jobs, first, njobs);
+}
+
+template <typename VT, typename FT, bool DEC>
+static int launch_group_p(const MttkrpJob *jobs, const unsigned *first, int njobs, unsigned nblocks, int rank,
+                          int nmode, hipStream_t s) {
+    switch (rank) {
+    case 8: launch_group_r<VT, FT, 8, DEC>(jobs, first, njobs, nblocks, nmode, s); break;
+    case 16: launch_group_r<VT, FT, 16, DEC>(jobs, first, njobs, nblocks, nmode, s); break;
+    case 32: launch_group_r<VT, FT, 32, DEC>(jobs, first, njobs, nblocks, nmode, s); break;
+    case 64: launch_group_r<VT, FT, 64, DEC>(jobs, first, njobs, nblocks, nmode, s); break;
+    default: return ALTO_ERR_UNSUPPORTED;
+    }
+    return ALTO_OK;
+}
+
+int launch_mttkrp_group(const MttkrpJob *jobs, const unsigned *first, int njobs, unsigned nblocks, int prec,
+                        int rank, int nmode, bool dec, hipStream_t s) {
+    if (njobs <= 0 || nblocks == 0) return ALTO_OK;
+    if (nmode != 3 && nmode != 4) return ALTO_ERR_UNSUPPORTED;
+    int st;
+    if (prec == ALTO_PREC_F64)
+        st = dec ? launch_group_p<double, double, true>(jobs, first, njobs, nblocks, rank, nmode, s)
+                 : launch_group_p<double, double, false>(jobs, first, njobs, nblocks, rank, nmode, s);
+    else if (prec == ALTO_PREC_F32)
+        st = dec ? launch_group_p<float, float, true>(jobs, first, njobs, nblocks, rank, nmode, s)
+                 : launch_group_p<float, float, false>(jobs, first, njobs, nblocks, rank, nmode, s);
+    else
+        st = ALTO_ERR_UNSUPPORTED;
     ALTO_TRY(st);
     ALTO_LAUNCH_CHECK();
     return ALTO_OK;

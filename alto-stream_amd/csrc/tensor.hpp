@@ -114,6 +114,43 @@ int prepare_mttkrp(TensorImpl *t, int target, int prec, int rank, const void *co
                    int out_f64, hipStream_t s);
 void fiber_levels(const TensorImpl *t, int mode, int *lvl);
 
+// Fiber levels of an output-ordered schedule: the mode of each level after
+// the target and the packed-key bits of every level.
+struct FibInfo {
+    int mode[4];              // mode of level q + 1, q = 0 .. NM1 - 1
+    uint64_t mlo[5], mhi[5];  // packed-key bits of level 0 (target) .. NM1
+};
+
+// One tensor's part of a grouped MTTKRP: the fiber kernel over its
+// output-ordered schedule of `target` into FP64 `out` (zero on entry), with
+// the default segment length; skipped while *active is 0.
+struct MttkrpJob {
+    const uint64_t *pkeys;
+    const void *vals;
+    uint64_t nnz;
+    const void *F[kMaxModes];
+    void *out;
+    PackInfo pk;
+    int target;
+    FibInfo fi;
+    uint64_t per_group;
+    int flags;
+    const int *active;
+    unsigned nblocks;  // workgroups of this job
+    int dec;           // 1: decode-once kernel, 0: fiber kernel
+};
+// 3 or 4 modes, 64-bit keys, FP32 or FP64 factors, rank 8, 16, 32 or 64.
+bool mttkrp_group_supported(const TensorImpl *t, int prec, int rank);
+// Fills *job (building the schedule); ALTO_ERR_UNSUPPORTED outside the
+// supported set.
+int mttkrp_group_job(TensorImpl *t, int target, int prec, int rank, const void *const *factors, void *out,
+                     const int *active, MttkrpJob *job, hipStream_t s);
+// jobs and first (njobs + 1 prefix sums of nblocks) in device memory; all
+// jobs share precision, rank and mode count, and take the decode-once kernel
+// (dec) or the fiber kernel.
+int launch_mttkrp_group(const MttkrpJob *jobs, const unsigned *first, int njobs, unsigned nblocks, int prec,
+                        int rank, int nmode, bool dec, hipStream_t s);
+
 // Hot-row MTTKRP (one workgroup per CU, popular rows in LDS): out += MTTKRP
 // of `target` over its built schedule.  Ranks 16, 32, 64; 3 to 5 modes.
 // shape: threads per workgroup / entries per gather batch (0: 1024 / 1,
@@ -123,6 +160,17 @@ int ensure_hot_modes(TensorImpl *t, hipStream_t s);
 int ensure_hot_schedule(TensorImpl *t, int target, int rb, hipStream_t s);
 int run_hot(TensorImpl *t, int target, int prec, int rank, const void *const *factors, void *out,
             bool out64, bool zeroed, int shape, hipStream_t s);
+
+
+// Grouped CP-ALS (als_batch.hip): tensors that qualify (3 or 4 modes, 64-bit
+// keys, rank 8, 16, 32 or 64, FP32 or FP64) run class by class, every ALS
+// step one launch per class.  classes: lists of tensor indices sharing mode
+// count, rank, precision and fit convention.
+bool als_group_eligible(const TensorImpl *t, const alto_hip_als_opts *o);
+int als_groups_run(const std::vector<std::vector<int>> &classes, const alto_hip_tensor *tensors,
+                   const alto_hip_als_opts *opts, double *const *const *factors, double *const *lambdas,
+                   double *const *fits, int *iters);
+bool grouped_als_enabled();
 
 }  // namespace alto
 
