@@ -41,6 +41,13 @@ HBM_PEAK_GBS = 8000.0
 EQUAL_FIT_TOL = 1e-5
 
 
+def _json_scalar(x):
+    """numpy scalars (np.bool_, np.float64, np.int64) as plain JSON values."""
+    if isinstance(x, np.generic):
+        return x.item()
+    raise TypeError("not JSON serializable: %r" % type(x))
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -103,7 +110,7 @@ def main():
         legs = {"plain_als": lambda: leg_plain_als(A), "bf16_5mode": lambda: leg_bf16_5mode(A),
                 "cp_stream": lambda: leg_cp_stream(A, args.stream_slices),
                 "batched_als": lambda: leg_batched_als(A, args.batch_tensors)}
-        print(json.dumps(legs[args.leg]()), flush=True)
+        print(json.dumps(legs[args.leg](), default=_json_scalar), flush=True)
         return
     if args.probe:
         # 128-byte row gathers: uniform rows over growing tables (L2, MALL, HBM),
@@ -253,7 +260,7 @@ def main():
             als["speedup_%s_vs_f64" % name] = round(als["f64"]["s_per_iter"] / als[name]["s_per_iter"], 3)
             if normX2_merged is not None:
                 als["dfit_exact_%s" % name] = abs(als[name]["fit_exact"] - als["f64"]["fit_exact"])
-                als["%s_at_equal_fit" % name] = als["dfit_exact_%s" % name] <= EQUAL_FIT_TOL
+                als["%s_at_equal_fit" % name] = bool(als["dfit_exact_%s" % name] <= EQUAL_FIT_TOL)
             als["lambda_rel_err_%s" % name] = float(np.linalg.norm(lam - lam64) / np.linalg.norm(lam64))
             als["factor_rel_err_%s" % name] = float(max(np.linalg.norm(a - b) / np.linalg.norm(b)
                                                         for a, b in zip(fs, fs64)))
@@ -272,7 +279,7 @@ def main():
         result["cpu_baseline"] = cpu_baseline(hkeys, hvals, dims, R, F64, kw, nnz)
         if not args.no_extra:
             result["other_workloads"]["plain_cp_als_1000x800x600"].update(plain_als_cpu())
-    print(json.dumps(result), flush=True)
+    print(json.dumps(result, default=_json_scalar), flush=True)
 
 
 def host_nonzeros(A, at, count):
@@ -327,7 +334,7 @@ def exact_fit(A, at, fs, lam, normX2):
     o.close()
     ip = float(np.sum((K * fs[-1]) @ lam))
     res2 = max(normX2 - 2.0 * ip + nM2, 0.0)
-    return 1.0 - np.sqrt(res2) / np.sqrt(normX2)
+    return float(1.0 - np.sqrt(res2) / np.sqrt(normX2))
 
 
 def measured_traffic(workload, prec):
