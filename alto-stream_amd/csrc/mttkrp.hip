@@ -293,6 +293,16 @@ template <typename FT, int VPL> struct RowLoad {
 };
 
 
+// FP64 boundary slots of an FP32-output MTTKRP: workgroup lb writes the sum
+// of its first row to slot 2 lb and of its last row to slot 2 lb + 1 (a
+// workgroup of one row repeats it with zero), in schedule order, and
+// k_bnd_slots sums the runs of equal rows in slot order in FP64.  With
+// row == nullptr the boundary rows take hardware atomics in the output type.
+struct BndSlots {
+    uint32_t *row;
+    double *val;
+};
+
 // Combines one workgroup's range partials: slot e = 0 .. 2G-1 holds group
 // e >> 1's first (e even) or last (e odd) row partial, in output order, with
 // row 0xFFFFFFFF for an empty slot.  Consecutive equal rows are summed; the
@@ -305,7 +315,7 @@ template <typename FT, int VPL> struct RowLoad {
 // runs in order.  Requires a 256-thread workgroup.
 template <typename OT, typename CT, int R, int G>
 __device__ __forceinline__ void block_combine(const uint32_t (*s_row)[G], const CT (*s_part)[G][R], OT *out,
-                                              bool zeroed) {
+                                              bool zeroed, BndSlots bs, unsigned lb) {
     constexpr uint32_t NONE = 0xFFFFFFFFu;
     constexpr int P = 256 / R, S = 2 * G, C = S / P;
     static_assert(256 % R == 0 && S % P == 0, "combine tiling");
@@ -347,8 +357,8 @@ __device__ __forceinline__ void block_combine(const uint32_t (*s_row)[G], const 
     m_sum[1][col][j] = csum;
     __syncthreads();
     if (j != 0) return;
-    uint32_t mrow = NONE;
-    CT a = 0;
+    uint32_t mrow = NONE, r0 = NONE;
+    CT a = 0, a0 = 0;
     bool first = true;
     for (int k = 0; k < 2 * P; ++k) {
         const uint32_t row = m_row[k & 1][col][k >> 1];
@@ -358,15 +368,56 @@ __device__ __forceinline__ void block_combine(const uint32_t (*s_row)[G], const 
             a += x;
         } else {
             if (mrow != NONE) {
-                if (first) atomic_add_out(o + (uint64_t)mrow * R, (OT)a);
-                else flush_owned(o + (uint64_t)mrow * R, a, zeroed);
+                if (first) {
+                    if (bs.row) {
+                        r0 = mrow;
+                        a0 = a;
+                    } else {
+                        atomic_add_out(o + (uint64_t)mrow * R, (OT)a);
+                    }
+                } else {
+                    flush_owned(o + (uint64_t)mrow * R, a, zeroed);
+                }
                 first = false;
             }
             mrow = row;
             a = x;
         }
     }
-    if (mrow != NONE) atomic_add_out(o + (uint64_t)mrow * R, (OT)a);
+    if (bs.row) {
+        if (first) {  // one row in the whole workgroup
+            r0 = mrow;
+            a0 = a;
+            a = 0;
+        }
+        if (col == 0) {
+            bs.row[2 * (uint64_t)lb] = r0;
+            bs.row[2 * (uint64_t)lb + 1] = mrow;
+        }
+        bs.val[2 * (uint64_t)lb * R + col] = (double)a0;
+        bs.val[(2 * (uint64_t)lb + 1) * R + col] = (double)a;
+    } else if (mrow != NONE) {
+        atomic_add_out(o + (uint64_t)mrow * R, (OT)a);
+    }
+}
+
+// Sums the FP64 boundary slots (rows non-decreasing in slot order): the
+// thread of (slot, column) that starts a run of equal rows adds the run in
+// slot order and writes the row (stored when the output held zeros).
+template <typename OT, int R>
+__global__ __launch_bounds__(256) void k_bnd_slots(const uint32_t *__restrict__ rows,
+                                                   const double *__restrict__ vals, uint64_t nslots,
+                                                   OT *__restrict__ out, int zeroed) {
+    const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const uint64_t i = t / R;
+    const int col = (int)(t % R);
+    if (i >= nslots) return;
+    const uint32_t row = rows[i];
+    if (row == 0xFFFFFFFFu || (i > 0 && rows[i - 1] == row)) return;
+    double sum = 0;
+    for (uint64_t j = i; j < nslots && rows[j] == row; ++j) sum += vals[j * R + col];
+    OT *p = out + (uint64_t)row * R + col;
+    *p = zeroed ? (OT)sum : (OT)((double)*p + sum);
 }
 
 template <typename VT, typename FT, typename OT, int R, int PKW, int NM1, int VPL, bool ZEROED>
@@ -374,7 +425,7 @@ __device__ __forceinline__ void k_mttkrp_dec_body(unsigned lb, const uint64_t *_
                                                   const VT *__restrict__ vals, uint64_t nnz,
                                                   const FPtrs<FT> &F, OT *__restrict__ out,
                                                   const PackInfo &pk, int target, uint64_t per_group,
-                                                  int flags) {
+                                                  int flags, BndSlots bs) {
     using CT = typename FVec<FT>::CT;
     constexpr int L = R / VPL;
     constexpr int G = 256 / L;
@@ -528,7 +579,7 @@ __device__ __forceinline__ void k_mttkrp_dec_body(unsigned lb, const uint64_t *_
         s_row[1][gib] = cur;
     }
     __syncthreads();
-    block_combine<OT, CT, R, G>(s_row, s_part, out, zeroed);
+    block_combine<OT, CT, R, G>(s_row, s_part, out, zeroed, bs, lb);
 }
 
 // Fiber variant of k_mttkrp_dec for output-ordered schedules whose rows are
@@ -551,9 +602,9 @@ __global__ __launch_bounds__(256) void k_mttkrp_dec(const uint64_t *__restrict__
                                                     const VT *__restrict__ vals, uint64_t nnz,
                                                     FPtrs<FT> F, OT *__restrict__ out,
                                                     PackInfo pk, int target, uint64_t per_group,
-                                                    int flags) {
+                                                    int flags, BndSlots bs) {
     k_mttkrp_dec_body<VT, FT, OT, R, PKW, NM1, VPL, ZEROED>(xcd_remap(blockIdx.x, gridDim.x), pkeys, vals, nnz, F,
-                                                             out, pk, target, per_group, flags);
+                                                             out, pk, target, per_group, flags, bs);
 }
 
 template <int PKW, int NM1>
@@ -575,7 +626,7 @@ __device__ __forceinline__ void k_mttkrp_fib_body(unsigned lb, const uint64_t *_
                                                   const VT *__restrict__ vals, uint64_t nnz,
                                                   const FPtrs<FT> &F, OT *__restrict__ out,
                                                   const PackInfo &pk, int target, const FibInfo &fi,
-                                                  uint64_t per_group, int flags) {
+                                                  uint64_t per_group, int flags, BndSlots bs) {
     using CT = typename FVec<FT>::CT;
     constexpr int L = R / VPL;
     constexpr int G = 256 / L;
@@ -723,7 +774,7 @@ __device__ __forceinline__ void k_mttkrp_fib_body(unsigned lb, const uint64_t *_
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
-    block_combine<OT, CT, R, G>(s_row, s_part, out, zeroed);
+    block_combine<OT, CT, R, G>(s_row, s_part, out, zeroed, bs, lb);
 }
 
 template <typename VT, typename FT, typename OT, int R, int PKW, int NM1, int VPL, int CB, bool ZEROED>
@@ -731,9 +782,9 @@ __global__ __launch_bounds__(256) void k_mttkrp_fib(const uint64_t *__restrict__
                                                     const VT *__restrict__ vals, uint64_t nnz,
                                                     FPtrs<FT> F, OT *__restrict__ out,
                                                     PackInfo pk, int target, FibInfo fi,
-                                                    uint64_t per_group, int flags) {
+                                                    uint64_t per_group, int flags, BndSlots bs) {
     k_mttkrp_fib_body<VT, FT, OT, R, PKW, NM1, VPL, CB, ZEROED>(xcd_remap(blockIdx.x, gridDim.x), pkeys, vals,
-                                                                 nnz, F, out, pk, target, fi, per_group, flags);
+                                                                 nnz, F, out, pk, target, fi, per_group, flags, bs);
 }
 
 // elements per lane of the decode kernel: 8 FP32 / BF16, 4 FP64 (<= R)
@@ -807,6 +858,7 @@ struct MttkrpArgs {
     bool hot;       // hot-row kernel (run_hot)
     int hot_shape;  // its workgroup shape
     FibInfo fi;
+    BndSlots bnd;   // FP64 boundary slots (FP32 output, output-ordered schedule)
 };
 
 template <typename VT, typename FT, typename OT, int R, int PKW, int VPL, int CB, bool Z>
@@ -818,15 +870,15 @@ static void launch_fib_z(const MttkrpArgs &a, const FPtrs<FT> &F, hipStream_t s)
     if (nm1 == 2)
         k_mttkrp_fib<VT, FT, OT, R, PKW, 2, VPL, CB, Z><<<blocks, 256, 0, s>>>(
             a.pkeys, (const VT *)a.vals, a.nnz, F, (OT *)a.out, a.pk, a.target, a.fi, a.per_group,
-            a.flags);
+            a.flags, a.bnd);
     else if (nm1 == 3)
         k_mttkrp_fib<VT, FT, OT, R, PKW, 3, VPL, CB, Z><<<blocks, 256, 0, s>>>(
             a.pkeys, (const VT *)a.vals, a.nnz, F, (OT *)a.out, a.pk, a.target, a.fi, a.per_group,
-            a.flags);
+            a.flags, a.bnd);
     else
         k_mttkrp_fib<VT, FT, OT, R, PKW, 4, VPL, CB, Z><<<blocks, 256, 0, s>>>(
             a.pkeys, (const VT *)a.vals, a.nnz, F, (OT *)a.out, a.pk, a.target, a.fi, a.per_group,
-            a.flags);
+            a.flags, a.bnd);
 }
 
 template <typename VT, typename FT, typename OT, int R, int PKW, int VPL, int CB>
@@ -851,21 +903,29 @@ static void launch_seg(const MttkrpArgs &a, hipStream_t s) {
         uint64_t dthreads = groups * LD;
         unsigned dblocks = cdiv(dthreads, 256);
         const int nm1 = a.pk.nmode - 1;
-        if (a.fiber) return launch_fib<VT, FT, OT, R, PKW, VPL, 1>(a, F, s);
         auto dec = [&](auto zc) {
             constexpr bool Z = decltype(zc)::value;
             if (nm1 == 2)
                 k_mttkrp_dec<VT, FT, OT, R, PKW, 2, VPL, Z><<<dblocks, 256, 0, s>>>(
-                    a.pkeys, (const VT *)a.vals, a.nnz, F, (OT *)a.out, a.pk, a.target, a.per_group, a.flags);
+                    a.pkeys, (const VT *)a.vals, a.nnz, F, (OT *)a.out, a.pk, a.target, a.per_group, a.flags,
+                    a.bnd);
             else if (nm1 == 3)
                 k_mttkrp_dec<VT, FT, OT, R, PKW, 3, VPL, Z><<<dblocks, 256, 0, s>>>(
-                    a.pkeys, (const VT *)a.vals, a.nnz, F, (OT *)a.out, a.pk, a.target, a.per_group, a.flags);
+                    a.pkeys, (const VT *)a.vals, a.nnz, F, (OT *)a.out, a.pk, a.target, a.per_group, a.flags,
+                    a.bnd);
             else
                 k_mttkrp_dec<VT, FT, OT, R, PKW, 4, VPL, Z><<<dblocks, 256, 0, s>>>(
-                    a.pkeys, (const VT *)a.vals, a.nnz, F, (OT *)a.out, a.pk, a.target, a.per_group, a.flags);
+                    a.pkeys, (const VT *)a.vals, a.nnz, F, (OT *)a.out, a.pk, a.target, a.per_group, a.flags,
+                    a.bnd);
         };
-        if (a.flags & 32) dec(std::true_type{});
+        if (a.fiber) launch_fib<VT, FT, OT, R, PKW, VPL, 1>(a, F, s);
+        else if (a.flags & 32) dec(std::true_type{});
         else dec(std::false_type{});
+        if (a.bnd.row) {  // the workgroups' boundary rows, from their FP64 slots
+            const uint64_t ns = 2 * (uint64_t)dblocks;
+            k_bnd_slots<OT, R><<<cdiv(ns * R, 256), 256, 0, s>>>(a.bnd.row, a.bnd.val, ns, (OT *)a.out,
+                                                                (a.flags & 32) ? 1 : 0);
+        }
     } else
         k_mttkrp_seg<VT, FT, OT, R, PKW, B><<<blocks, 256, 0, s>>>(
             a.pkeys, (const VT *)a.vals, a.nnz, F, (OT *)a.out, a.pk, a.target, a.per_group, a.flags);
@@ -944,10 +1004,23 @@ static bool autotune_enabled() {
 
 // One launch of the configuration in `a`: the hot-row kernel or the range
 // kernels.
-static int launch_cfg(TensorImpl *t, const MttkrpArgs &a, hipStream_t s) {
-    if (a.hot)
-        return run_hot(t, a.target, a.prec, a.rank, a.factors, a.out, a.out64, (a.flags & 32) != 0,
-                       a.hot_shape, s);
+static int launch_cfg(TensorImpl *t, const MttkrpArgs &a0, hipStream_t s) {
+    if (a0.hot)
+        return run_hot(t, a0.target, a0.prec, a0.rank, a0.factors, a0.out, a0.out64, (a0.flags & 32) != 0,
+                       a0.hot_shape, s);
+    MttkrpArgs a = a0;
+    a.bnd = BndSlots{nullptr, nullptr};
+    const int R = a.rank;
+    if (!a.out64 && (a.flags & 4) && a.decode_once && (R == 8 || R == 16 || R == 32 || R == 64)) {
+        // FP32 output of the decode-once / fiber kernels: the workgroups'
+        // boundary rows go through FP64 slots instead of FP32 atomics
+        const int vpl = std::min(a.prec == ALTO_PREC_F64 ? 4 : 8, R);
+        const uint64_t groups = (a.nnz + a.per_group - 1) / a.per_group;
+        const uint64_t ns = 2 * (uint64_t)cdiv(groups * (R / vpl), 256);
+        const size_t rb = (ns * sizeof(uint32_t) + 255) & ~(size_t)255;
+        ALTO_TRY(t->bnd_slots.alloc(rb + ns * R * sizeof(double)));
+        a.bnd = BndSlots{t->bnd_slots.as<uint32_t>(), (double *)((char *)t->bnd_slots.p + rb)};
+    }
     return t->pkw == 1 ? dispatch_prec<1>(a, s) : dispatch_prec<2>(a, s);
 }
 
@@ -1198,12 +1271,12 @@ __global__ __launch_bounds__(256) void k_mttkrp_group(const MttkrpJob *__restric
     if constexpr (DEC) {
         k_mttkrp_dec_body<VT, FT, double, R, 1, NM1, VPL, true>(b - first[j], J.pkeys, (const VT *)J.vals, J.nnz, F,
                                                                 (double *)J.out, J.pk, J.target, J.per_group,
-                                                                J.flags);
+                                                                J.flags, BndSlots{nullptr, nullptr});
     } else {
         const FibInfo fi = J.fi;
         k_mttkrp_fib_body<VT, FT, double, R, 1, NM1, VPL, 1, true>(b - first[j], J.pkeys, (const VT *)J.vals,
                                                                    J.nnz, F, (double *)J.out, J.pk, J.target, fi,
-                                                                   J.per_group, J.flags);
+                                                                   J.per_group, J.flags, BndSlots{nullptr, nullptr});
     }
 }
 

@@ -77,6 +77,7 @@ struct TensorImpl {
     ModeSchedule sched[kMaxModes];
     HotModes hot;
     DevBuf hot_bnd;     // boundary runs of the hot-row MTTKRP's workgroups
+    DevBuf bnd_slots;   // FP64 boundary rows of the FP32-output MTTKRP's workgroups
     SortScratch sort_ws;
     DevBuf scratch;     // generic per-call scratch
     DevBuf k0, k1, bad, ivbuf, chunkbuf;  // construction scratch, kept for rebuilds

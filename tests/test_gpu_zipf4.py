@@ -10,9 +10,11 @@ output), FP32 and BF16 factor storage (FP32 output, the benchmark's launch),
 and is compared with the oracle's COO MTTKRP (src/mttkrp.cpp:122-156, the
 reference's -c check, main.cpp:532-649).  Values and factors are positive,
 so the |X|-MTTKRP bound of the other MTTKRP tests is the MTTKRP itself:
-FP64 within 1e-12 relative per element, FP32 within 2e-5 (FP32 output: the
-boundary partials of a hot row are added in FP32), BF16 within 2e-5 against
-the oracle fed the same rounded factors.
+FP64 within 1e-12 relative per element, FP32 within 2e-5 (FP32 output:
+runs accumulate in FP32; the partial sums of rows that cross workgroup
+boundaries are added in FP64), BF16 within 2e-5 against the oracle fed the
+same rounded factors.  The FP32 output takes no atomics, so repeated calls
+give identical bits.
 
 CP-ALS (cpd_alto, include/cpd.hpp:47-191), 3 iterations from the same
 initial factors: the plain FP64 path equals the oracle (fits within 1e-8,
@@ -80,6 +82,20 @@ def test_zipf4_default_mttkrp_matches_oracle(gpu, zipf4, prec, tol):
             print("  call %d max rel err %.3g" % (call, err.max()), flush=True)
             assert err.max() <= tol, (prec, mode, call, err.max())
             out.close()
+
+
+def test_zipf4_fp32_output_is_deterministic(gpu, zipf4):
+    cidx, vals, at = zipf4
+    rng = np.random.default_rng(SEED + 1)
+    devF = [gpu.to_prec(rng.random((d, R)), gpu.PREC_F32) for d in DIMS]
+    for mode in range(4):
+        outs = []
+        for _ in range(3):
+            out = gpu.DeviceArray.zeros((DIMS[mode], R), np.float32)
+            gpu.mttkrp_alto_par(mode, devF, R, at, out, prec=gpu.PREC_F32, out_f64=False)
+            outs.append(out.numpy())
+            out.close()
+        assert all(np.array_equal(outs[0], o) for o in outs[1:]), mode
 
 
 def _rel(a, b):
