@@ -34,6 +34,9 @@
 
 namespace alto {
 
+// Rows of a factor per solve / Gram workgroup in a grouped launch.
+constexpr uint64_t kGroupRows = 1024;
+
 __global__ void k_set_active(int *__restrict__ active, const int *__restrict__ max_iters, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) active[i] = max_iters[i] > 0 ? 1 : 0;
@@ -341,8 +344,12 @@ static int group_start(AlsGroup &G, const alto_hip_tensor *tensors, const alto_h
         int ns = 1, ng = 1;
         for (int m = 0; m < N; ++m) {
             M.maxI = std::max<uint64_t>(M.maxI, t->dims[m]);
-            M.nbs[m] = row_blocks(t->dims[m]);
-            M.nbg[m] = gram_blocks(t->dims[m], R);
+            // the class's tensors fill the device together, so each takes
+            // one block per kGroupRows rows of a factor (not the single-run
+            // count of one block per 64 rows): fewer and smaller partials
+            const int nb = (int)std::max<uint64_t>(1, (t->dims[m] + kGroupRows - 1) / kGroupRows);
+            M.nbs[m] = std::min(row_blocks(t->dims[m]), nb);
+            M.nbg[m] = std::min(gram_blocks(t->dims[m], R), nb);
             ns = std::max(ns, M.nbs[m]);
             ng = std::max(ng, M.nbg[m]);
         }
