@@ -360,6 +360,38 @@ def cpd_alto_batch(tensors, factors, ranks, max_iters, eps=1e-30, prec=PREC_F64,
     return fs, lams, [f[: iters[k]] for k, f in enumerate(fits)], float(tim[0])
 
 
+def cpd_alto_batch_dev(tensors, factors, ranks, max_iters, eps=1e-30, prec=PREC_F64, fit_mode=FIT_REFERENCE_DIAG,
+                       nstreams=8):
+    """alto_hip_cpd_als_batch_dev: as cpd_alto_batch with device-resident
+    models.  factors[k]: DeviceArrays (FP64, dims x rank) holding tensor k's
+    initial model, overwritten with the final one.  Returns (lambdas, fits,
+    wall_seconds)."""
+    n = len(tensors)
+    for k in range(n):
+        for m, f in enumerate(factors[k]):
+            if f.dtype != np.float64 or f.shape != (int(tensors[k].dims[m]), int(ranks[k])):
+                raise ValueError("factor %d of tensor %d: need FP64 %d x %d" %
+                                 (m, k, int(tensors[k].dims[m]), int(ranks[k])))
+    lams = [np.zeros(int(r)) for r in ranks]
+    fits = [np.zeros(max(int(max_iters), 1)) for _ in range(n)]
+    iters = np.zeros(n, dtype=np.int32)
+    handles = (_vp * n)(*[t.handle for t in tensors])
+    opts = (_AlsOpts * n)(*[_AlsOpts(int(r), int(max_iters), eps, prec, fit_mode, 0) for r in ranks])
+    farr = (ctypes.POINTER(_f64p) * n)()
+    keep = []
+    for k in range(n):
+        a = (_f64p * len(factors[k]))(*[ctypes.cast(_vp(f.ptr), _f64p) for f in factors[k]])
+        keep.append(a)
+        farr[k] = ctypes.cast(a, ctypes.POINTER(_f64p))
+    larr = (_f64p * n)(*[l.ctypes.data_as(_f64p) for l in lams])
+    fitarr = (_f64p * n)(*[f.ctypes.data_as(_f64p) for f in fits])
+    tim = np.zeros(2)
+    check(lib().alto_hip_cpd_als_batch_dev(ctypes.c_int(n), handles, opts, farr, larr, fitarr,
+                                           iters.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                           ctypes.c_int(nstreams), tim.ctypes.data_as(_f64p)), "cpd_als_batch_dev")
+    return lams, [f[: iters[k]] for k, f in enumerate(fits)], float(tim[0])
+
+
 def cpstream(dims, cidx, vals, stream_mode, rank, max_iters, eps, nonneg, init_factors, max_slices=0,
              prec=PREC_F64, nnz_threshold=0, timeslice_limit=1, stream=None):
     """CP-stream (cpstream, src/streaming_cpd.cpp:26-366), host COO input.

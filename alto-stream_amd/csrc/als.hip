@@ -353,10 +353,10 @@ extern "C" int alto_hip_cpd_als(alto_hip_tensor th, const alto_hip_als_opts *o,
                    to_stream(stream));
 }
 
-extern "C" int alto_hip_cpd_als_batch(int ntensors, const alto_hip_tensor *tensors,
-                                      const alto_hip_als_opts *opts, double *const *const *factors,
-                                      double *const *lambdas, double *const *fits, int *iters,
-                                      int nstreams, double *timings) {
+// The batched CP-ALS over host (dev_io false) or device (true) FP64 models.
+static int cpd_als_batch(int ntensors, const alto_hip_tensor *tensors, const alto_hip_als_opts *opts,
+                         double *const *const *factors, double *const *lambdas, double *const *fits, int *iters,
+                         int nstreams, double *timings, bool dev_io) {
     if (ntensors < 0 || (ntensors > 0 && (!tensors || !opts || !factors)) || nstreams < 1 ||
         nstreams > 64)
         return ALTO_ERR_ARG;
@@ -409,7 +409,7 @@ extern "C" int alto_hip_cpd_als_batch(int ntensors, const alto_hip_tensor *tenso
         }
         std::vector<std::vector<int>> classes;
         for (auto &kv : cls) classes.push_back(kv.second);
-        ALTO_TRY(als_groups_run(classes, tensors, opts, factors, lambdas, fits, iters));
+        ALTO_TRY(als_groups_run(classes, tensors, opts, factors, lambdas, fits, iters, dev_io));
     } else {
         rest = order;
     }
@@ -431,8 +431,27 @@ extern "C" int alto_hip_cpd_als_batch(int ntensors, const alto_hip_tensor *tenso
             AlsWork w;
             for (int i = next++; i < (int)rest.size() && status == ALTO_OK; i = next++) {
                 const int k = rest[i];
-                int st = als_run(&tensors[k]->impl, &opts[k], factors[k], lambdas ? lambdas[k] : nullptr,
+                const TensorImpl *t = &tensors[k]->impl;
+                // a device model outside the grouped classes: a host round trip
+                std::vector<std::vector<double>> hf(dev_io ? t->nmode : 0);
+                double *hp[kMaxModes] = {};
+                int st = ALTO_OK;
+                for (int m = 0; m < (int)hf.size() && st == ALTO_OK; ++m) {
+                    const size_t cnt = (size_t)t->dims[m] * opts[k].rank;
+                    hf[m].resize(cnt);
+                    hp[m] = hf[m].data();
+                    if (hipMemcpyAsync(hp[m], factors[k][m], 8 * cnt, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                        hipStreamSynchronize(s) != hipSuccess)
+                        st = ALTO_ERR_HIP;
+                }
+                if (st == ALTO_OK)
+                    st = als_run(&tensors[k]->impl, &opts[k], dev_io ? hp : factors[k], lambdas ? lambdas[k] : nullptr,
                                  fits ? fits[k] : nullptr, nullptr, iters ? iters + k : nullptr, nullptr, w, s);
+                for (int m = 0; m < (int)hf.size() && st == ALTO_OK; ++m)
+                    if (hipMemcpyAsync(factors[k][m], hp[m], 8 * hf[m].size(), hipMemcpyHostToDevice, s) !=
+                            hipSuccess ||
+                        hipStreamSynchronize(s) != hipSuccess)
+                        st = ALTO_ERR_HIP;
                 if (st != ALTO_OK) {
                     std::lock_guard<std::mutex> g(err_mu);
                     if (status == ALTO_OK) {
@@ -459,6 +478,20 @@ extern "C" int alto_hip_cpd_als_batch(int ntensors, const alto_hip_tensor *tenso
         timings[1] = std::chrono::duration<double>(t1 - t0).count();
     }
     return ALTO_OK;
+}
+
+extern "C" int alto_hip_cpd_als_batch(int ntensors, const alto_hip_tensor *tensors,
+                                      const alto_hip_als_opts *opts, double *const *const *factors,
+                                      double *const *lambdas, double *const *fits, int *iters,
+                                      int nstreams, double *timings) {
+    return cpd_als_batch(ntensors, tensors, opts, factors, lambdas, fits, iters, nstreams, timings, false);
+}
+
+extern "C" int alto_hip_cpd_als_batch_dev(int ntensors, const alto_hip_tensor *tensors,
+                                          const alto_hip_als_opts *opts, double *const *const *factors,
+                                          double *const *lambdas, double *const *fits, int *iters,
+                                          int nstreams, double *timings) {
+    return cpd_als_batch(ntensors, tensors, opts, factors, lambdas, fits, iters, nstreams, timings, true);
 }
 
 // pseudo_inverse(grams, M, mode) (cpd.hpp:377-480): U <- U (Had_{m != mode}

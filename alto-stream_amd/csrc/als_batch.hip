@@ -243,6 +243,17 @@ static int download_region(const std::vector<Seg> &segs, size_t bytes, int prec,
     return ALTO_OK;
 }
 
+// Device-resident models: the segments' FP64 matrices are device arrays,
+// converted into / out of the region on the stream (no host copy).
+static int upload_region_dev(const std::vector<Seg> &segs, int prec, char *dev, hipStream_t s) {
+    for (const Seg &g : segs) ALTO_TRY(alto_hip_convert(dev + g.off, prec, g.host, ALTO_PREC_F64, g.n, s));
+    return ALTO_OK;
+}
+static int download_region_dev(const std::vector<Seg> &segs, int prec, const char *dev, hipStream_t s) {
+    for (const Seg &g : segs) ALTO_TRY(alto_hip_convert(g.host, ALTO_PREC_F64, dev + g.off, prec, g.n, s));
+    return ALTO_OK;
+}
+
 static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // One class of tensors in flight: device memory, jobs, graph.
@@ -250,6 +261,7 @@ struct AlsGroup {
     std::vector<int> ks;  // tensor indices
     int N = 0, R = 0, prec = 0, fit_mode = 0, maxit = 0;
     bool any_eps = false;
+    bool dev_io = false;  // the models are device arrays
     hipStream_t s = nullptr;
     DevBuf arena, jobs;
     char *Ubase = nullptr;
@@ -521,7 +533,8 @@ static int group_start(AlsGroup &G, const alto_hip_tensor *tensors, const alto_h
         const TensorImpl *t = &tensors[k]->impl;
         for (int m = 0; m < N; ++m) G.segs.push_back(Seg{mem[i].U[m], (size_t)t->dims[m] * R, factors[k][m]});
     }
-    ALTO_TRY(upload_region(G.segs, G.Ubytes, G.prec, G.Ubase, st, pool, s));
+    if (G.dev_io) ALTO_TRY(upload_region_dev(G.segs, G.prec, G.Ubase, s));
+    else ALTO_TRY(upload_region(G.segs, G.Ubytes, G.prec, G.Ubase, st, pool, s));
     for (int m = 0; m < N; ++m) {
         ALTO_TRY(launch_gram_group(G.bj[m], G.fg[m], n, G.tg[m], R, G.prec, s));
         ALTO_TRY(launch_gram_reduce_group(G.bj[m], n, R, s));
@@ -564,7 +577,8 @@ static int group_start(AlsGroup &G, const alto_hip_tensor *tensors, const alto_h
 static int group_finish(AlsGroup &G, double *const *lambdas, double *const *fits, int *iters, Staging &st,
                         HostPool &pool) {
     const int n = (int)G.ks.size(), R = G.R;
-    ALTO_TRY(download_region(G.segs, G.Ubytes, G.prec, G.Ubase, st, pool, G.s));
+    if (G.dev_io) ALTO_TRY(download_region_dev(G.segs, G.prec, G.Ubase, G.s));
+    else ALTO_TRY(download_region(G.segs, G.Ubytes, G.prec, G.Ubase, st, pool, G.s));
     std::vector<double> lam((size_t)n * R), fd((size_t)n * 2 * G.maxit);
     std::vector<int> cnt(n);
     ALTO_HIP_TRY(hipMemcpyAsync(lam.data(), G.lam, 8 * lam.size(), hipMemcpyDeviceToHost, G.s));
@@ -604,15 +618,16 @@ bool als_group_eligible(const TensorImpl *t, const alto_hip_als_opts *o) {
 
 int als_groups_run(const std::vector<std::vector<int>> &classes, const alto_hip_tensor *tensors,
                    const alto_hip_als_opts *opts, double *const *const *factors, double *const *lambdas,
-                   double *const *fits, int *iters) {
+                   double *const *fits, int *iters, bool dev_io) {
     if (classes.empty()) return ALTO_OK;
-    Staging st;
-    ALTO_TRY(st.init());
+    Staging st;  // pinned staging and host converter threads: host models only
+    if (!dev_io) ALTO_TRY(st.init());
     unsigned hc = std::thread::hardware_concurrency();
-    HostPool pool((int)std::min<unsigned>(hc ? hc : 1, 16));
+    HostPool pool(dev_io ? 1 : (int)std::min<unsigned>(hc ? hc : 1, 16));
     std::vector<AlsGroup> groups(classes.size());
     for (size_t g = 0; g < classes.size(); ++g) {
         groups[g].ks = classes[g];
+        groups[g].dev_io = dev_io;
         ALTO_TRY(group_start(groups[g], tensors, opts, factors, st, pool));
         if (!concurrent_classes()) ALTO_HIP_TRY(hipStreamSynchronize(groups[g].s));
     }

@@ -166,11 +166,12 @@ int run_hot(TensorImpl *t, int target, int prec, int rank, const void *const *fa
 // Grouped CP-ALS (als_batch.hip): tensors that qualify (3 or 4 modes, 64-bit
 // keys, rank 8, 16, 32 or 64, FP32 or FP64) run class by class, every ALS
 // step one launch per class.  classes: lists of tensor indices sharing mode
-// count, rank, precision and fit convention.
+// count, rank, precision and fit convention.  dev_io: factors[k][m] are
+// device FP64 arrays (else host).
 bool als_group_eligible(const TensorImpl *t, const alto_hip_als_opts *o);
 int als_groups_run(const std::vector<std::vector<int>> &classes, const alto_hip_tensor *tensors,
                    const alto_hip_als_opts *opts, double *const *const *factors, double *const *lambdas,
-                   double *const *fits, int *iters);
+                   double *const *fits, int *iters, bool dev_io);
 bool grouped_als_enabled();
 
 }  // namespace alto

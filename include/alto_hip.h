@@ -213,6 +213,17 @@ int alto_hip_cpd_als_batch(int ntensors, const alto_hip_tensor *tensors,
                            const alto_hip_als_opts *opts, double *const *const *factors,
                            double *const *lambdas, double *const *fits, int *iters,
                            int nstreams, double *timings);
+/* The same with device-resident models: factors[k][m] are device FP64
+ * arrays (dims x rank; initial model in, final model out), converted into
+ * and out of the working precision on the device, so no model crosses
+ * PCIe (lambdas, fits and iters stay host arrays).  A tensor outside the
+ * grouped classes (see alto_hip_cpd_als_batch) takes a host round trip.
+ * No reference counterpart (the batched workload has none); for callers
+ * whose models already live in HBM. */
+int alto_hip_cpd_als_batch_dev(int ntensors, const alto_hip_tensor *tensors,
+                               const alto_hip_als_opts *opts, double *const *const *factors,
+                               double *const *lambdas, double *const *fits, int *iters,
+                               int nstreams, double *timings);
 
 /* ---- pseudo_inverse (include/cpd.hpp:377-480): U <- U V^{-1} row by row with
  * V = Hadamard of grams[m], m != mode (nmode consecutive rank x rank

@@ -65,3 +65,40 @@ def test_batch_rejects_verbose_and_bad_stream_count(gpu):
     with pytest.raises(gpu.AltoError, match="invalid argument"):
         gpu.cpd_alto_batch([at], [F], [R], 2, nstreams=0)
     at.close()
+
+
+@pytest.mark.parametrize("prec,ftol", [(0, 1e-10), (1, 1e-6), (2, 1e-6)])
+def test_batch_device_models_equal_host_models(gpu, prec, ftol):
+    """alto_hip_cpd_als_batch_dev (models in device memory, converted into
+    and out of the working precision on the device) gives the host-model
+    batch's results: the same kernels run on the same converted values
+    (FP64 -> FP32 / BF16 rounds to nearest on the host and on the device).
+    The mix holds 3- and 4-mode tensors at ranks 8, 16, 32 (the grouped
+    classes) and a rank-24 tensor, which the grouped path does not take (a
+    host round trip of its device model).  Workgroup-boundary rows of the
+    FP64 MTTKRP outputs are added with atomics, so bars as for the batch
+    against single runs: fits within ftol, FP64 lambda and factors within
+    1e-9 relative."""
+    inst = _instances(7, 700)
+    d, c, v = rand_tensor([150, 90, 60], 4000, seed=777, dup_frac=0.05)
+    inst.append((d, c, v, 24, rand_factors(d, 24, seed=778)))
+    ats = [gpu.create_alto(d, c, v, nprtn=4) for d, c, v, _, _ in inst]
+    ranks = [x[3] for x in inst]
+    iters = 5
+    hf, hl, hfits, _ = gpu.cpd_alto_batch(ats, [x[4] for x in inst], ranks, iters, prec=prec, nstreams=2)
+    dF = [[gpu.DeviceArray.from_numpy(np.ascontiguousarray(f)) for f in x[4]] for x in inst]
+    dl, dfits, wall = gpu.cpd_alto_batch_dev(ats, dF, ranks, iters, prec=prec, nstreams=2)
+    assert wall > 0
+    for k in range(len(inst)):
+        assert len(hfits[k]) == len(dfits[k]) == iters
+        np.testing.assert_allclose(dfits[k], hfits[k], rtol=0, atol=ftol)
+        got = [f.numpy() for f in dF[k]]
+        if prec == 0:
+            np.testing.assert_allclose(dl[k], hl[k], rtol=1e-9)
+            for a, b in zip(got, hf[k]):
+                np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-12)
+        else:  # the final model is the stored FP32 / BF16 factor, widened
+            for a, b in zip(got, hf[k]):
+                np.testing.assert_allclose(a, b, rtol=1e-4, atol=1e-6)
+    for a in ats:
+        a.close()
