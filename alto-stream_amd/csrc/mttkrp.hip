@@ -293,14 +293,16 @@ template <typename FT, int VPL> struct RowLoad {
 };
 
 
-// FP64 boundary slots of an FP32-output MTTKRP: workgroup lb writes the sum
+// Boundary slots of an FP32-output MTTKRP: workgroup lb writes its FP32 sum
 // of its first row to slot 2 lb and of its last row to slot 2 lb + 1 (a
 // workgroup of one row repeats it with zero), in schedule order, and
-// k_bnd_slots sums the runs of equal rows in slot order in FP64.  With
+// launch_bnd_merge adds the runs of equal rows in slot order in FP64.  With
 // row == nullptr the boundary rows take hardware atomics in the output type.
 struct BndSlots {
     uint32_t *row;
-    double *val;
+    float *val;         // a workgroup's partial is an FP32 sum: exact in FP32
+    uint32_t *lrow[2];  // the merge's level buffers (FP64 sums)
+    double *lval[2];
 };
 
 // Combines one workgroup's range partials: slot e = 0 .. 2G-1 holds group
@@ -394,30 +396,113 @@ __device__ __forceinline__ void block_combine(const uint32_t (*s_row)[G], const 
             bs.row[2 * (uint64_t)lb] = r0;
             bs.row[2 * (uint64_t)lb + 1] = mrow;
         }
-        bs.val[2 * (uint64_t)lb * R + col] = (double)a0;
-        bs.val[(2 * (uint64_t)lb + 1) * R + col] = (double)a;
+        bs.val[2 * (uint64_t)lb * R + col] = (float)a0;
+        bs.val[(2 * (uint64_t)lb + 1) * R + col] = (float)a;
     } else if (mrow != NONE) {
         atomic_add_out(o + (uint64_t)mrow * R, (OT)a);
     }
 }
 
-// Sums the FP64 boundary slots (rows non-decreasing in slot order): the
-// thread of (slot, column) that starts a run of equal rows adds the run in
-// slot order and writes the row (stored when the output held zeros).
-template <typename OT, int R>
-__global__ __launch_bounds__(256) void k_bnd_slots(const uint32_t *__restrict__ rows,
-                                                   const double *__restrict__ vals, uint64_t nslots,
-                                                   OT *__restrict__ out, int zeroed) {
+// Sums the FP64 boundary slots (rows non-decreasing in slot order) as a
+// fixed tree, so the result does not depend on scheduling: at each level a
+// thread per (chunk of kSlotChunk slots, column) adds the runs of equal rows
+// of its chunk in slot order, writes the runs that start and end inside the
+// chunk to the output (stored when the output held zeros), and passes its
+// first and last runs -- which may continue in the neighbouring chunks -- to
+// the next level as two slots (a chunk of one run passes the row twice, the
+// second with zero).  At the last level (one chunk) every run is complete.
+// A hot output row spans thousands of workgroups, so a single serial walk of
+// its run would take milliseconds.
+constexpr int kSlotChunk = 32;
+
+template <typename OT, int R, typename VT>
+__global__ __launch_bounds__(256) void k_bnd_level(const uint32_t *__restrict__ rows,
+                                                   const VT *__restrict__ vals, uint64_t n,
+                                                   uint32_t *__restrict__ orow, double *__restrict__ oval,
+                                                   OT *__restrict__ out, int zeroed, int last) {
     const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    const uint64_t i = t / R;
+    const uint64_t c = t / R;
     const int col = (int)(t % R);
-    if (i >= nslots) return;
-    const uint32_t row = rows[i];
-    if (row == 0xFFFFFFFFu || (i > 0 && rows[i - 1] == row)) return;
-    double sum = 0;
-    for (uint64_t j = i; j < nslots && rows[j] == row; ++j) sum += vals[j * R + col];
-    OT *p = out + (uint64_t)row * R + col;
-    *p = zeroed ? (OT)sum : (OT)((double)*p + sum);
+    const uint64_t a = c * kSlotChunk;
+    if (a >= n) return;
+    const uint64_t b = a + kSlotChunk < n ? a + kSlotChunk : n;
+    auto put = [&](uint32_t row, double v) {
+        OT *p = out + (uint64_t)row * R + col;
+        *p = zeroed ? (OT)v : (OT)((double)*p + v);
+    };
+    // the chunk's rows, then its values, as independent loads (one memory
+    // round trip each), walked from registers
+    uint32_t rr[kSlotChunk];
+    double xv[kSlotChunk];
+#pragma unroll
+    for (int k = 0; k < kSlotChunk; ++k) rr[k] = a + k < b ? rows[a + k] : 0xFFFFFFFFu;
+#pragma unroll
+    for (int k = 0; k < kSlotChunk; ++k) xv[k] = rr[k] != 0xFFFFFFFFu ? (double)vals[(a + k) * R + col] : 0.0;
+    const uint32_t r0 = rr[0];
+    uint32_t cur = r0;
+    double s0 = 0, cs = 0;
+    bool first = true;
+#pragma unroll
+    for (int k = 0; k < kSlotChunk; ++k) {
+        const uint32_t r = rr[k];
+        if (r == 0xFFFFFFFFu) break;  // empty slots close the array
+        const double x = xv[k];
+        if (r == cur) {
+            cs += x;
+            continue;
+        }
+        if (first) {  // the chunk's first run ends here
+            s0 = cs;
+            first = false;
+        } else {      // a run inside the chunk: complete
+            put(cur, cs);
+        }
+        cur = r;
+        cs = x;
+    }
+    if (r0 == 0xFFFFFFFFu) {
+        if (!last && col == 0) orow[2 * c] = orow[2 * c + 1] = 0xFFFFFFFFu;
+        return;
+    }
+    if (first) {  // one run over the chunk
+        s0 = cs;
+        cs = 0;
+    }
+    if (last) {
+        put(r0, s0);
+        if (!first) put(cur, cs);
+        return;
+    }
+    if (col == 0) {
+        orow[2 * c] = r0;
+        orow[2 * c + 1] = cur;
+    }
+    oval[2 * c * R + col] = s0;
+    oval[(2 * c + 1) * R + col] = cs;
+}
+
+// Launches the levels over ns slots; lv: two level buffers of
+// 2 * ceil(ns / kSlotChunk) slots each (rows, then FP64 values).
+template <typename OT, int R>
+static void launch_bnd_merge(const uint32_t *rows, const float *vals32, uint64_t ns, uint32_t *const *lrow,
+                             double *const *lval, OT *out, int zeroed, hipStream_t s) {
+    const double *vals = nullptr;  // FP64 level sums after the first level
+    int k = 0;
+    for (;;) {
+        const uint64_t nch = (ns + kSlotChunk - 1) / kSlotChunk;
+        const int last = nch <= 1;
+        if (vals)
+            k_bnd_level<OT, R, double><<<cdiv(nch * R, 256), 256, 0, s>>>(rows, vals, ns, lrow[k], lval[k], out,
+                                                                         zeroed, last);
+        else
+            k_bnd_level<OT, R, float><<<cdiv(nch * R, 256), 256, 0, s>>>(rows, vals32, ns, lrow[k], lval[k], out,
+                                                                        zeroed, last);
+        if (last) return;
+        rows = lrow[k];
+        vals = lval[k];
+        ns = 2 * nch;
+        k ^= 1;
+    }
 }
 
 template <typename VT, typename FT, typename OT, int R, int PKW, int NM1, int VPL, bool ZEROED>
@@ -923,8 +1008,8 @@ static void launch_seg(const MttkrpArgs &a, hipStream_t s) {
         else dec(std::false_type{});
         if (a.bnd.row) {  // the workgroups' boundary rows, from their FP64 slots
             const uint64_t ns = 2 * (uint64_t)dblocks;
-            k_bnd_slots<OT, R><<<cdiv(ns * R, 256), 256, 0, s>>>(a.bnd.row, a.bnd.val, ns, (OT *)a.out,
-                                                                (a.flags & 32) ? 1 : 0);
+            launch_bnd_merge<OT, R>(a.bnd.row, a.bnd.val, ns, a.bnd.lrow, a.bnd.lval, (OT *)a.out,
+                                    (a.flags & 32) ? 1 : 0, s);
         }
     } else
         k_mttkrp_seg<VT, FT, OT, R, PKW, B><<<blocks, 256, 0, s>>>(
@@ -1009,7 +1094,7 @@ static int launch_cfg(TensorImpl *t, const MttkrpArgs &a0, hipStream_t s) {
         return run_hot(t, a0.target, a0.prec, a0.rank, a0.factors, a0.out, a0.out64, (a0.flags & 32) != 0,
                        a0.hot_shape, s);
     MttkrpArgs a = a0;
-    a.bnd = BndSlots{nullptr, nullptr};
+    a.bnd = BndSlots{};
     const int R = a.rank;
     if (!a.out64 && (a.flags & 4) && a.decode_once && (R == 8 || R == 16 || R == 32 || R == 64)) {
         // FP32 output of the decode-once / fiber kernels: the workgroups'
@@ -1017,9 +1102,18 @@ static int launch_cfg(TensorImpl *t, const MttkrpArgs &a0, hipStream_t s) {
         const int vpl = std::min(a.prec == ALTO_PREC_F64 ? 4 : 8, R);
         const uint64_t groups = (a.nnz + a.per_group - 1) / a.per_group;
         const uint64_t ns = 2 * (uint64_t)cdiv(groups * (R / vpl), 256);
-        const size_t rb = (ns * sizeof(uint32_t) + 255) & ~(size_t)255;
-        ALTO_TRY(t->bnd_slots.alloc(rb + ns * R * sizeof(double)));
-        a.bnd = BndSlots{t->bnd_slots.as<uint32_t>(), (double *)((char *)t->bnd_slots.p + rb)};
+        const uint64_t nl = 2 * ((ns + kSlotChunk - 1) / kSlotChunk);  // slots of the merge's first level
+        auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+        const size_t rb = al(ns * sizeof(uint32_t)), vb = al(ns * R * sizeof(float));
+        const size_t lrb = al(nl * sizeof(uint32_t)), lvb = al(nl * R * sizeof(double));
+        ALTO_TRY(t->bnd_slots.alloc(rb + vb + 2 * (lrb + lvb)));
+        char *base = (char *)t->bnd_slots.p;
+        a.bnd.row = (uint32_t *)base;
+        a.bnd.val = (float *)(base + rb);
+        for (int k = 0; k < 2; ++k) {
+            a.bnd.lrow[k] = (uint32_t *)(base + rb + vb + k * (lrb + lvb));
+            a.bnd.lval[k] = (double *)(base + rb + vb + k * (lrb + lvb) + lrb);
+        }
     }
     return t->pkw == 1 ? dispatch_prec<1>(a, s) : dispatch_prec<2>(a, s);
 }
@@ -1271,12 +1365,12 @@ __global__ __launch_bounds__(256) void k_mttkrp_group(const MttkrpJob *__restric
     if constexpr (DEC) {
         k_mttkrp_dec_body<VT, FT, double, R, 1, NM1, VPL, true>(b - first[j], J.pkeys, (const VT *)J.vals, J.nnz, F,
                                                                 (double *)J.out, J.pk, J.target, J.per_group,
-                                                                J.flags, BndSlots{nullptr, nullptr});
+                                                                J.flags, BndSlots{});
     } else {
         const FibInfo fi = J.fi;
         k_mttkrp_fib_body<VT, FT, double, R, 1, NM1, VPL, 1, true>(b - first[j], J.pkeys, (const VT *)J.vals,
                                                                    J.nnz, F, (double *)J.out, J.pk, J.target, fi,
-                                                                   J.per_group, J.flags, BndSlots{nullptr, nullptr});
+                                                                   J.per_group, J.flags, BndSlots{});
     }
 }
 

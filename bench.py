@@ -57,6 +57,10 @@ def workload(name):
         return dict(dims=[500000, 200000, 50000, 1000], nnz=100_000_000, seed=0xA1700001, rank=32,
                     mode_dist=[1, 1, 1, 1], zipf=[1.1] * 4, vdist=1, p1=0.0, p2=1.0,
                     desc="4-mode 500000x200000x50000x1000, 1e8 nnz, Zipf(1.1) indices, lognormal(0,1) values, rank 32")
+    if name == "bf16_5mode":  # the 5-mode 128-bit-key workload (for --counters)
+        return dict(dims=[1 << 20, 1 << 20, 1 << 18, 1 << 16, 1 << 12], nnz=50_000_000, seed=0xA1700003, rank=64,
+                    mode_dist=[0] * 5, zipf=None, vdist=2, p1=0.0, p2=1.0, block_frac=0.3, block=256,
+                    desc="5-mode 2^20x2^20x2^18x2^16x2^12, 5e7 nnz (30% in a 256^5 block), N(0,1) values, rank 64")
     if name == "zipf4_small":  # reduced shape for quick checks (not a bench line)
         return dict(dims=[50000, 20000, 5000, 100], nnz=2_000_000, seed=0xA1700001, rank=32,
                     mode_dist=[1, 1, 1, 1], zipf=[1.1] * 4, vdist=1, p1=0.0, p2=1.0,
@@ -131,7 +135,8 @@ def main():
     prec = {"f32": A.PREC_F32, "bf16": A.PREC_BF16, "f64": A.PREC_F64}[args.prec]
     log("device:", dev)
     t0 = time.time()
-    cidx, vals = A.generate(dims, nnz, W["seed"], W["mode_dist"], W["zipf"], W["vdist"], W["p1"], W["p2"])
+    cidx, vals = A.generate(dims, nnz, W["seed"], W["mode_dist"], W["zipf"], W["vdist"], W["p1"], W["p2"],
+                            block_frac=W.get("block_frac", 0.0), block=W.get("block", 0))
     A.synchronize()
     log("generated %d nnz in %.1f s" % (nnz, time.time() - t0))
     t0 = time.time()
