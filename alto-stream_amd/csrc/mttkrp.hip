@@ -403,108 +403,6 @@ __device__ __forceinline__ void block_combine(const uint32_t (*s_row)[G], const 
     }
 }
 
-// Sums the FP64 boundary slots (rows non-decreasing in slot order) as a
-// fixed tree, so the result does not depend on scheduling: at each level a
-// thread per (chunk of kSlotChunk slots, column) adds the runs of equal rows
-// of its chunk in slot order, writes the runs that start and end inside the
-// chunk to the output (stored when the output held zeros), and passes its
-// first and last runs -- which may continue in the neighbouring chunks -- to
-// the next level as two slots (a chunk of one run passes the row twice, the
-// second with zero).  At the last level (one chunk) every run is complete.
-// A hot output row spans thousands of workgroups, so a single serial walk of
-// its run would take milliseconds.
-constexpr int kSlotChunk = 32;
-
-template <typename OT, int R, typename VT>
-__global__ __launch_bounds__(256) void k_bnd_level(const uint32_t *__restrict__ rows,
-                                                   const VT *__restrict__ vals, uint64_t n,
-                                                   uint32_t *__restrict__ orow, double *__restrict__ oval,
-                                                   OT *__restrict__ out, int zeroed, int last) {
-    const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    const uint64_t c = t / R;
-    const int col = (int)(t % R);
-    const uint64_t a = c * kSlotChunk;
-    if (a >= n) return;
-    const uint64_t b = a + kSlotChunk < n ? a + kSlotChunk : n;
-    auto put = [&](uint32_t row, double v) {
-        OT *p = out + (uint64_t)row * R + col;
-        *p = zeroed ? (OT)v : (OT)((double)*p + v);
-    };
-    // the chunk's rows, then its values, as independent loads (one memory
-    // round trip each), walked from registers
-    uint32_t rr[kSlotChunk];
-    double xv[kSlotChunk];
-#pragma unroll
-    for (int k = 0; k < kSlotChunk; ++k) rr[k] = a + k < b ? rows[a + k] : 0xFFFFFFFFu;
-#pragma unroll
-    for (int k = 0; k < kSlotChunk; ++k) xv[k] = rr[k] != 0xFFFFFFFFu ? (double)vals[(a + k) * R + col] : 0.0;
-    const uint32_t r0 = rr[0];
-    uint32_t cur = r0;
-    double s0 = 0, cs = 0;
-    bool first = true;
-#pragma unroll
-    for (int k = 0; k < kSlotChunk; ++k) {
-        const uint32_t r = rr[k];
-        if (r == 0xFFFFFFFFu) break;  // empty slots close the array
-        const double x = xv[k];
-        if (r == cur) {
-            cs += x;
-            continue;
-        }
-        if (first) {  // the chunk's first run ends here
-            s0 = cs;
-            first = false;
-        } else {      // a run inside the chunk: complete
-            put(cur, cs);
-        }
-        cur = r;
-        cs = x;
-    }
-    if (r0 == 0xFFFFFFFFu) {
-        if (!last && col == 0) orow[2 * c] = orow[2 * c + 1] = 0xFFFFFFFFu;
-        return;
-    }
-    if (first) {  // one run over the chunk
-        s0 = cs;
-        cs = 0;
-    }
-    if (last) {
-        put(r0, s0);
-        if (!first) put(cur, cs);
-        return;
-    }
-    if (col == 0) {
-        orow[2 * c] = r0;
-        orow[2 * c + 1] = cur;
-    }
-    oval[2 * c * R + col] = s0;
-    oval[(2 * c + 1) * R + col] = cs;
-}
-
-// Launches the levels over ns slots; lv: two level buffers of
-// 2 * ceil(ns / kSlotChunk) slots each (rows, then FP64 values).
-template <typename OT, int R>
-static void launch_bnd_merge(const uint32_t *rows, const float *vals32, uint64_t ns, uint32_t *const *lrow,
-                             double *const *lval, OT *out, int zeroed, hipStream_t s) {
-    const double *vals = nullptr;  // FP64 level sums after the first level
-    int k = 0;
-    for (;;) {
-        const uint64_t nch = (ns + kSlotChunk - 1) / kSlotChunk;
-        const int last = nch <= 1;
-        if (vals)
-            k_bnd_level<OT, R, double><<<cdiv(nch * R, 256), 256, 0, s>>>(rows, vals, ns, lrow[k], lval[k], out,
-                                                                         zeroed, last);
-        else
-            k_bnd_level<OT, R, float><<<cdiv(nch * R, 256), 256, 0, s>>>(rows, vals32, ns, lrow[k], lval[k], out,
-                                                                        zeroed, last);
-        if (last) return;
-        rows = lrow[k];
-        vals = lval[k];
-        ns = 2 * nch;
-        k ^= 1;
-    }
-}
-
 template <typename VT, typename FT, typename OT, int R, int PKW, int NM1, int VPL, bool ZEROED>
 __device__ __forceinline__ void k_mttkrp_dec_body(unsigned lb, const uint64_t *__restrict__ pkeys,
                                                   const VT *__restrict__ vals, uint64_t nnz,
@@ -1008,8 +906,8 @@ static void launch_seg(const MttkrpArgs &a, hipStream_t s) {
         else dec(std::false_type{});
         if (a.bnd.row) {  // the workgroups' boundary rows, from their FP64 slots
             const uint64_t ns = 2 * (uint64_t)dblocks;
-            launch_bnd_merge<OT, R>(a.bnd.row, a.bnd.val, ns, a.bnd.lrow, a.bnd.lval, (OT *)a.out,
-                                    (a.flags & 32) ? 1 : 0, s);
+            launch_bnd_merge<OT, R, float>(a.bnd.row, a.bnd.val, ns, a.bnd.lrow, a.bnd.lval, (OT *)a.out,
+                                           (a.flags & 32) ? 1 : 0, s);
         }
     } else
         k_mttkrp_seg<VT, FT, OT, R, PKW, B><<<blocks, 256, 0, s>>>(
@@ -1102,7 +1000,7 @@ static int launch_cfg(TensorImpl *t, const MttkrpArgs &a0, hipStream_t s) {
         const int vpl = std::min(a.prec == ALTO_PREC_F64 ? 4 : 8, R);
         const uint64_t groups = (a.nnz + a.per_group - 1) / a.per_group;
         const uint64_t ns = 2 * (uint64_t)cdiv(groups * (R / vpl), 256);
-        const uint64_t nl = 2 * ((ns + kSlotChunk - 1) / kSlotChunk);  // slots of the merge's first level
+        const uint64_t nl = bnd_level_slots(ns);  // slots of the merge's level buffers
         auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
         const size_t rb = al(ns * sizeof(uint32_t)), vb = al(ns * R * sizeof(float));
         const size_t lrb = al(nl * sizeof(uint32_t)), lvb = al(nl * R * sizeof(double));

@@ -30,7 +30,7 @@
 // owned and stored directly.  The first and last runs of every group are
 // combined across the workgroup in FP64 in LDS (the hot tables' space, after
 // a barrier); rows that start and end inside the workgroup are owned, and the
-// workgroup's first and last rows go to a boundary buffer that k_bnd_merge
+// workgroup's first and last rows go to a boundary buffer that launch_bnd_merge
 // sums in schedule order in FP64.  No atomics: the result does not depend on
 // scheduling.
 #include <algorithm>
@@ -323,10 +323,18 @@ int run_hot(TensorImpl *t, int target, int prec, int rank, const void *const *fa
     h.ha.tsh = hs.hpk.shift[target];
     h.ha.tw = hs.hpk.width[target];
     h.hot_bytes = lds;
-    const size_t nslot = 2 * (size_t)h.nwg;
-    ALTO_TRY(t->hot_bnd.alloc(nslot * sizeof(uint32_t) + 256 + nslot * rank * sizeof(double)));
-    h.bnd_row = t->hot_bnd.as<uint32_t>();
-    h.bnd_val = (double *)((char *)t->hot_bnd.p + ((nslot * sizeof(uint32_t) + 255) & ~(size_t)255));
+    const size_t nslot = 2 * (size_t)h.nwg, nl = bnd_level_slots(nslot);
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t srb = al(nslot * sizeof(uint32_t)), svb = al(nslot * rank * sizeof(double));
+    const size_t lrb = al(nl * sizeof(uint32_t)), lvb = al(nl * rank * sizeof(double));
+    ALTO_TRY(t->hot_bnd.alloc(srb + svb + 2 * (lrb + lvb)));
+    char *base = (char *)t->hot_bnd.p;
+    h.bnd_row = (uint32_t *)base;
+    h.bnd_val = (double *)(base + srb);
+    for (int k = 0; k < 2; ++k) {
+        h.lrow[k] = (uint32_t *)(base + srb + svb + k * (lrb + lvb));
+        h.lval[k] = (double *)(base + srb + svb + k * (lrb + lvb) + lrb);
+    }
     if (prec == ALTO_PREC_F64) return hot_rank<double, double, double>(rank, h, s);
     if (prec == ALTO_PREC_F32)
         return out64 ? hot_rank<float, float, double>(rank, h, s) : hot_rank<float, float, float>(rank, h, s);

@@ -466,24 +466,6 @@ __global__ __launch_bounds__(NT) void k_mttkrp_hot(const uint64_t *__restrict__ 
                                      bnd_val + (size_t)(2 * lb + 1) * R, 0);
 }
 
-// The boundary runs of all workgroups, in schedule order: equal rows summed
-// in FP64 and written to the output (added unless it was zero on entry).
-template <typename OT, int R>
-__global__ __launch_bounds__(kHotThreads) void k_bnd_merge(const uint32_t *__restrict__ rows,
-                                                           const double *__restrict__ vals, int nslots,
-                                                           OT *__restrict__ out, int zeroed) {
-    __shared__ uint32_t f_row[R], l_row[R];
-    __shared__ double f_val[R], l_val[R];
-    extern __shared__ __align__(16) char scr[];
-    combine_runs<OT, R, kHotThreads>(rows, vals, nslots, scr, out, zeroed != 0, f_row, f_val, l_row, l_val, 1);
-    if (threadIdx.x % (kHotThreads / R) == 0) {
-        const int col = threadIdx.x / (kHotThreads / R);
-        if (f_row[col] != kNone) store_final(out + (uint64_t)f_row[col] * R + col, f_val[col], zeroed != 0);
-        if (l_row[col] != kNone) store_final(out + (uint64_t)l_row[col] * R + col, l_val[col], zeroed != 0);
-    }
-}
-
-
 struct HotLaunch {
     const uint64_t *keys;
     const void *vals;
@@ -496,6 +478,8 @@ struct HotLaunch {
     size_t hot_bytes;
     uint32_t *bnd_row;
     double *bnd_val;
+    uint32_t *lrow[2];  // the boundary merge's level buffers
+    double *lval[2];
 };
 
 // Launches the kernel and the boundary merge for ranks 16, 32, 64.
@@ -534,8 +518,8 @@ static int hot_launch_nt(const HotLaunch &h, hipStream_t s) {
     k<<<h.nwg, NT, lds, s>>>(h.keys, (const VT *)h.vals, h.n, h.ha, (OT *)h.out, pg, h.nwg, h.zeroed,
                              h.bnd_row, h.bnd_val);
     ALTO_LAUNCH_CHECK();
-    k_bnd_merge<OT, R><<<1, kHotThreads, 24 * kHotThreads, s>>>(h.bnd_row, h.bnd_val, 2 * h.nwg, (OT *)h.out,
-                                                               h.zeroed ? 1 : 0);
+    launch_bnd_merge<OT, R, double>(h.bnd_row, h.bnd_val, 2 * (uint64_t)h.nwg, h.lrow, h.lval, (OT *)h.out,
+                                    h.zeroed ? 1 : 0, s);
     ALTO_LAUNCH_CHECK();
     return ALTO_OK;
 }
