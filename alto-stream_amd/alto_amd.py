@@ -177,11 +177,22 @@ class AltoTensor:
                                            _vp(stream), ctypes.byref(h)), "create_alto")
         del keep
         self.handle = h.value
+        self._info(dims)
+
+    def _info(self, dims):
         nm, nnz, kw, npr, tb = ctypes.c_int(), ctypes.c_uint64(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         check(lib().alto_hip_tensor_info(_vp(self.handle), ctypes.byref(nm), ctypes.byref(nnz), ctypes.byref(kw),
                                          ctypes.byref(npr), ctypes.byref(tb)), "tensor_info")
         self.nmode, self.nnz, self.kw, self.nprtn, self.total_bits = nm.value, nnz.value, kw.value, npr.value, tb.value
         self.dims = np.ascontiguousarray(dims, dtype=np.uint64).copy()
+
+    def update(self, dims, cidx, vals, nprtn=1, on_device=False, stream=None):
+        """update_salto (alto.hpp:195-299): rebuild in place from a new COO tensor."""
+        c, keep = _coo(dims, cidx, vals, on_device)
+        check(lib().alto_hip_tensor_update(_vp(self.handle), ctypes.byref(c), ctypes.c_int(nprtn), _vp(stream)),
+              "update_salto")
+        del keep
+        self._info(dims)
 
     def masks(self):
         m = np.zeros((self.nmode, self.kw), dtype=np.uint64)
@@ -200,6 +211,13 @@ class AltoTensor:
         check(lib().alto_hip_tensor_packed_keys(_vp(self.handle), k.ctypes.data_as(_u64p),
                                                 pos.ctypes.data_as(ctypes.POINTER(ctypes.c_int))), "packed_keys")
         return k, pos
+
+    def cr_bits(self):
+        """GENERATE_CR_BITS (alto.hpp:1250-1346): sets the conflict-resolution
+        bits in the keys; returns the per-mode masks, shape (nmode, kw)."""
+        m = np.zeros((len(self.dims), self.kw), dtype=np.uint64)
+        check(lib().alto_hip_tensor_cr_bits(_vp(self.handle), m.ctypes.data_as(_u64p), None), "cr_bits")
+        return m
 
     def vals(self):
         v = np.zeros(self.nnz, dtype=np.float64)

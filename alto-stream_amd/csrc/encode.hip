@@ -173,6 +173,50 @@ __global__ void k_init_intervals(unsigned long long *iv, uint64_t nprtn, int nmo
     }
 }
 
+// Conflict-resolution bits (include/alto.hpp:1250-1346).  Per CR mode c the
+// lowest and highest partition that touches each row; a row is a conflict
+// when they differ (the reference's owner array ends at -1 exactly then).
+struct CrArgs {
+    int n;                  // CR modes
+    int shift[kMaxModes];   // packed field of the mode
+    int width[kMaxModes];
+    int bit[kMaxModes];     // its bit in the key (same position in both layouts)
+    uint64_t off[kMaxModes];  // its rows' offset in the owner arrays
+};
+
+__global__ __launch_bounds__(256) void k_cr_owner(const uint64_t *__restrict__ pkeys, int pkw, uint64_t nnz,
+                                                  uint64_t per, CrArgs a, uint32_t *__restrict__ plo,
+                                                  uint32_t *__restrict__ phi) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnz) return;
+    const uint32_t p = (uint32_t)(i / per);  // partition p holds [p * per, (p + 1) * per)
+    const uint64_t lo = pkeys[i * pkw], hi = pkw == 2 ? pkeys[i * pkw + 1] : 0;
+    for (int c = 0; c < a.n; ++c) {
+        const uint64_t row = a.off[c] + extract_field(lo, hi, a.shift[c], a.width[c]);
+        atomicMin(plo + row, p);
+        atomicMax(phi + row, p);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cr_set(uint64_t *__restrict__ keys, int kw, uint64_t *__restrict__ pkeys,
+                                                int pkw, uint64_t nnz, CrArgs a, const uint32_t *__restrict__ plo,
+                                                const uint32_t *__restrict__ phi) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnz) return;
+    const uint64_t lo = pkeys[i * pkw], hi = pkw == 2 ? pkeys[i * pkw + 1] : 0;
+    uint64_t set[2] = {0, 0};
+    for (int c = 0; c < a.n; ++c) {
+        const uint64_t row = a.off[c] + extract_field(lo, hi, a.shift[c], a.width[c]);
+        if (plo[row] != phi[row]) set[a.bit[c] >> 6] |= 1ULL << (a.bit[c] & 63);
+    }
+    if (set[0] | set[1]) {
+        pkeys[i * pkw] = lo | set[0];
+        if (pkw == 2) pkeys[i * pkw + 1] = hi | set[1];
+        keys[i * kw] |= set[0];
+        if (kw == 2) keys[i * kw + 1] |= set[1];
+    }
+}
+
 __global__ void k_to_f32(float *d, const double *s, uint64_t n) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) d[i] = (float)s[i];
@@ -403,6 +447,7 @@ int build_alto(TensorImpl *t, const alto_hip_coo *coo, int nprtn, int key_words,
     t->nmode = coo->nmode;
     t->nnz = coo->nnz;
     t->nprtn = nprtn;
+    memset(t->cr_masks, 0, sizeof(t->cr_masks));  // update_salto clears them too (alto.hpp:228, 337)
     t->vals32_valid = false;
     for (int m = 0; m < kMaxModes; ++m) {
         t->sched[m].built = false;
@@ -526,11 +571,73 @@ int build_alto(TensorImpl *t, const alto_hip_coo *coo, int nprtn, int key_words,
     return ALTO_OK;
 }
 
+// GENERATE_CR_BITS (include/alto.hpp:1250-1346).  A mode takes a bit when
+// it has more than MIN_NUM_FIBERS = 512 rows, at most MIN_FIBER_REUSE = 4
+// nonzeros per row (integer nnz / dims) and a free key bit remains; the
+// bits are handed out from the lowest free position (total_bits) upwards in
+// mode order, within the key width (64 * kw bits, the reference's LIT).
+// Idempotent: the fields, and with them the result, do not change.
+int apply_cr_bits(TensorImpl *t, hipStream_t s) {
+    if (t->nprtn < 1 || t->prtn_ptr.size() != (size_t)t->nprtn + 1) {
+        set_error("conflict-resolution bits need the reference partitions");
+        return ALTO_ERR_ARG;
+    }
+    constexpr uint64_t kMinNumFibers = 512, kMinFiberReuse = 4;
+    int free_bits = 64 * t->kw - t->total_bits;
+    CrArgs a{};
+    uint64_t rows = 0, cm[kMaxModes * 2] = {};
+    for (int m = 0; m < t->nmode; ++m) {
+        const uint64_t reuse = t->nnz / t->dims[m];
+        if (t->dims[m] > kMinNumFibers && reuse <= kMinFiberReuse && free_bits > 0) {
+            const int bit = 64 * t->kw - free_bits;
+            if (bit >= 64 * t->pkw) {
+                set_error("conflict-resolution bit %d of mode %d lies beyond the %d-bit packed keys", bit, m,
+                          64 * t->pkw);
+                return ALTO_ERR_UNSUPPORTED;
+            }
+            cm[m * t->kw + (bit >> 6)] = 1ULL << (bit & 63);
+            a.shift[a.n] = t->pack.shift[m];
+            a.width[a.n] = t->pack.width[m];
+            a.bit[a.n] = bit;
+            a.off[a.n] = rows;
+            rows += t->dims[m];
+            ++a.n;
+            --free_bits;
+        }
+    }
+    if (a.n == 0 || t->nnz == 0) {
+        memcpy(t->cr_masks, cm, sizeof(cm));
+        return ALTO_OK;
+    }
+    DevBuf own;
+    ALTO_TRY(own.alloc(sizeof(uint32_t) * 2 * rows));
+    uint32_t *plo = own.as<uint32_t>(), *phi = plo + rows;
+    ALTO_HIP_TRY(hipMemsetAsync(plo, 0xFF, sizeof(uint32_t) * rows, s));
+    ALTO_HIP_TRY(hipMemsetAsync(phi, 0, sizeof(uint32_t) * rows, s));
+    const uint64_t per = (t->nnz + t->nprtn - 1) / t->nprtn;
+    k_cr_owner<<<cdiv(t->nnz, 256), 256, 0, s>>>(t->pkeys.as<uint64_t>(), t->pkw, t->nnz, per, a, plo, phi);
+    ALTO_LAUNCH_CHECK();
+    k_cr_set<<<cdiv(t->nnz, 256), 256, 0, s>>>(t->keys.as<uint64_t>(), t->kw, t->pkeys.as<uint64_t>(), t->pkw,
+                                               t->nnz, a, plo, phi);
+    ALTO_LAUNCH_CHECK();
+    ALTO_HIP_TRY(hipStreamSynchronize(s));  // the owner arrays are released on return
+    memcpy(t->cr_masks, cm, sizeof(cm));
+    return ALTO_OK;
+}
+
 }  // namespace alto
 
 using namespace alto;
 
 extern "C" {
+
+int alto_hip_tensor_cr_bits(alto_hip_tensor t, uint64_t *cr_masks, void *stream) {
+    if (!t) return ALTO_ERR_ARG;
+    ALTO_TRY(ensure_init());
+    ALTO_TRY(apply_cr_bits(&t->impl, to_stream(stream)));
+    if (cr_masks) memcpy(cr_masks, t->impl.cr_masks, sizeof(uint64_t) * t->impl.nmode * t->impl.kw);
+    return ALTO_OK;
+}
 
 int alto_hip_tensor_update(alto_hip_tensor t, const alto_hip_coo *coo, int nprtn, void *stream) {
     if (!t || !coo || nprtn < 1) return ALTO_ERR_ARG;

@@ -73,6 +73,9 @@ struct TensorImpl {
 
     std::vector<uint64_t> prtn_ptr;    // nprtn + 1
     std::vector<uint64_t> intervals;   // nprtn * nmode * 2
+    // Conflict-resolution bit of each mode (kw words per mode; zero: none),
+    // set in keys and pkeys by apply_cr_bits, cleared by every rebuild.
+    uint64_t cr_masks[kMaxModes * 2] = {};
 
     ModeSchedule sched[kMaxModes];
     HotModes hot;
@@ -96,6 +99,11 @@ struct TensorImpl {
 // coordinate range check; with both off the call does not synchronise.
 int build_alto(TensorImpl *t, const alto_hip_coo *coo, int nprtn, int key_words, bool check,
                hipStream_t s);
+
+// GENERATE_CR_BITS (include/alto.hpp:1250-1346): picks the modes that take
+// a conflict-resolution bit and sets it in every nonzero whose row of that
+// mode is touched by more than one partition.
+int apply_cr_bits(TensorImpl *t, hipStream_t s);
 
 // Host-side mask construction: LSB_FIRST + SHORT_FIRST packing
 // (include/alto.hpp:954-1057).  Returns total bits or -1 if > 64*kw.

@@ -31,6 +31,18 @@
 typedef unsigned long long IType;
 typedef double FType;
 
+// ALTO key type (include/common.hpp:64-73): 64-bit keys unless
+// ALTO_MASK_LENGTH is 128 (or ALTO_MASK_128 is defined).
+#if (defined(ALTO_MASK_LENGTH) && ALTO_MASK_LENGTH == 128) || defined(ALTO_MASK_128)
+typedef unsigned long long RLIType;
+typedef unsigned __int128 LIType;
+#else
+typedef unsigned long long LIType;
+#endif
+
+// Decomposition models of the command line (include/common.hpp:57).
+typedef enum Model_ { ALS, CPSTREAM, CPSTREAM_ALTO, SPCPSTREAM, SPCPSTREAM_ALTO } Model;
+
 namespace altoamd {
 
 [[noreturn]] inline void fail(const char *what, int st) {
@@ -78,12 +90,16 @@ template <typename LIT> struct AltoTensor {
     int nprtn;
     IType nnz;
     IType *dims;
+    LIT *cr_masks;     // conflict-resolution bit per mode (zero: none)
+    LIT alto_cr_mask;  // their union
 };
 
 template <typename LIT> inline int alto_key_words() { return sizeof(LIT) <= 8 ? 1 : 2; }
 
-// create_alto (alto.hpp:436-582): masks, linearisation, sort, partitions on the GPU.
-template <typename LIT> void create_alto(SparseTensor *spt, AltoTensor<LIT> **at, int nprtn) {
+namespace altoamd {
+// create_alto's body; cr_bits: prtn_alto's conflict-resolution bits
+// (alto.hpp:1250-1346), which the streaming tensor never takes.
+template <typename LIT> void make_alto(SparseTensor *spt, AltoTensor<LIT> **at, int nprtn, bool cr_bits) {
     alto_hip_coo c{spt->nmodes, (const uint64_t *)spt->dims, spt->nnz,
                    (const uint64_t *const *)spt->cidx, spt->vals, 0};
     auto *a = new AltoTensor<LIT>;
@@ -94,12 +110,37 @@ template <typename LIT> void create_alto(SparseTensor *spt, AltoTensor<LIT> **at
     a->nnz = spt->nnz;
     a->dims = (IType *)malloc(sizeof(IType) * a->nmode);
     for (int m = 0; m < a->nmode; ++m) a->dims[m] = spt->dims[m];
+    a->cr_masks = (LIT *)calloc(a->nmode, sizeof(LIT));
+    a->alto_cr_mask = 0;
+    if (cr_bits) {
+        const int kw = alto_key_words<LIT>();
+        std::vector<uint64_t> w((size_t)a->nmode * kw);
+        check(alto_hip_tensor_cr_bits(a->handle, w.data(), nullptr), "create_alto (GENERATE_CR_BITS)");
+        for (int m = 0; m < a->nmode; ++m) {
+            LIT v = (LIT)w[(size_t)m * kw];
+            if (kw == 2) v |= (LIT)w[(size_t)m * kw + 1] << 32 << 32;
+            a->cr_masks[m] = v;
+            a->alto_cr_mask |= v;
+        }
+    }
     *at = a;
+}
+}  // namespace altoamd
+
+// create_alto (alto.hpp:436-582): masks, linearisation, sort, partitions on
+// the GPU; with GENERATE_CR_BITS defined, also the conflict-resolution bits.
+template <typename LIT> void create_alto(SparseTensor *spt, AltoTensor<LIT> **at, int nprtn) {
+#ifdef GENERATE_CR_BITS
+    altoamd::make_alto(spt, at, nprtn, true);
+#else
+    altoamd::make_alto(spt, at, nprtn, false);
+#endif
 }
 
 template <typename LIT> void destroy_alto(AltoTensor<LIT> *at) {
     alto_hip_tensor_destroy(at->handle);
     free(at->dims);
+    free(at->cr_masks);
     delete at;
 }
 
@@ -109,7 +150,7 @@ template <typename LIT> void destroy_alto(AltoTensor<LIT> *at) {
 template <typename LIT>
 void init_salto(SparseTensor *spt, AltoTensor<LIT> **at, int max_nprtn, int stream_mode) {
     (void)stream_mode;
-    create_alto(spt, at, max_nprtn > 0 ? max_nprtn : 1);
+    altoamd::make_alto(spt, at, max_nprtn > 0 ? max_nprtn : 1, false);
 }
 
 // update_salto (alto.hpp:195-299): rebuilds the tensor from a new slice,
@@ -123,6 +164,9 @@ template <typename LIT> void update_salto(SparseTensor *spt, AltoTensor<LIT> *at
     at->nnz = spt->nnz;
     at->dims = (IType *)realloc(at->dims, sizeof(IType) * at->nmode);
     for (int m = 0; m < at->nmode; ++m) at->dims[m] = spt->dims[m];
+    free(at->cr_masks);  // the rebuild clears the conflict-resolution bits (alto.hpp:337)
+    at->cr_masks = (LIT *)calloc(at->nmode, sizeof(LIT));
+    at->alto_cr_mask = 0;
 }
 
 // create_da_mem / destroy_da_mem (alto.hpp:606-679): the reference's

@@ -23,13 +23,7 @@
 #include <string>
 #include <vector>
 
-#include "alto_amd.hpp"
-
-#ifdef ALTO_MASK_128
-typedef unsigned __int128 LIType;
-#else
-typedef unsigned long long LIType;
-#endif
+#include "alto_amd.hpp"  // LIType: 128-bit keys with ALTO_MASK_128
 
 static const struct option long_opt[] = {
     {"help", 0, NULL, 'h'},          {"version", 0, NULL, 'v'},      {"input", 1, NULL, 'i'},

@@ -111,6 +111,19 @@ int alto_hip_tensor_vals(alto_hip_tensor t, double *vals);
  * pos: nmode ints (may be NULL). */
 int alto_hip_tensor_packed_keys(alto_hip_tensor t, uint64_t *keys, int *pos);
 int alto_hip_tensor_perm(alto_hip_tensor t, uint32_t *perm);
+/* GENERATE_CR_BITS (include/alto.hpp:1250-1346, a build option of
+ * create_alto, off by default at alto.hpp:127): a mode with more than 512
+ * rows and at most 4 nonzeros per row (integer nnz / dims) takes the next
+ * free key bit, from bit total_bits upwards in mode order while the key
+ * (64 * key_words bits) has one free; every nonzero whose row of that mode
+ * is touched by more than one partition gets the bit, in the keys of both
+ * layouts (alto_hip_tensor_keys, alto_hip_tensor_packed_keys).  cr_masks
+ * (nmode * key_words words, may be NULL) receives each mode's bit (zero:
+ * none).  The MTTKRP ignores the bits (its schedules need no atomics per
+ * nonzero); alto_hip_tensor_update clears them, as update_salto does.
+ * ALTO_ERR_UNSUPPORTED when a bit would lie beyond the packed keys (forced
+ * two-word keys over at most 64 used bits). */
+int alto_hip_tensor_cr_bits(alto_hip_tensor t, uint64_t *cr_masks, void *stream);
 int alto_hip_tensor_partitions(alto_hip_tensor t, uint64_t *prtn_ptr,
                                uint64_t *intervals);
 /* Device pointers of the sorted keys / FP64 values (for device-side users). */

@@ -229,6 +229,54 @@ void orc_prtn_alto(int nmode, const uint64_t *dims, uint64_t nnz, int kw,
     }
 }
 
+/* alto.hpp:1250-1346 GENERATE_CR_BITS (at the end of prtn_alto).  Free bits
+ * = 64*kw - alto_bits (alto_bits = popcount of all masks).  Mode n takes the
+ * bit 1 << (64*kw - free_bits) when dims[n] > MIN_NUM_FIBERS (512),
+ * nnz / dims[n] <= MIN_FIBER_REUSE (4) and free_bits > 0, then free_bits--.
+ * Per such mode an owner array over its rows: 0 untouched, p+1 touched only
+ * by partition p, -1 touched by more than one (the reference's CAS loop
+ * reaches the same state in any order); every nonzero whose row owner is -1
+ * gets the mode's bit OR'ed into its key.  keys (nnz x kw) are updated in
+ * place; cr_masks (nmode x kw words) receive each mode's bit (zero: none).
+ * Returns the number of modes with a bit. */
+int orc_cr_bits(int nmode, const uint64_t *dims, uint64_t nnz, int kw, uint64_t *keys,
+                const uint64_t *masks, int nprtn, const uint64_t *prtn_ptr, uint64_t *cr_masks)
+{
+    int alto_bits = 0;
+    for (int n = 0; n < nmode; ++n)
+        for (int w = 0; w < kw; ++w) alto_bits += __builtin_popcountll(masks[(size_t)n * kw + w]);
+    int free_bits = 64 * kw - alto_bits, ncr = 0;
+    memset(cr_masks, 0, sizeof(uint64_t) * (size_t)nmode * kw);
+    for (int n = 0; n < nmode; ++n) {
+        uint64_t reuse = nnz / dims[n];
+        if (dims[n] > 512 && reuse <= 4 && free_bits > 0) {
+            int bit = 64 * kw - free_bits;
+            cr_masks[(size_t)n * kw + bit / 64] = 1ULL << (bit % 64);
+            free_bits--;
+            ncr++;
+        }
+    }
+    for (int n = 0; n < nmode; ++n) {
+        int has = 0;
+        for (int w = 0; w < kw; ++w) has |= cr_masks[(size_t)n * kw + w] != 0;
+        if (!has) continue;
+        int *owner = (int *)calloc(dims[n], sizeof(int));
+        for (int p = 0; p < nprtn; ++p)
+            for (uint64_t i = prtn_ptr[p]; i < prtn_ptr[p + 1]; ++i) {
+                uint64_t row = orc_pext(keys + i * kw, masks + (size_t)n * kw, kw);
+                if (owner[row] == 0) owner[row] = p + 1;
+                else if (owner[row] != p + 1) owner[row] = -1;
+            }
+        for (uint64_t i = 0; i < nnz; ++i) {
+            uint64_t row = orc_pext(keys + i * kw, masks + (size_t)n * kw, kw);
+            if (owner[row] == -1)
+                for (int w = 0; w < kw; ++w) keys[i * kw + w] |= cr_masks[(size_t)n * kw + w];
+        }
+        free(owner);
+    }
+    return ncr;
+}
+
 /* util.hpp:30-48 get_num_ptrn (partition count used by CP-stream). */
 int orc_get_num_ptrn(uint64_t nnz, int max_threads)
 {

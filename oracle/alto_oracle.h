@@ -42,6 +42,8 @@ void orc_sort_alto(uint64_t nnz, int kw, uint64_t *keys, double *vals,
 void orc_prtn_alto(int nmode, const uint64_t *dims, uint64_t nnz, int kw,
                    const uint64_t *keys, const uint64_t *masks, int nprtn,
                    uint64_t *prtn_ptr, uint64_t *intervals);
+int orc_cr_bits(int nmode, const uint64_t *dims, uint64_t nnz, int kw, uint64_t *keys,
+                const uint64_t *masks, int nprtn, const uint64_t *prtn_ptr, uint64_t *cr_masks);
 int orc_get_num_ptrn(uint64_t nnz, int max_threads);
 
 void orc_mttkrp_coo(int nmode, uint64_t nnz, const uint64_t *const *cidx,

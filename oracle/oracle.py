@@ -48,6 +48,7 @@ def lib():
         L.orc_admm_cf.restype = ctypes.c_double
         L.orc_cpstream.restype = ctypes.c_int
         L.orc_get_threads.restype = ctypes.c_int
+        L.orc_cr_bits.restype = ctypes.c_int
     return _lib
 
 
@@ -158,6 +159,20 @@ def prtn_alto(dims, keys, masks, nprtn):
                         _p64(keys), _p64(np.ascontiguousarray(masks)), ctypes.c_int(nprtn),
                         _p64(ptr), _p64(iv))
     return ptr, iv
+
+
+def cr_bits(dims, keys, masks, nprtn, prtn_ptr):
+    """orc_cr_bits: GENERATE_CR_BITS (alto.hpp:1250-1346) on sorted keys
+    (nnz, kw) and their partitions.  Returns (keys with the conflict bits,
+    cr_masks (nmode, kw))."""
+    dims = np.ascontiguousarray(dims, dtype=np.uint64)
+    keys = np.ascontiguousarray(keys, dtype=np.uint64).copy()
+    nnz, kw = keys.shape
+    cm = np.zeros((len(dims), kw), dtype=np.uint64)
+    lib().orc_cr_bits(ctypes.c_int(len(dims)), _p64(dims), ctypes.c_uint64(nnz), ctypes.c_int(kw), _p64(keys),
+                      _p64(np.ascontiguousarray(masks, dtype=np.uint64)), ctypes.c_int(nprtn),
+                      _p64(np.ascontiguousarray(prtn_ptr, dtype=np.uint64)), _p64(cm))
+    return keys, cm
 
 
 def get_num_ptrn(nnz, max_threads):

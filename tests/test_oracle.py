@@ -152,6 +152,92 @@ def test_partition_intervals_brute_force():
             assert at.intervals[p, n].tolist() == [min(idx), max(idx)]
 
 
+# ---------- conflict-resolution bits (GENERATE_CR_BITS, alto.hpp:1250-1346) ----------
+
+def _cr_case(dims, nnz, seed, hot_rows=None):
+    """Uniform coordinates; with hot_rows, mode 0 draws from that many rows
+    only, so that rows recur across partitions."""
+    dims, cidx, vals = rand_tensor(dims, nnz, seed=seed)
+    if hot_rows:
+        rng = np.random.default_rng(seed + 1)
+        cidx[0] = rng.integers(0, hot_rows, size=nnz).astype(np.uint64)
+    return dims, cidx, vals
+
+
+def _cr_brute(dims, cidx, perm, ptr, cr_masks):
+    """Independent statement: a nonzero gets mode n's bit iff the rows of
+    mode n it lies in is held by nonzeros of two or more partitions."""
+    nnz = len(perm)
+    part = np.searchsorted(np.asarray(ptr[1:], dtype=np.int64), np.arange(nnz), side="right")
+    extra = np.zeros((nnz, cr_masks.shape[1]), dtype=np.uint64)
+    for n in range(len(dims)):
+        if not cr_masks[n].any():
+            continue
+        rows = np.asarray(cidx[n], dtype=np.int64)[np.asarray(perm, dtype=np.int64)]
+        owners = {}
+        for r, p in zip(rows.tolist(), part.tolist()):
+            owners.setdefault(r, set()).add(p)
+        hit = np.array([len(owners[r]) > 1 for r in rows.tolist()])
+        extra[hit] |= cr_masks[n]
+    return extra
+
+
+@pytest.mark.parametrize("nprtn", [1, 2, 7])
+def test_cr_bits_mode_choice_and_brute_force(nprtn):
+    # bits: 11 + 10 + 10 + 3 = 34; nnz 3000: reuse 1, 4, 5, 600; mode 3 has
+    # 5 <= 512 rows.  So mode 0 takes bit 34 and mode 1 bit 35 (reuse 4 is
+    # still <= MIN_FIBER_REUSE); mode 2 (reuse 5) and mode 3 take none.
+    dims, cidx, vals = _cr_case([2000, 700, 600, 5], 3000, seed=40, hot_rows=120)
+    at = O.AltoTensor(dims, cidx, vals, nprtn=nprtn)
+    keys, cm = O.cr_bits(dims, at.keys, at.masks, nprtn, at.prtn_ptr)
+    assert cm[:, 0].tolist() == [1 << 34, 1 << 35, 0, 0]
+    extra = _cr_brute(dims, cidx, at.perm, at.prtn_ptr, cm)
+    assert np.array_equal(keys, at.keys | extra)
+    assert np.array_equal(keys & ~np.uint64((1 << 34) | (1 << 35)), at.keys)
+    if nprtn == 1:
+        assert not extra.any()
+    else:  # conflicts occur, and mode 1 (uniform rows) also has rows held by one partition
+        col0 = (extra[:, 0] >> np.uint64(34)) & np.uint64(1)
+        col1 = (extra[:, 0] >> np.uint64(35)) & np.uint64(1)
+        assert int(col0.sum()) > 0
+        assert 0 < int(col1.sum()) < len(col1)
+
+
+def test_cr_bits_thresholds_are_strict_and_inclusive():
+    # dims 512 is not > MIN_NUM_FIBERS; 513 is.  nnz 2565: reuse 5 for 513
+    # rows (no bit), so use nnz 2052: reuse 4 (bit).
+    dims, cidx, vals = _cr_case([512, 513, 4], 2052, seed=41)
+    at = O.AltoTensor(dims, cidx, vals, nprtn=3)
+    _, cm = O.cr_bits(dims, at.keys, at.masks, 3, at.prtn_ptr)
+    assert cm[:, 0].tolist() == [0, 1 << 21, 0]
+    dims, cidx, vals = _cr_case([512, 513, 4], 2565, seed=41)
+    at = O.AltoTensor(dims, cidx, vals, nprtn=3)
+    _, cm = O.cr_bits(dims, at.keys, at.masks, 3, at.prtn_ptr)
+    assert not cm.any()
+
+
+def test_cr_bits_run_out_of_free_bits():
+    # 21 + 21 + 21 = 63 bits: one free bit, which mode 0 takes (bit 63)
+    dims, cidx, vals = _cr_case([1 << 21, 1 << 21, 1 << 21], 400, seed=42, hot_rows=30)
+    at = O.AltoTensor(dims, cidx, vals, nprtn=4)
+    keys, cm = O.cr_bits(dims, at.keys, at.masks, 4, at.prtn_ptr)
+    assert cm[:, 0].tolist() == [1 << 63, 0, 0]
+    assert np.array_equal(keys, at.keys | _cr_brute(dims, cidx, at.perm, at.prtn_ptr, cm))
+
+
+def test_cr_bits_128_bit_keys():
+    # 30 + 30 + 10 = 70 bits: two-word keys; all three modes qualify and take
+    # bits 70, 71, 72 (word 1, bits 6, 7, 8)
+    dims, cidx, vals = _cr_case([1 << 30, 1 << 30, 1000], 500, seed=43, hot_rows=40)
+    at = O.AltoTensor(dims, cidx, vals, nprtn=5)
+    assert at.kw == 2
+    keys, cm = O.cr_bits(dims, at.keys, at.masks, 5, at.prtn_ptr)
+    assert cm.tolist() == [[0, 1 << 6], [0, 1 << 7], [0, 1 << 8]]
+    extra = _cr_brute(dims, cidx, at.perm, at.prtn_ptr, cm)
+    assert extra[:, 1].any()
+    assert np.array_equal(keys, at.keys | extra)
+
+
 def test_get_num_ptrn():
     # util.hpp:30-48
     assert O.get_num_ptrn(5, 8) == 1
