@@ -1291,17 +1291,21 @@ __global__ __launch_bounds__(256) void k_sumsq_part(const double *__restrict__ v
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
-__global__ void k_sum_final(const double *__restrict__ part, int n, double *__restrict__ out) {
-    if (threadIdx.x == 0) {
-        double s = 0.0;
-        for (int b = 0; b < n; ++b) s += part[b];
-        *out = s;
-    }
+// One wave: lane j adds part[j], part[j + 64], ... in order, then a fixed
+// shuffle tree (the loads of all lanes are in flight together; a single
+// thread walking 1024 partials took ~57 us).
+__global__ __launch_bounds__(64) void k_sum_final(const double *__restrict__ part, int n, double *__restrict__ out) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < n; b += 64) s += part[b];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if (threadIdx.x == 0) *out = s;
 }
 
 int launch_sumsq(const double *v, uint64_t n, double *out, double *part, hipStream_t s) {
-    k_sumsq_part<<<kSumsqBlocks, 256, 0, s>>>(v, n, part);
-    k_sum_final<<<1, 64, 0, s>>>(part, kSumsqBlocks, out);
+    // about 4096 values per block: a small tensor takes a few blocks
+    const int nb = (int)std::min<uint64_t>(kSumsqBlocks, std::max<uint64_t>(1, cdiv(n, 4096)));
+    k_sumsq_part<<<nb, 256, 0, s>>>(v, n, part);
+    k_sum_final<<<1, 64, 0, s>>>(part, nb, out);
     ALTO_LAUNCH_CHECK();
     return ALTO_OK;
 }
