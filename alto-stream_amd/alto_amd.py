@@ -573,7 +573,8 @@ def cpd_error_noalloc(dims, cidx_dev, vals_dev, factors_dev, out_dev, stream=Non
 
 def gather_probe(rows, gathers=1 << 25, dist=0, zipf_s=1.1, layout=0, seed=1, stream=None):
     """ms per launch of the 128-byte row gather probe over a table of `rows`
-    rows: dist 0 uniform, 1 Zipf(zipf_s); layout 0 = 8 lanes x 16 B, 1 = 4
+    rows: dist 0 uniform, 1 Zipf(zipf_s) over a permutation, 2 Zipf(zipf_s)
+    with the popular rows first; layout 0 = 8 lanes x 16 B, 1 = 4
     lanes x 2 x 16 B per row."""
     ms = ctypes.c_float()
     check(lib().alto_hip_gather_probe(ctypes.c_uint64(rows), ctypes.c_uint64(gathers), ctypes.c_int(dist),

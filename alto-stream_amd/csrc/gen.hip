@@ -226,8 +226,8 @@ extern "C" int alto_hip_generate(int nmode, const uint64_t *dims, uint64_t nnz, 
 extern "C" int alto_hip_gather_probe(uint64_t rows, uint64_t gathers, int dist, double zipf_s,
                                      int layout, uint64_t seed, float *ms, void *stream) {
     using namespace alto;
-    if (!ms || rows < 1 || rows >= (1ULL << 32) || gathers < 64 || (dist != 0 && dist != 1) ||
-        (layout != 0 && layout != 1) || (dist == 1 && !(zipf_s > 0.0)))
+    if (!ms || rows < 1 || rows >= (1ULL << 32) || gathers < 64 || dist < 0 || dist > 2 ||
+        (layout != 0 && layout != 1) || (dist >= 1 && !(zipf_s > 0.0)))
         return ALTO_ERR_ARG;
     ALTO_TRY(ensure_init());
     hipStream_t s = to_stream(stream);
@@ -240,7 +240,7 @@ extern "C" int alto_hip_gather_probe(uint64_t rows, uint64_t gathers, int dist, 
     ALTO_TRY(idx.alloc(sizeof(uint32_t) * n));
     ALTO_TRY(out.alloc(sizeof(float) * (size_t)blocks * 256));
     ALTO_HIP_TRY(hipMemsetAsync(tab.p, 0, rows * 128, s));
-    if (dist == 1) {
+    if (dist >= 1) {
         std::vector<double> c(rows);
         double acc = 0.0;
         for (uint64_t k = 0; k < rows; ++k) {
@@ -251,7 +251,8 @@ extern "C" int alto_hip_gather_probe(uint64_t rows, uint64_t gathers, int dist, 
         c.back() = 2.0;
         std::vector<uint32_t> pm(rows);
         std::iota(pm.begin(), pm.end(), 0u);
-        for (uint64_t k = rows - 1; k > 0; --k) std::swap(pm[k], pm[rnd(seed, 301, k) % (k + 1)]);
+        if (dist == 1)  // popular rows scattered over the table; dist 2 keeps them first
+            for (uint64_t k = rows - 1; k > 0; --k) std::swap(pm[k], pm[rnd(seed, 301, k) % (k + 1)]);
         ALTO_TRY(cdf.alloc(sizeof(double) * rows));
         ALTO_TRY(perm.alloc(sizeof(uint32_t) * rows));
         ALTO_HIP_TRY(hipMemcpyAsync(cdf.p, c.data(), sizeof(double) * rows, hipMemcpyHostToDevice, s));

@@ -375,8 +375,9 @@ int alto_hip_generate(int nmode, const uint64_t *dims, uint64_t nnz, uint64_t se
 /* ---- gather-rate probe (no reference counterpart: roofline calibration) ----
  * Average ms of one launch gathering `gathers` 128-byte rows of a device
  * table of `rows` rows, the access pattern of the MTTKRP's factor-row
- * gathers: row indices from a coalesced 4-byte stream, uniform (dist 0) or
- * Zipf(zipf_s) over a seeded permutation (dist 1); layout 0 = 8 lanes x
+ * gathers: row indices from a coalesced 4-byte stream, uniform (dist 0),
+ * Zipf(zipf_s) over a seeded permutation (dist 1) or Zipf(zipf_s) with the
+ * popular rows first (dist 2: a table relabelled by popularity); layout 0 = 8 lanes x
  * 16 B per row, layout 1 = 4 lanes x 2 x 16 B (the fiber kernel's FP32
  * rank-32 row). */
 int alto_hip_gather_probe(uint64_t rows, uint64_t gathers, int dist, double zipf_s, int layout,
