@@ -556,7 +556,7 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
                 cp[m] = m == s ? zidx.as<uint64_t>() : dci[m].as<uint64_t>() + sl.start;
             alto_hip_coo sc{N, bd, sl.end - sl.start, cp, dvals.as<double>() + sl.start, 1};
             ALTO_HIP_TRY(hipEventRecord(ev[2], st));
-            ALTO_TRY(build_alto(&at, &sc, 0, 0, false, st));
+            ALTO_TRY(build_alto(&at, &sc, 0, 0, false, st, false));  // only the schedules are read
             ALTO_HIP_TRY(hipEventRecord(ev[3], st));
         }
         if (k == 0) {  // cpstream_alto_iter, iter == 0 (streaming_cpd.cpp:1248-1261)

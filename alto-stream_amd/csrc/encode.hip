@@ -368,6 +368,7 @@ int ensure_schedule(TensorImpl *t, int mode, bool need64, hipStream_t s) {
     const int pkw = t->pkw;
     fiber_levels(t, mode, sc.lvl);
     sc.n = 0;
+    sc.neff = -1;  // set below; an empty schedule is never copied
     if (n == 0) {
         ALTO_TRY(sc.pkeys.alloc(sizeof(uint64_t) * pkw));
         ALTO_TRY(sc.vals32.alloc(sizeof(float)));
@@ -380,6 +381,31 @@ int ensure_schedule(TensorImpl *t, int mode, bool need64, hipStream_t s) {
     ord.n = t->nmode;
     ord.mode[0] = mode;
     for (int q = 0; q < t->nmode - 1; ++q) ord.mode[q + 1] = sc.lvl[q];
+    sc.neff = 0;
+    for (int q = 0; q < ord.n; ++q)
+        if (t->pack.width[ord.mode[q]] > 0) sc.eff[sc.neff++] = ord.mode[q];
+    // A mode of size one adds no sort field: a mode whose remaining fields
+    // come in the same order as those of a built schedule (the stream mode of
+    // a one-step time slice and the smallest other mode) gets a copy of it.
+    for (int m2 = 0; m2 < t->nmode; ++m2) {
+        const ModeSchedule &o = t->sched[m2];
+        if (m2 == mode || !o.built || (need64 && !o.has64) || o.neff != sc.neff ||
+            memcmp(o.eff, sc.eff, sizeof(int) * sc.neff) != 0)
+            continue;
+        const uint64_t m = o.n;
+        ALTO_TRY(sc.pkeys.alloc(sizeof(uint64_t) * pkw * std::max<uint64_t>(m, 1)));
+        ALTO_TRY(sc.vals32.alloc(sizeof(float) * std::max<uint64_t>(m, 1)));
+        ALTO_HIP_TRY(hipMemcpyAsync(sc.pkeys.p, o.pkeys.p, sizeof(uint64_t) * pkw * m, hipMemcpyDeviceToDevice, s));
+        ALTO_HIP_TRY(hipMemcpyAsync(sc.vals32.p, o.vals32.p, sizeof(float) * m, hipMemcpyDeviceToDevice, s));
+        if (need64) {
+            ALTO_TRY(sc.vals64.alloc(sizeof(double) * std::max<uint64_t>(m, 1)));
+            ALTO_HIP_TRY(hipMemcpyAsync(sc.vals64.p, o.vals64.p, sizeof(double) * m, hipMemcpyDeviceToDevice, s));
+        }
+        sc.n = m;
+        sc.built = true;
+        sc.has64 = need64;
+        return ALTO_OK;
+    }
     const bool wide = t->total_bits > 64;
     DevBuf lk0, lk1, lperm, lsk, lsv, llongs;
     const bool keep = t->keep_sched_scratch;
@@ -435,9 +461,9 @@ int ensure_schedule(TensorImpl *t, int mode, bool need64, hipStream_t s) {
 
 
 int build_alto(TensorImpl *t, const alto_hip_coo *coo, int nprtn, int key_words, bool check,
-               hipStream_t s) {
+               hipStream_t s, bool alto_order) {
     if (!coo || !coo->dims || !coo->cidx || !coo->vals) return ALTO_ERR_ARG;
-    if (coo->nmode < 1 || coo->nmode > kMaxModes || nprtn < 0) return ALTO_ERR_ARG;
+    if (coo->nmode < 1 || coo->nmode > kMaxModes || nprtn < 0 || (!alto_order && nprtn != 0)) return ALTO_ERR_ARG;
     if (coo->nnz >= (1ULL << 32)) {
         set_error("nnz %llu exceeds 2^32 - 1", (unsigned long long)coo->nnz);
         return ALTO_ERR_UNSUPPORTED;
@@ -507,8 +533,9 @@ int build_alto(TensorImpl *t, const alto_hip_coo *coo, int nprtn, int key_words,
                                                  t->bad.as<uint32_t>());
         ALTO_LAUNCH_CHECK();
         ALTO_TRY(iota_u32(t->perm.as<uint32_t>(), n, s));
-        ALTO_TRY(radix_sort_pairs(t->k0.as<uint64_t>(), kw == 2 ? t->k1.as<uint64_t>() : nullptr,
-                                  t->perm.as<uint32_t>(), n, total, t->sort_ws, s));
+        if (alto_order)
+            ALTO_TRY(radix_sort_pairs(t->k0.as<uint64_t>(), kw == 2 ? t->k1.as<uint64_t>() : nullptr,
+                                      t->perm.as<uint32_t>(), n, total, t->sort_ws, s));
         k_finish<<<cdiv(n, 256), 256, 0, s>>>(t->k0.as<uint64_t>(),
                                               kw == 2 ? t->k1.as<uint64_t>() : nullptr,
                                               t->perm.as<uint32_t>(), vin, n, t->bp, t->pack, kw,

@@ -436,7 +436,7 @@ extern "C" int alto_hip_spcpstream(const alto_hip_coo *coo, int s, int R, int ma
             const uint64_t *cp[kMaxModes];
             for (int m = 0; m < N; ++m) cp[m] = m == s ? zidx.as<uint64_t>() : cco[m].as<uint64_t>();
             alto_hip_coo sc{N, nnzr, bn, cp, dvals.as<double>() + sl.start, 1};
-            ALTO_TRY(build_alto(&at, &sc, 0, 0, false, st));
+            ALTO_TRY(build_alto(&at, &sc, 0, 0, false, st, false));  // only the schedules are read
         }
         ALTO_HIP_TRY(hipEventRecord(ev[3], st));
         // ---- row-sparse matrices and the c / h Grams (streaming_cpd.cpp:607-656) ----

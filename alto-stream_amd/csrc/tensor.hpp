@@ -46,6 +46,10 @@ struct ModeSchedule {
     bool built = false;
     bool has64 = false;
     int lvl[kMaxModes] = {};  // fiber levels: the other modes, shallow to leaf
+    // The sort fields: the target and the levels that have a nonzero width.
+    // Two modes with the same sequence have the same schedule.
+    int eff[kMaxModes] = {};
+    int neff = 0;
     uint64_t n = 0;           // entries (distinct coordinates)
     DevBuf pkeys;   // packed keys in row order
     DevBuf vals32;  // FP32 values in row order
@@ -97,8 +101,13 @@ struct TensorImpl {
 // counterpart of init_salto + update_salto, include/alto.hpp:151-299).
 // nprtn == 0 skips the reference partitions; check == false skips the
 // coordinate range check; with both off the call does not synchronise.
+// alto_order == false (only with nprtn == 0) keeps the nonzeros in input
+// order instead of sorting them by ALTO key: for callers that use only the
+// per-mode schedules, which come out the same either way (their stable sort
+// is on every coordinate field, and equal coordinates keep input order in
+// the ALTO sort too).
 int build_alto(TensorImpl *t, const alto_hip_coo *coo, int nprtn, int key_words, bool check,
-               hipStream_t s);
+               hipStream_t s, bool alto_order = true);
 
 // GENERATE_CR_BITS (include/alto.hpp:1250-1346): picks the modes that take
 // a conflict-resolution bit and sets it in every nonzero whose row of that

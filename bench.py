@@ -95,6 +95,8 @@ def main():
                     help="skip the other workloads (plain CP-ALS, 5-mode BF16, CP-stream sample)")
     ap.add_argument("--stream-slices", type=int, default=1000, help="CP-stream: number of time slices (of 1000)")
     ap.add_argument("--batch-tensors", type=int, default=512, help="batched CP-ALS: number of tensors")
+    ap.add_argument("--batch-ranks", default="8,16,32",
+                    help="batched CP-ALS, with --leg: run only the tensors of these ranks (same tensors)")
     ap.add_argument("--probe", action="store_true",
                     help="random 128-byte row gather rate vs table size (the MTTKRP's miss-service bound)")
     ap.add_argument("--leg", choices=["plain_als", "bf16_5mode", "cp_stream", "batched_als"],
@@ -113,7 +115,8 @@ def main():
     if args.leg:
         legs = {"plain_als": lambda: leg_plain_als(A), "bf16_5mode": lambda: leg_bf16_5mode(A),
                 "cp_stream": lambda: leg_cp_stream(A, args.stream_slices),
-                "batched_als": lambda: leg_batched_als(A, args.batch_tensors)}
+                "batched_als": lambda: leg_batched_als(A, args.batch_tensors,
+                                                       [int(r) for r in args.batch_ranks.split(",")])}
         print(json.dumps(legs[args.leg](), default=_json_scalar), flush=True)
         return
     if args.probe:
@@ -634,7 +637,7 @@ def leg_cp_stream(A, slices):
     return out
 
 
-def leg_batched_als(A, ntensors):
+def leg_batched_als(A, ntensors, only_ranks=(8, 16, 32)):
     """Batched CP-ALS of uneven instances: 3-mode tensors, each mode size
     uniform in [500, 50000], nnz log-uniform in [1e4, 5e6], uniform
     coordinates, U(0,1) values; ranks 8/16/32 round robin; 25 iterations
@@ -651,17 +654,21 @@ def leg_batched_als(A, ntensors):
         dims = [int(x) for x in rng.integers(500, 50001, size=3)]
         nnz = int(round(10 ** rng.uniform(4, np.log10(5e6))))
         R = (8, 16, 32)[k % 3]
+        F = [rng.random((d, R)) for d in dims]  # drawn for every tensor: the same models whatever is run
+        if R not in only_ranks:
+            continue
         cidx, vals = A.generate(dims, nnz, seed + k, [0, 0, 0], None, 0)
         ats.append(A.create_alto(dims, cidx, vals, nprtn=1, on_device=True))
         A.synchronize()
         for c in cidx:
             c.close()
         vals.close()
-        Fs.append([rng.random((d, R)) for d in dims])
+        Fs.append(F)
         ranks.append(R)
         total += nnz
     out = {"workload": "%d 3-mode tensors, modes U[500, 50000], nnz log-uniform [1e4, 5e6] (%d total), "
-                       "ranks 8/16/32, 25 iterations, FP32 MTTKRP + FP64 solve" % (ntensors, total)}
+                       "ranks %s, 25 iterations, FP32 MTTKRP + FP64 solve"
+                       % (len(ats), total, "/".join(str(r) for r in sorted(set(ranks))))}
     def dev_models():
         return [[A.DeviceArray.from_numpy(f) for f in fk] for fk in Fs]
 
@@ -690,7 +697,7 @@ def leg_batched_als(A, ntensors):
     for a in ats:
         a.close()
     log("batched CP-ALS: %d tensors (%d nnz): %.3f s with models in HBM, %.3f s with host models" %
-        (ntensors, total, wall, wall_h))
+        (len(ats), total, wall, wall_h))
     return out
 
 if __name__ == "__main__":
