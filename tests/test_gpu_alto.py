@@ -95,6 +95,38 @@ def test_forced_128bit_key_matches_oracle(gpu):
     np.testing.assert_array_equal(at.vals(), ref.vals)
 
 
+def test_empty_tensor(gpu):
+    """No nonzeros: masks as for any tensor of these dims, no keys, every
+    partition empty ([0, 0] intervals, alto.hpp:1220-1225), the MTTKRP adds
+    nothing (out += 0) in every mode and precision, and the conflict bits'
+    masks follow the mode rule with no key to mark."""
+    dims = [600, 700, 8]
+    cidx = [np.zeros(0, np.uint64) for _ in dims]
+    vals = np.zeros(0)
+    ref = O.AltoTensor(dims, cidx, vals, nprtn=4)
+    at = gpu.create_alto(dims, cidx, vals, nprtn=4)
+    assert at.nnz == 0
+    np.testing.assert_array_equal(at.masks(), ref.masks)
+    assert at.keys().shape == (0, at.kw)
+    ptr, iv = at.partitions()
+    np.testing.assert_array_equal(ptr, ref.prtn_ptr)
+    np.testing.assert_array_equal(iv, ref.intervals)
+    assert not ptr.any() and not iv.any()
+    R = 16
+    F = rand_factors(dims, R, seed=7)
+    for prec in (gpu.PREC_F64, gpu.PREC_F32):
+        devF = [gpu.to_prec(f, prec) for f in F]
+        for algo in (1, 2):
+            for mode in range(3):
+                dt = np.float64 if prec == gpu.PREC_F64 else np.float32
+                out = gpu.DeviceArray.from_numpy(np.ones((dims[mode], R), dt))
+                gpu.mttkrp_alto_par(mode, devF, R, at, out, prec=prec, out_f64=prec == gpu.PREC_F64, algo=algo)
+                assert np.array_equal(out.numpy(), np.ones((dims[mode], R))), (prec, algo, mode)
+    _, want = O.cr_bits(dims, ref.keys, ref.masks, 4, ref.prtn_ptr)
+    np.testing.assert_array_equal(at.cr_bits(), want)
+    at.close()
+
+
 def test_errors_are_status_codes(gpu):
     dims, cidx, vals = rand_tensor([10, 10], 50, seed=37)
     bad = [c.copy() for c in cidx]
