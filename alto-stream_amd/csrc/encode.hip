@@ -616,6 +616,11 @@ int apply_cr_bits(TensorImpl *t, hipStream_t s) {
     for (int m = 0; m < t->nmode; ++m) {
         const uint64_t reuse = t->nnz / t->dims[m];
         if (t->dims[m] > kMinNumFibers && reuse <= kMinFiberReuse && free_bits > 0) {
+            if (t->dims[m] >= (1ULL << 32)) {  // the owner arrays hold one entry per row
+                set_error("conflict-resolution bits: mode %d has %llu rows (at most 2^32 - 1)", m,
+                          (unsigned long long)t->dims[m]);
+                return ALTO_ERR_UNSUPPORTED;
+            }
             const int bit = 64 * t->kw - free_bits;
             if (bit >= 64 * t->pkw) {
                 set_error("conflict-resolution bit %d of mode %d lies beyond the %d-bit packed keys", bit, m,

@@ -63,6 +63,16 @@ def test_cr_bits_beyond_packed_keys_are_refused(gpu):
     at.close()
 
 
+def test_cr_bits_mode_too_long_is_refused(gpu):
+    # a qualifying mode of 2^33 rows: its owner array would need an entry per row
+    dims, cidx, vals = _case([600, 700, 4], 200, 47)
+    dims = [1 << 33, 700, 4]
+    at = gpu.create_alto(dims, cidx, vals, nprtn=2)
+    with pytest.raises(gpu.AltoError, match="unsupported"):
+        at.cr_bits()
+    at.close()
+
+
 @pytest.mark.parametrize("rank", [16, 32])
 def test_mttkrp_ignores_cr_bits(gpu, rank):
     dims, cidx, vals = _case([100000, 80000, 600], 200000, 44)
