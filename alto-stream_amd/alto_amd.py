@@ -542,12 +542,28 @@ def rowsparse_mttkrp(dims, cidx, vals, mode, stream_mode, rank, rsp_vals, ridx, 
     return rowind[:n], out[:n]
 
 
-def cpd_error(dims, cidx, vals, factors, stream=None):
-    """cpd_error (src/streaming_cpd.cpp:368-464) on host arrays: relative
-    residual of the Kruskal model (weights 1).  Returns (err, (||X||^2,
-    ||M||^2, <X, M>))."""
-    c, keep = _coo(dims, cidx, vals, False)
+def cpd_error(dims, cidx, vals, factors, stream=None, on_device=False):
+    """cpd_error (src/streaming_cpd.cpp:368-464): relative residual of the
+    Kruskal model (weights 1).  Host COO and host FP64 factors, or, with
+    on_device, DeviceArrays for both (the C ABI reads every array where
+    coo->on_device says).  Returns (err, (||X||^2, ||M||^2, <X, M>))."""
     N = len(dims)
+    if on_device:
+        if not all(isinstance(f, DeviceArray) for f in factors):
+            raise AltoError("cpd_error(on_device=True) takes DeviceArray factors")
+        R = int(factors[0].shape[1])
+        for m, f in enumerate(factors):
+            if f.shape != (int(dims[m]), R) or f.dtype != np.float64:
+                raise AltoError("factor %d is %s %s, expected (%d, %d) float64" % (m, f.shape, f.dtype,
+                                                                                  int(dims[m]), R))
+        c, keep = _coo(dims, cidx, vals, True)
+        arr = _ptrs(factors)
+        err = ctypes.c_double()
+        parts = np.zeros(3)
+        check(lib().alto_hip_cpd_error(ctypes.byref(c), ctypes.c_int(R), arr, ctypes.byref(err),
+                                       parts.ctypes.data_as(_f64p), _vp(stream)), "cpd_error")
+        return err.value, tuple(parts)
+    c, keep = _coo(dims, cidx, vals, False)
     R = int(np.asarray(factors[0]).shape[1])
     fs = [np.ascontiguousarray(f, dtype=np.float64) for f in factors]
     for m, f in enumerate(fs):

@@ -42,6 +42,11 @@ const char *last_error();
 hipStream_t default_stream();
 inline hipStream_t to_stream(void *s) { return s ? (hipStream_t)s : default_stream(); }
 int ensure_init();
+// A caller's array that kernels will read: device, managed or pinned host
+// memory.  Pageable host memory (or an address the runtime does not know)
+// would fault the GPU when a kernel reads it, so the entry points that take
+// device arrays refuse it with ALTO_ERR_ARG.  Not for graph-capture paths.
+int check_device_access(const void *p, const char *what);
 
 // Graph capture and device memory.  A thread that captures a stream into a
 // graph holds capture_mutex() shared from the start of the capture to the

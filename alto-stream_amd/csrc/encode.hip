@@ -519,7 +519,11 @@ int build_alto(TensorImpl *t, const alto_hip_coo *coo, int nprtn, int key_words,
         if (n) ALTO_HIP_TRY(hipMemcpyAsync(dvals.p, coo->vals, sizeof(double) * n, hipMemcpyHostToDevice, s));
         vin = dvals.as<double>();
     } else {
-        for (int m = 0; m < t->nmode; ++m) ip.p[m] = coo->cidx[m];
+        for (int m = 0; m < t->nmode; ++m) {
+            if (n) ALTO_TRY(check_device_access(coo->cidx[m], "ALTO tensor coordinates"));
+            ip.p[m] = coo->cidx[m];
+        }
+        if (n) ALTO_TRY(check_device_access(coo->vals, "ALTO tensor values"));
     }
     ALTO_TRY(t->k0.alloc(sizeof(uint64_t) * n));
     if (kw == 2) ALTO_TRY(t->k1.alloc(sizeof(uint64_t) * n));

@@ -158,6 +158,13 @@ extern "C" int alto_hip_cpd_error(const alto_hip_coo *coo, int rank, const doubl
     const uint64_t *cp[kMaxModes];
     const double *fp[kMaxModes];
     const double *vp = coo->vals;
+    if (coo->on_device) {
+        for (int m = 0; m < N; ++m) {
+            ALTO_TRY(check_device_access(factors[m], "cpd_error factor"));
+            if (nnz) ALTO_TRY(check_device_access(coo->cidx[m], "cpd_error coordinates"));
+        }
+        if (nnz) ALTO_TRY(check_device_access(coo->vals, "cpd_error values"));
+    }
     for (int m = 0; m < N; ++m) {
         cp[m] = coo->cidx[m];
         fp[m] = factors[m];

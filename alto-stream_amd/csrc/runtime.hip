@@ -43,6 +43,18 @@ int ensure_init() {
 
 hipStream_t default_stream() { return g_stream; }
 
+int check_device_access(const void *p, const char *what) {
+    if (!p) return ALTO_OK;  // empty arrays: the callers check their own nulls
+    hipPointerAttribute_t a{};
+    const hipError_t e = hipPointerGetAttributes(&a, p);
+    if (e != hipSuccess || a.type == hipMemoryTypeUnregistered) {
+        (void)hipGetLastError();  // clear the sticky query error
+        set_error("%s is not memory the GPU can read (pageable host memory?): pass device arrays", what);
+        return ALTO_ERR_ARG;
+    }
+    return ALTO_OK;
+}
+
 __global__ void k_f64_to_f32(float *d, const double *s, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) d[i] = (float)s[i];

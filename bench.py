@@ -663,6 +663,57 @@ def leg_cp_stream(A, slices):
                                      "final_error": fin(un["final_error"])}
     log("CP-stream: spCP-stream %.2f slices/s (%.2f outer iterations), CP-stream unconstrained %.2f slices/s (%.2f)" %
         (slices / ts, np.mean(sp["outer_iters"]), slices / tu, np.mean(un["outer_iters"])))
+    try:
+        exact = stream_exact_errors(A, dims, hc, hv, {"cpstream_nonneg": res, "spcpstream": sp,
+                                                      "cpstream_unconstrained": un})
+        out["final_error_exact"] = exact["cpstream_nonneg"]
+        out["spcpstream"]["final_error_exact"] = exact["spcpstream"]
+        out["cpstream_unconstrained"]["final_error_exact"] = exact["cpstream_unconstrained"]
+        out["final_error_exact_note"] = exact["note"]
+    except Exception as e:  # reported, not fatal to the bench line
+        out["final_error_exact"] = None
+        out["final_error_exact_note"] = "not computed: %s" % e
+    return out
+
+
+def stream_exact_errors(A, dims, hc, hv, models):
+    """||X - M|| / ||X|| of each streamed model over the processed stream with
+    duplicate coordinates merged (||X||^2 = sum over distinct coordinates of
+    the squared summed values), in the relabelled index space the closing
+    cpd_error uses (the time factor = the time rows).  <X, M> and ||M||^2 come
+    from alto_hip_cpd_error, which is linear in the values; only ||X||^2 needs
+    the merge: equal coordinates have equal ALTO keys, which sort adjacent."""
+    sc, sv, _, _ = A.stream_prepare(dims, hc, hv, 0)
+    res0 = models["cpstream_nonneg"]
+    rel_dims = [len(res0["time_rows"])] + [int(d) for d in res0["dims"][1:]]
+    for m in range(1, len(dims)):
+        rel_dims[m] = max(rel_dims[m], int(sc[m].max()) + 1)
+    dc = [A.DeviceArray.from_numpy(c) for c in sc]
+    dv = A.DeviceArray.from_numpy(sv)
+    at = A.create_alto(rel_dims, dc, dv, nprtn=1, on_device=True)
+    keys, vals = at.keys(), at.vals()
+    at.close()
+    diff = keys[1:, 0] != keys[:-1, 0] if keys.shape[1] == 1 else np.any(keys[1:] != keys[:-1], axis=1)
+    heads = np.concatenate([[0], np.flatnonzero(diff) + 1])
+    merged = np.add.reduceat(vals, heads)
+    nx = float(np.dot(merged, merged))
+    del keys, vals, diff, heads
+    out = {"note": ("relative residual over the processed stream with duplicate coordinates merged (%d stored, "
+                    "%d distinct); final_error keeps the reference's ||X||^2 = sum of squared stored values, "
+                    "NaN here" % (len(sv), len(merged)))}
+    for name, r in models.items():
+        fs = [np.asarray(r["time_rows"])] + [np.asarray(f) for f in r["factors"][1:]]
+        fs = [np.vstack([f, np.zeros((rel_dims[m] - f.shape[0], f.shape[1]))]) if f.shape[0] < rel_dims[m] else f
+              for m, f in enumerate(fs)]
+        df = [A.DeviceArray.from_numpy(np.ascontiguousarray(f, dtype=np.float64)) for f in fs]
+        _, (_, nm, inner) = A.cpd_error(rel_dims, dc, dv, df, on_device=True)
+        for f in df:
+            f.close()
+        r2 = nx + nm - 2.0 * inner
+        out[name] = float(np.sqrt(r2) / np.sqrt(nx)) if r2 >= 0 else None
+    for c in dc:
+        c.close()
+    dv.close()
     return out
 
 

@@ -113,3 +113,32 @@ def test_cpstream_dynamic_batches_match_oracle(gpu, thr, lim, nonneg):
     np.testing.assert_allclose(got["time_rows"], ref["time_rows"], rtol=1e-8, atol=1e-10)
     for a, b in zip(got["factors"][1:], ref["factors"][1:]):
         np.testing.assert_allclose(a, b, rtol=1e-8, atol=1e-10)
+
+
+def test_exact_closing_error_with_merged_duplicates(gpu):
+    """The bench's exact closing error of a streamed model (duplicate
+    coordinates merged into one value, which the reference's sum of squared
+    stored values does not do) equals ||X - M||_F / ||X||_F computed densely
+    in numpy from the relabelled stream, the time rows and the factors."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    keep = {k: os.environ.get(k) for k in ("OMP_PROC_BIND", "OMP_PLACES", "OMP_WAIT_POLICY")}
+    import bench  # sets OpenMP defaults for its CPU legs: undone here
+    for k, v in keep.items():
+        if v is None:
+            os.environ.pop(k, None)
+        else:
+            os.environ[k] = v
+    dims, cidx, vals = _stream_tensor(6, [40, 30], 3000, seed=95, zipf=True)
+    init = _init(dims, 4, 96)
+    res = gpu.cpstream(dims, cidx, vals, 0, 4, 5, 1e-5, True, init)
+    got = bench.stream_exact_errors(gpu, dims, cidx, vals, {"cpstream_nonneg": res})
+    sc, sv, _, _ = gpu.stream_prepare(dims, cidx, vals, 0)
+    T, I, J = len(res["time_rows"]), int(res["dims"][1]), int(res["dims"][2])
+    X = np.zeros((T, I, J))
+    np.add.at(X, (sc[0].astype(np.int64), sc[1].astype(np.int64), sc[2].astype(np.int64)), sv)
+    assert np.count_nonzero(X) < len(sv)  # the stream does repeat coordinates
+    M = np.einsum("tr,ir,jr->tij", res["time_rows"], res["factors"][1], res["factors"][2])
+    want = np.linalg.norm(X - M) / np.linalg.norm(X)
+    assert abs(got["cpstream_nonneg"] - want) <= 1e-9 * want

@@ -47,3 +47,23 @@ def test_cpstream_final_error_matches_oracle(gpu, nonneg):
     rerr, _ = O.cpd_error(rc, rv, Fr)
     assert np.isfinite(got["final_error"])
     assert abs(got["final_error"] - rerr) <= 1e-7 * abs(rerr)
+
+
+def test_device_coo_takes_device_factors(gpu):
+    """With the COO on the device the C ABI reads the factors there too: the
+    binding refuses host factors before any call (a kernel reading a host
+    address would fault the GPU), and device factors give the host result."""
+    d, c, v = rand_tensor([30, 20, 10], 500, seed=71)
+    F = rand_factors(d, 8, seed=72)
+    dc = [gpu.DeviceArray.from_numpy(x) for x in c]
+    dv = gpu.DeviceArray.from_numpy(v)
+    with pytest.raises(gpu.AltoError, match="DeviceArray"):
+        gpu.cpd_error(d, dc, dv, F, on_device=True)
+    dF = [gpu.DeviceArray.from_numpy(f) for f in F]
+    err_d, parts_d = gpu.cpd_error(d, dc, dv, dF, on_device=True)
+    err_h, parts_h = gpu.cpd_error(d, c, v, F)
+    # the <X, M> MTTKRP adds with atomics: equal up to the order of the sums
+    np.testing.assert_allclose(err_d, err_h, rtol=1e-12)
+    np.testing.assert_allclose(parts_d, parts_h, rtol=1e-12)
+    for x in dc + dF + [dv]:
+        x.close()
