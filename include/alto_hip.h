@@ -254,8 +254,9 @@ int alto_hip_pseudo_inverse(int nmode, int rank, const double *grams, int mode, 
  * <X, M> from the COO MTTKRP of the mode with the fewest rows (dims[m] rows
  * per factor, rank columns, FP64), ||X||^2 = sum of squared stored values.
  * The reference prints err^2 as "final fit error".  parts (may be NULL):
- * ||X||^2, ||M||^2, <X, M>.  Arrays host or device as coo->on_device says;
- * a coordinate >= dims[m] is ALTO_ERR_ARG. */
+ * ||X||^2, ||M||^2, <X, M>.  Arrays (coordinates, values and factors) host
+ * or device as coo->on_device says; with on_device, an array the GPU cannot
+ * read (pageable host memory) is ALTO_ERR_ARG, as is a coordinate >= dims[m]. */
 int alto_hip_cpd_error(const alto_hip_coo *coo, int rank, const double *const *factors,
                        double *err, double *parts, void *stream);
 
