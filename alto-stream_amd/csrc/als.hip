@@ -491,6 +491,15 @@ extern "C" int alto_hip_cpd_als_batch_dev(int ntensors, const alto_hip_tensor *t
                                           const alto_hip_als_opts *opts, double *const *const *factors,
                                           double *const *lambdas, double *const *fits, int *iters,
                                           int nstreams, double *timings) {
+    if (ntensors > 0 && (!tensors || !factors)) return ALTO_ERR_ARG;
+    ALTO_TRY(ensure_init());
+    for (int k = 0; k < ntensors; ++k) {  // device models: refuse memory the GPU cannot read
+        if (!tensors[k] || !factors[k]) return ALTO_ERR_ARG;
+        for (int m = 0; m < tensors[k]->impl.nmode; ++m) {
+            if (!factors[k][m]) return ALTO_ERR_ARG;
+            ALTO_TRY(check_device_access(factors[k][m], "batched CP-ALS device model"));
+        }
+    }
     return cpd_als_batch(ntensors, tensors, opts, factors, lambdas, fits, iters, nstreams, timings, true);
 }
 

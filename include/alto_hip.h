@@ -231,8 +231,9 @@ int alto_hip_cpd_als_batch(int ntensors, const alto_hip_tensor *tensors,
  * and out of the working precision on the device, so no model crosses
  * PCIe (lambdas, fits and iters stay host arrays).  A tensor outside the
  * grouped classes (see alto_hip_cpd_als_batch) takes a host round trip.
- * No reference counterpart (the batched workload has none); for callers
- * whose models already live in HBM. */
+ * A factor array the GPU cannot read (pageable host memory) is
+ * ALTO_ERR_ARG.  No reference counterpart (the batched workload has none);
+ * for callers whose models already live in HBM. */
 int alto_hip_cpd_als_batch_dev(int ntensors, const alto_hip_tensor *tensors,
                                const alto_hip_als_opts *opts, double *const *const *factors,
                                double *const *lambdas, double *const *fits, int *iters,
