@@ -252,14 +252,17 @@ int main(int argc, char **argv) {
     } else if (model == 1 || model == 2) {
         if (streaming_mode < 0 || streaming_mode >= X->nmodes) {
             fprintf(stderr, "Need to provide -streaming_mode: %d\n", streaming_mode);
+            DestroySparseTensor(X);
             return -1;
         }
         if (prec == ALTO_PREC_BF16) {
             fprintf(stderr, "CP-stream runs its MTTKRP in f64 or f32 (--prec bf16 is for -l als and -p)\n");
+            DestroySparseTensor(X);
             return -1;
         }
         if (timeslice_limit < 1) {
             fprintf(stderr, "Invalid -timeslice-limit: %llu\n", timeslice_limit);
+            DestroySparseTensor(X);
             return -1;
         }
         double w = now();
@@ -268,6 +271,7 @@ int main(int argc, char **argv) {
         printf("%s (ALTO): %f s\n", model == 2 ? "spCPSTREAM" : "CPSTREAM", now() - w);
     } else {
         fprintf(stderr, "No -l model given (als | cpstream_alto | spcpstream_alto)\n");
+        DestroySparseTensor(X);
         return -1;
     }
     DestroySparseTensor(X);
