@@ -173,6 +173,10 @@ __device__ __forceinline__ void flush_lane(OT *p, const CT *v, bool zeroed) {
     }
 }
 
+// a * b + c with one rounding, in the type of the operands
+__device__ __forceinline__ float fused_madd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fused_madd(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 template <int PKW>
 __device__ __forceinline__ uint32_t hfield(const uint64_t *k, int sh, int w) {
     if (PKW == 1) return (uint32_t)((k[0] >> sh) & ((1ULL << w) - 1ULL));
@@ -331,6 +335,14 @@ __global__ __launch_bounds__(NT) void k_mttkrp_hot(const uint64_t *__restrict__ 
     for (int q = 0; q < NM1; ++q) fb[q] = (const char *)ha.fac[q];
     const uint32_t lane_b = (uint32_t)(lig * VPL * sizeof(FT));
     OT *outp = out + lig * VPL;
+    // A hot code carries its LDS row address a (a multiple of RB) with the
+    // row's chunk swizzle s in bits 4.. and the flag bit: kHotFlag | a | s<<4.
+    // Chunk j of the row sits at a + ((j ^ s) << 4) = (a | s<<4) ^ (j << 4),
+    // so a lane's chunk address is one xor with a lane constant, which also
+    // clears the flag bit.
+    uint32_t lds_x[LC];
+#pragma unroll
+    for (int c = 0; c < LC; ++c) lds_x[c] = kHotFlag | ((uint32_t)(lig * LC + c) << 4);
 
     uint32_t cur = kNone, first_row = kNone;
     bool in_first = true;
@@ -355,7 +367,8 @@ __global__ __launch_bounds__(NT) void k_mttkrp_hot(const uint64_t *__restrict__ 
 #pragma unroll
             for (int q = 0; q < NM1; ++q) {
                 const uint32_t code = hfield<PKW>(k, ha.fsh[q], ha.fw[q]);
-                roff[q][e] = code >= ha.dim[q] ? (kHotFlag | (ha.lds[q] + (code - ha.dim[q]) * (uint32_t)RB))
+                const uint32_t la = ha.lds[q] + (code - ha.dim[q]) * (uint32_t)RB;
+                roff[q][e] = code >= ha.dim[q] ? (kHotFlag | la | (((la / RB) & (C16 - 1)) << 4))
                                                : code * (uint32_t)RB;
             }
             rrow[e] = hfield<PKW>(k, ha.tsh, ha.tw);
@@ -390,11 +403,9 @@ __global__ __launch_bounds__(NT) void k_mttkrp_hot(const uint64_t *__restrict__ 
                 for (int k = 0; k < CB; ++k) {
                     const uint32_t o = ob[k][q];
                     if (o & kHotFlag) {
-                        const uint32_t a = o & ~kHotFlag;
-                        const uint32_t swz = (a / RB) & (C16 - 1);
 #pragma unroll
                         for (int c = 0; c < LC; ++c) {
-                            const uint4 u = *(const uint4 *)(smem + a + (((lig * LC + c) ^ swz) << 4));
+                            const uint4 u = *(const uint4 *)(smem + (o ^ lds_x[c]));
                             HotRow<FT>::cvt(u, f[q][k] + c * HotRow<FT>::E16);
                         }
                     } else {
@@ -408,19 +419,12 @@ __global__ __launch_bounds__(NT) void k_mttkrp_hot(const uint64_t *__restrict__ 
                 }
 #pragma unroll
             for (int k = 0; k < CB; ++k) {
-                if (e0 + k >= nh) break;
-                CT g[VPL];
-#pragma unroll
-                for (int v = 0; v < VPL; ++v) g[v] = vv[k] * f[0][k][v];
-#pragma unroll
-                for (int q = 1; q < NM1; ++q)
-#pragma unroll
-                    for (int v = 0; v < VPL; ++v) g[v] *= f[q][k][v];
-                const uint32_t row = tr[k];
-                if (row == cur) {
-#pragma unroll
-                    for (int v = 0; v < VPL; ++v) acc[v] += g[v];
-                } else {
+                // entries past the group's end add zero to the current run
+                // (no branch around them: every path then consumes the loads)
+                const bool live = e0 + k < nh;
+                const uint32_t row = live ? tr[k] : cur;
+                const CT vk = live ? vv[k] : (CT)0;
+                if (row != cur) {  // a new run: hand the finished one on, restart from zero
                     if (cur != kNone) {
                         if (in_first) {
                             first_row = cur;
@@ -433,8 +437,19 @@ __global__ __launch_bounds__(NT) void k_mttkrp_hot(const uint64_t *__restrict__ 
                     }
                     cur = row;
 #pragma unroll
-                    for (int v = 0; v < VPL; ++v) acc[v] = g[v];
+                    for (int v = 0; v < VPL; ++v) acc[v] = 0;
                 }
+                // acc += value * f_0 * ... * f_{NM1-1}, the last product fused
+                // with the add (its first term, acc = 0, is the rounded product)
+                CT g[VPL];
+#pragma unroll
+                for (int v = 0; v < VPL; ++v) g[v] = vk * f[0][k][v];
+#pragma unroll
+                for (int q = 1; q + 1 < NM1; ++q)
+#pragma unroll
+                    for (int v = 0; v < VPL; ++v) g[v] *= f[q][k][v];
+#pragma unroll
+                for (int v = 0; v < VPL; ++v) acc[v] = fused_madd(g[v], f[NM1 - 1][k][v], acc[v]);
             }
         };
         static_for<0, BLK / CB>([&](auto i) {
