@@ -502,39 +502,41 @@ __device__ __forceinline__ void k_mttkrp_dec_body(unsigned lb, const uint64_t *_
                 for (int k = 0; k < CB; ++k) RowLoad<FT, VPL>::load(fb[q] + ob[k][q], f[q][k]);
 #pragma unroll
             for (int k = 0; k < CB; ++k) {
+                // entries past the range's end add zero to the current run
+                const bool live = c0 + k < nh;
+                const uint32_t row = live ? tr[k] : cur;
+                const CT vk = live ? vv[k] : (CT)0;
+                if (row != cur) {  // a new run: hand the finished one on, restart from zero
+                    if (cur != NONE) {
+                        if (!excl) {
+#pragma unroll
+                            for (int v = 0; v < VPL; ++v)
+                                atomic_add_out(outp + (uint64_t)cur * R + v, (OT)acc[v]);
+                        } else if (in_first) {
+                            first_row = cur;
+#pragma unroll
+                            for (int v = 0; v < VPL; ++v) facc[v] = acc[v];
+                            in_first = false;
+                        } else {
+#pragma unroll
+                            for (int v = 0; v < VPL; ++v) flush_owned(outp + (uint64_t)cur * R + v, acc[v], zeroed);
+                        }
+                    }
+                    cur = row;
+#pragma unroll
+                    for (int v = 0; v < VPL; ++v) acc[v] = 0;
+                }
+                // acc += value * f_0 * ... * f_{NM1-1}, the last product fused
+                // with the add (its first term, acc = 0, is the rounded product)
                 CT g[VPL];
 #pragma unroll
-                for (int v = 0; v < VPL; ++v) g[v] = vv[k] * f[0][k][v];
+                for (int v = 0; v < VPL; ++v) g[v] = vk * f[0][k][v];
 #pragma unroll
-                for (int q = 1; q < NM1; ++q)
+                for (int q = 1; q + 1 < NM1; ++q)
 #pragma unroll
                     for (int v = 0; v < VPL; ++v) g[v] *= f[q][k][v];
-                if (c0 + k < nh) {
-                    const uint32_t row = tr[k];
-                    if (row == cur) {
 #pragma unroll
-                        for (int v = 0; v < VPL; ++v) acc[v] += g[v];
-                    } else {
-                        if (cur != NONE) {
-                            if (!excl) {
-#pragma unroll
-                                for (int v = 0; v < VPL; ++v)
-                                    atomic_add_out(outp + (uint64_t)cur * R + v, (OT)acc[v]);
-                            } else if (in_first) {
-                                first_row = cur;
-#pragma unroll
-                                for (int v = 0; v < VPL; ++v) facc[v] = acc[v];
-                                in_first = false;
-                            } else {
-#pragma unroll
-                                for (int v = 0; v < VPL; ++v) flush_owned(outp + (uint64_t)cur * R + v, acc[v], zeroed);
-                            }
-                        }
-                        cur = row;
-#pragma unroll
-                        for (int v = 0; v < VPL; ++v) acc[v] = g[v];
-                    }
-                }
+                for (int v = 0; v < VPL; ++v) acc[v] = fused_madd(g[v], f[NM1 - 1][k][v], acc[v]);
             }
         }
         __builtin_amdgcn_wave_barrier();

@@ -23,6 +23,10 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned nb) {
     return x < r ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
 }
 
+// a * b + c with one rounding, in the type of the operands
+__device__ __forceinline__ float fused_madd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fused_madd(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 // Compile-time loop: f(integral_constant<int, I>) for I in [B, E).
 template <int B, int E, typename F> __device__ __forceinline__ void static_for(F &&f) {
     if constexpr (B < E) {

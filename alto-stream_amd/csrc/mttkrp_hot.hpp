@@ -173,10 +173,6 @@ __device__ __forceinline__ void flush_lane(OT *p, const CT *v, bool zeroed) {
     }
 }
 
-// a * b + c with one rounding, in the type of the operands
-__device__ __forceinline__ float fused_madd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
-__device__ __forceinline__ double fused_madd(double a, double b, double c) { return __builtin_fma(a, b, c); }
-
 template <int PKW>
 __device__ __forceinline__ uint32_t hfield(const uint64_t *k, int sh, int w) {
     if (PKW == 1) return (uint32_t)((k[0] >> sh) & ((1ULL << w) - 1ULL));
