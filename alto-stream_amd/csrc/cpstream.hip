@@ -150,23 +150,32 @@ __device__ __forceinline__ void wait_stores() {
 }
 
 // Sums over workgroups of the three device-scope partials (wave 0; at most
-// 1024 workgroups): all loads are issued before any is used.
+// 1024 workgroups): lane l adds the partials of workgroups l, l + 64, ...
+// in that order, loading four workgroups' partials at a time, then the
+// lanes' sums are combined by a shuffle tree.
 __device__ void sum_partials(const double *part, unsigned nblk, double *out) {
     const int lane = threadIdx.x & 63;
-    double v[16][3];
+    double a[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        const unsigned b = lane + 64u * j;
+    for (int j0 = 0; j0 < 16; j0 += 4) {
+        if ((unsigned)(64 * j0) >= nblk) break;  // uniform
+        double v[4][3];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) v[j][k] = b < nblk ? load_dev(part + (size_t)b * 3 + k) : 0.0;
+        for (int j = 0; j < 4; ++j) {
+            const unsigned b = lane + 64u * (j0 + j);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) v[j][k] = b < nblk ? load_dev(part + (size_t)b * 3 + k) : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) a[k] += v[j][k];
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        double a = 0.0;
-#pragma unroll
-        for (int j = 0; j < 16; ++j) a += v[j][k];
-        for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
-        out[k] = a;
+        double x = a[k];
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+        out[k] = x;
     }
 }
 
@@ -184,6 +193,15 @@ __device__ void sum_partials(const double *part, unsigned nblk, double *out) {
 // 0) (the new primal).  The grid must be resident at once (the launch sizes
 // it from the occupancy); a wait longer than `limit` wall-clock ticks sets
 // *err and ends the kernel (its results are then not used).
+//
+// When every workgroup owns at most kAdmmCacheTiles row tiles (a grid-uniform
+// test), a thread keeps its elements of mt, aux and dual in registers for
+// the whole loop: it reads them once, and writes only dual and U at the end
+// (aux is scratch between the first pass and this loop), instead of reading
+// three and writing two arrays per iteration. The operations and the order
+// of every sum are those of the streaming path.
+constexpr int kAdmmCacheTiles = 5;
+
 __global__ __launch_bounds__(256) void k_admm_loop(const double *__restrict__ mt,
                                                    double *__restrict__ aux,
                                                    double *__restrict__ dual,
@@ -203,9 +221,68 @@ __global__ __launch_bounds__(256) void k_admm_loop(const double *__restrict__ mt
     for (int e = t; e < R * R; e += 256) Wv[e] = vinv[e];
     const double rho = *rho_p;
     const uint64_t ntiles = (rows + TR - 1) / TR;
+    const bool cached = ntiles <= (uint64_t)kAdmmCacheTiles * nblk;
+    double cm[kAdmmCacheTiles][4], ca[kAdmmCacheTiles][4], cd[kAdmmCacheTiles][4];
+    if (cached) {
+#pragma unroll
+        for (int i = 0; i < kAdmmCacheTiles; ++i) {
+            const uint64_t tile = blockIdx.x + (uint64_t)i * nblk;
+            const uint64_t r0 = tile * TR;
+            const int nr = tile < ntiles ? (int)((rows - r0) < (uint64_t)TR ? (rows - r0) : TR) : 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int rr = rl + q * lanes;
+                cm[i][q] = ca[i][q] = cd[i][q] = 0.0;
+                if (c < R && rr < nr) {
+                    const uint64_t g = (r0 + rr) * R + c;
+                    cm[i][q] = mt[g];
+                    ca[i][q] = aux[g];
+                    cd[i][q] = dual[g];
+                }
+            }
+        }
+    }
     for (int it = 0; it < max_iter; ++it) {
         double pn = 0.0, dn = 0.0, pr = 0.0;
-        for (uint64_t tile = blockIdx.x; tile < ntiles; tile += nblk) {
+        if (cached) {
+#pragma unroll
+            for (int i = 0; i < kAdmmCacheTiles; ++i) {
+                const uint64_t tile = blockIdx.x + (uint64_t)i * nblk;
+                if (tile >= ntiles) break;  // uniform in the workgroup
+                const uint64_t r0 = tile * TR;
+                const int nr = (int)((rows - r0) < (uint64_t)TR ? (rows - r0) : TR);
+                __syncthreads();
+                if (c < R)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int rr = rl + q * lanes;
+                        if (rr < nr) {
+                            const double a = ca[i][q], d = cd[i][q];
+                            double x = a - d;
+                            x = x < 0.0 ? 0.0 : x;
+                            pn += x * x;
+                            const double y = x - a;
+                            const double di = d + y;
+                            dn += di * di;
+                            pr += y * y;
+                            cd[i][q] = di;
+                            T2[rr * R + c] = cm[i][q] + rho * (x + di);
+                        }
+                    }
+                __syncthreads();
+                if (c < R)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int rr = rl + q * lanes;
+                        if (rr < nr) {
+                            double a = 0.0;
+                            for (int k = 0; k < R; ++k) a += T2[rr * R + k] * Wv[k * R + c];
+                            ca[i][q] = a;
+                        }
+                    }
+            }
+        }
+        for (uint64_t tile = blockIdx.x; !cached && tile < ntiles; tile += nblk) {
             uint64_t r0 = tile * TR;
             int nr = (int)((rows - r0) < (uint64_t)TR ? (rows - r0) : TR);
             __syncthreads();
@@ -288,6 +365,27 @@ __global__ __launch_bounds__(256) void k_admm_loop(const double *__restrict__ mt
         __syncthreads();
         if (conv < 0) return;
         if (conv) break;
+    }
+    if (cached) {
+#pragma unroll
+        for (int i = 0; i < kAdmmCacheTiles; ++i) {
+            const uint64_t tile = blockIdx.x + (uint64_t)i * nblk;
+            if (tile >= ntiles) break;
+            const uint64_t r0 = tile * TR;
+            const int nr = (int)((rows - r0) < (uint64_t)TR ? (rows - r0) : TR);
+            if (c < R)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int rr = rl + q * lanes;
+                    if (rr < nr) {
+                        const uint64_t g = (r0 + rr) * R + c;
+                        const double x = ca[i][q] - cd[i][q];
+                        U[g] = x > 0.0 ? x : 0.0;
+                        dual[g] = cd[i][q];
+                    }
+                }
+        }
+        return;
     }
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += nblk) {
         uint64_t r0 = tile * TR;
