@@ -16,6 +16,7 @@
 // grows come from caller-supplied initial matrices (the reference fills
 // them with rand(), kruskal_model.cpp:101-133).
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -194,14 +195,16 @@ __device__ void sum_partials(const double *part, unsigned nblk, double *out) {
 // it from the occupancy); a wait longer than `limit` wall-clock ticks sets
 // *err and ends the kernel (its results are then not used).
 //
-// When every workgroup owns at most kAdmmCacheTiles row tiles (a grid-uniform
-// test), a thread keeps its elements of mt, aux and dual in registers for
-// the whole loop: it reads them once, and writes only dual and U at the end
+// CT > 0 (the launch guarantees that every workgroup owns at most CT row
+// tiles): a workgroup keeps its rows of mt in LDS (after Wv and T2, CT tiles
+// of TR x R) and a thread its elements of aux and dual in registers for the
+// whole loop: they are read once, and only dual and U are written at the end
 // (aux is scratch between the first pass and this loop), instead of reading
-// three and writing two arrays per iteration. The operations and the order
-// of every sum are those of the streaming path.
-constexpr int kAdmmCacheTiles = 5;
+// three and writing two arrays per iteration. CT = 0 streams them. The
+// operations and the order of every sum are the same in both.
+constexpr int kAdmmCacheTiles = 7;
 
+template <int CT>
 __global__ __launch_bounds__(256) void k_admm_loop(const double *__restrict__ mt,
                                                    double *__restrict__ aux,
                                                    double *__restrict__ dual,
@@ -221,21 +224,23 @@ __global__ __launch_bounds__(256) void k_admm_loop(const double *__restrict__ mt
     for (int e = t; e < R * R; e += 256) Wv[e] = vinv[e];
     const double rho = *rho_p;
     const uint64_t ntiles = (rows + TR - 1) / TR;
-    const bool cached = ntiles <= (uint64_t)kAdmmCacheTiles * nblk;
-    double cm[kAdmmCacheTiles][4], ca[kAdmmCacheTiles][4], cd[kAdmmCacheTiles][4];
-    if (cached) {
+    constexpr bool cached = CT > 0;
+    constexpr int NC = CT > 0 ? CT : 1;
+    double ca[NC][4], cd[NC][4];
+    double *Mt = T2 + TR * R;  // cached form: tile i's rows of mt at Mt + i TR R
+    if constexpr (cached) {
 #pragma unroll
-        for (int i = 0; i < kAdmmCacheTiles; ++i) {
+        for (int i = 0; i < CT; ++i) {
             const uint64_t tile = blockIdx.x + (uint64_t)i * nblk;
             const uint64_t r0 = tile * TR;
             const int nr = tile < ntiles ? (int)((rows - r0) < (uint64_t)TR ? (rows - r0) : TR) : 0;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int rr = rl + q * lanes;
-                cm[i][q] = ca[i][q] = cd[i][q] = 0.0;
+                ca[i][q] = cd[i][q] = 0.0;
                 if (c < R && rr < nr) {
                     const uint64_t g = (r0 + rr) * R + c;
-                    cm[i][q] = mt[g];
+                    Mt[i * TR * R + rr * R + c] = mt[g];
                     ca[i][q] = aux[g];
                     cd[i][q] = dual[g];
                 }
@@ -244,9 +249,9 @@ __global__ __launch_bounds__(256) void k_admm_loop(const double *__restrict__ mt
     }
     for (int it = 0; it < max_iter; ++it) {
         double pn = 0.0, dn = 0.0, pr = 0.0;
-        if (cached) {
+        if constexpr (cached) {
 #pragma unroll
-            for (int i = 0; i < kAdmmCacheTiles; ++i) {
+            for (int i = 0; i < CT; ++i) {
                 const uint64_t tile = blockIdx.x + (uint64_t)i * nblk;
                 if (tile >= ntiles) break;  // uniform in the workgroup
                 const uint64_t r0 = tile * TR;
@@ -266,7 +271,7 @@ __global__ __launch_bounds__(256) void k_admm_loop(const double *__restrict__ mt
                             dn += di * di;
                             pr += y * y;
                             cd[i][q] = di;
-                            T2[rr * R + c] = cm[i][q] + rho * (x + di);
+                            T2[rr * R + c] = Mt[i * TR * R + rr * R + c] + rho * (x + di);
                         }
                     }
                 __syncthreads();
@@ -366,9 +371,9 @@ __global__ __launch_bounds__(256) void k_admm_loop(const double *__restrict__ mt
         if (conv < 0) return;
         if (conv) break;
     }
-    if (cached) {
+    if constexpr (cached) {
 #pragma unroll
-        for (int i = 0; i < kAdmmCacheTiles; ++i) {
+        for (int i = 0; i < CT; ++i) {
             const uint64_t tile = blockIdx.x + (uint64_t)i * nblk;
             if (tile >= ntiles) break;
             const uint64_t r0 = tile * TR;
@@ -562,6 +567,7 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
     const TileShape ts = tile_shape(R);
     const size_t lds_first = sizeof(double) * (2 * RR + 2 * (size_t)ts.TR * R);
     const size_t lds_iter = sizeof(double) * (RR + (size_t)ts.TR * R);
+    const size_t lds_iter_c = lds_iter + sizeof(double) * kAdmmCacheTiles * (size_t)ts.TR * R;
     if (lds_first > 160 * 1024 - 512) {
         set_error("CP-stream supports rank <= 64 (rank %d)", R);
         return ALTO_ERR_UNSUPPORTED;
@@ -571,25 +577,44 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
         std::call_once(attr, [] {
             (void)hipFuncSetAttribute((const void *)k_admm_first,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 256);
-            (void)hipFuncSetAttribute((const void *)k_admm_loop,
+            (void)hipFuncSetAttribute((const void *)k_admm_loop<0>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 256);
+            (void)hipFuncSetAttribute((const void *)k_admm_loop<kAdmmCacheTiles>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 256);
         });
     }
-    // ADMM loop grid: resident at once (occupancy x CUs, at most 4 per CU and
-    // 1024 in all); its barrier gives up after 50 ms of wall clock
-    int admm_resident = 1;
+    // ADMM grids, all resident at once (occupancy x CUs, at most 4 per CU and
+    // 1024 in all): the first pass alone (unconstrained), and the streamed
+    // and the cached form of the ADMM loop, whose grid barrier gives up after
+    // 50 ms of wall clock (the barrier measured slower at 4 workgroups per CU
+    // than at 2: 262 against 279 slices/s over the 1000-slice stream, so the
+    // streamed loop takes at most 2)
+    int admm_resident = 1, admm_resident_s = 1, admm_resident_c = 0;
     long long bar_limit = 0;
+    // ALTO_ADMM_CACHE=0 in the environment streams the ADMM state from memory
+    // every iteration (the on-chip form is the default where it fits)
+    bool admm_cache = true;
+    {
+        const char *e = getenv("ALTO_ADMM_CACHE");
+        if (e && e[0] == '0') admm_cache = false;
+    }
     {
         int dev = 0, cus = 0, occ = 0, khz = 0;
         ALTO_HIP_TRY(hipGetDevice(&dev));
         ALTO_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         ALTO_HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
-        ALTO_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_admm_loop, 256, lds_iter));
+        int occ_c = 0;
+        ALTO_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_admm_loop<0>, 256, lds_iter));
+        if (lds_iter_c <= 160 * 1024 - 512)
+            ALTO_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_c, k_admm_loop<kAdmmCacheTiles>, 256,
+                                                                      lds_iter_c));
+        admm_resident_c = std::min(cus * std::min(occ_c, 4), 1024);
         if (occ < 1 || cus < 1 || khz < 1) {
             set_error("ADMM loop kernel: no resident workgroup (occupancy %d, %d CUs)", occ, cus);
             return ALTO_ERR_UNSUPPORTED;
         }
         admm_resident = std::min(cus * std::min(occ, 4), 1024);
+        admm_resident_s = std::min(cus * std::min(occ, 2), 1024);
         bar_limit = (long long)khz * 50;
     }
     EventSet evs;
@@ -713,7 +738,14 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
                     ALTO_TRY(launch_chol_inv(G.as<double>(), N, j, R, 1e-12, false, vinv.as<double>(),
                                              flag.as<int>() + 1, nullptr, st, 1, nullptr));
                 }
-                const int nba = admm_blocks(I, ts, admm_resident);
+                // the ADMM loop keeps its rows on chip when every workgroup of
+                // the cached form's resident grid owns at most kAdmmCacheTiles
+                // row tiles; the first pass runs on the same grid (the loop
+                // reads its per-workgroup partials)
+                const bool cache = nonneg && admm_cache && admm_resident_c > 0 &&
+                                   (I + ts.TR - 1) / ts.TR <= (uint64_t)kAdmmCacheTiles * admm_resident_c;
+                const int nba =
+                    admm_blocks(I, ts, !nonneg ? admm_resident : cache ? admm_resident_c : admm_resident_s);
                 k_admm_first<<<nba, 256, lds_first, st>>>(buf.as<double>(), P[j].as<double>(),
                                                          ata.as<double>(), U[j].as<double>(),
                                                          duals[j].as<double>(), vinv.as<double>(),
@@ -721,12 +753,11 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
                                                          aux.as<double>(), part0.as<double>());
                 ALTO_LAUNCH_CHECK();
                 if (nonneg) {
-                    k_admm_loop<<<nba, 256, lds_iter, st>>>(buf.as<double>(), aux.as<double>(),
-                                                            duals[j].as<double>(), vinv.as<double>(),
-                                                            rho.as<double>(), U[j].as<double>(), I, R, ts.RP,
-                                                            part.as<double>(), part0.as<double>(),
-                                                            bar.as<unsigned>(), flag.as<int>() + 3,
-                                                            bar_limit, kAdmmMaxIter, admm_it.as<double>());
+                    (cache ? k_admm_loop<kAdmmCacheTiles> : k_admm_loop<0>)<<<nba, 256,
+                                                                           cache ? lds_iter_c : lds_iter, st>>>(
+                        buf.as<double>(), aux.as<double>(), duals[j].as<double>(), vinv.as<double>(),
+                        rho.as<double>(), U[j].as<double>(), I, R, ts.RP, part.as<double>(), part0.as<double>(),
+                        bar.as<unsigned>(), flag.as<int>() + 3, bar_limit, kAdmmMaxIter, admm_it.as<double>());
                     ALTO_LAUNCH_CHECK();
                 }
                 ALTO_TRY(gram(j, I));

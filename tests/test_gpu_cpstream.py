@@ -35,9 +35,19 @@ def _init(dims, R, seed):
     return [rng.random((int(d), R)) for d in dims]
 
 
+@pytest.fixture(params=["cached", "streamed"])
+def admm_state(request, monkeypatch):
+    """The non-negative ADMM loop keeps its row state in registers when each
+    workgroup owns few row tiles (every slice here); ALTO_ADMM_CACHE=0 makes it
+    stream the state from memory every iteration, the path of large slices."""
+    if request.param == "streamed":
+        monkeypatch.setenv("ALTO_ADMM_CACHE", "0")
+    return request.param
+
+
 @pytest.mark.parametrize("nonneg", [True, False])
 @pytest.mark.parametrize("R", [4, 16, 32])
-def test_cpstream_matches_oracle(gpu, nonneg, R):
+def test_cpstream_matches_oracle(gpu, nonneg, R, admm_state):
     dims, cidx, vals = _stream_tensor(6, [300, 200], 6000, seed=90)
     init = _init(dims, R, 91)
     O.set_threads(1)
@@ -57,7 +67,7 @@ def test_cpstream_matches_oracle(gpu, nonneg, R):
                                rtol=1e-8, atol=1e-9)
 
 
-def test_cpstream_stream_mode_not_first_and_4_modes(gpu):
+def test_cpstream_stream_mode_not_first_and_4_modes(gpu, admm_state):
     rng = np.random.default_rng(92)
     T, nnz = 5, 5000
     dims = [120, T, 90, 40]
