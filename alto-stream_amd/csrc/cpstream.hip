@@ -195,14 +195,40 @@ __device__ void sum_partials(const double *part, unsigned nblk, double *out) {
 // it from the occupancy); a wait longer than `limit` wall-clock ticks sets
 // *err and ends the kernel (its results are then not used).
 //
-// CT > 0 (the launch guarantees that every workgroup owns at most CT row
-// tiles): a workgroup keeps its rows of mt in LDS (after Wv and T2, CT tiles
-// of TR x R) and a thread its elements of aux and dual in registers for the
-// whole loop: they are read once, and only dual and U are written at the end
-// (aux is scratch between the first pass and this loop), instead of reading
-// three and writing two arrays per iteration. CT = 0 streams them. The
-// operations and the order of every sum are the same in both.
+// CT = 0: the loop alone, after k_admm_first, streaming aux, dual and mt
+// from memory every iteration.  CT > 0 (the launch guarantees that every
+// workgroup owns at most CT row tiles): the first pass of k_admm_first and
+// the loop in one launch, on chip: a workgroup keeps its rows of
+// mt + P * ata in LDS (after Wv and T2: CT tiles of TR x R, then ata and a
+// staging tile of P) and a thread its elements of aux and dual in registers;
+// P, mt, U and dual are read once, dual and U written once at the end (mt and
+// aux are scratch), instead of writing two and reading three arrays in the
+// first pass and every iteration.  The first pass's residual partial goes to
+// part0 through device-scope stores, read by the workgroup that decides the
+// first iteration.  The operations and the order of every sum are those of
+// k_admm_first and the streamed loop.
 constexpr int kAdmmCacheTiles = 7;
+
+// part0 summed in the order of wave_reduce_entry (four strided accumulators,
+// then the lanes), with device-scope loads: the partials were stored by other
+// workgroups of the same launch.
+__device__ double sum_part0_dev(const double *part0, int nblk) {
+    const int lane = threadIdx.x & 63;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int b = lane;
+    for (; b + 192 < nblk; b += 256) {
+        const double v0 = load_dev(part0 + b), v1 = load_dev(part0 + b + 64);
+        const double v2 = load_dev(part0 + b + 128), v3 = load_dev(part0 + b + 192);
+        a0 += v0;
+        a1 += v1;
+        a2 += v2;
+        a3 += v3;
+    }
+    for (; b < nblk; b += 64) a0 += load_dev(part0 + b);
+    double a = (a0 + a1) + (a2 + a3);
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
+    return a;
+}
 
 template <int CT>
 __global__ __launch_bounds__(256) void k_admm_loop(const double *__restrict__ mt,
@@ -211,9 +237,11 @@ __global__ __launch_bounds__(256) void k_admm_loop(const double *__restrict__ mt
                                                    const double *__restrict__ vinv,
                                                    const double *__restrict__ rho_p,
                                                    double *__restrict__ U, uint64_t rows, int R, int RP,
-                                                   double *part, const double *__restrict__ part0,
+                                                   double *part, double *part0,
                                                    unsigned *bar, int *err, long long limit,
-                                                   int max_iter, double *iters) {
+                                                   int max_iter, double *iters,
+                                                   const double *__restrict__ P,
+                                                   const double *__restrict__ ata) {
     extern __shared__ double lds[];
     const int lanes = 256 / RP, TR = lanes * 4;
     double *Wv = lds, *T2 = lds + R * R;
@@ -227,25 +255,57 @@ __global__ __launch_bounds__(256) void k_admm_loop(const double *__restrict__ mt
     constexpr bool cached = CT > 0;
     constexpr int NC = CT > 0 ? CT : 1;
     double ca[NC][4], cd[NC][4];
-    double *Mt = T2 + TR * R;  // cached form: tile i's rows of mt at Mt + i TR R
+    double *Mt = T2 + TR * R;  // on-chip form: tile i's rows of mt + P ata at Mt + i TR R
     if constexpr (cached) {
+        // the first pass (k_admm_first, non-negative), tile by tile
+        double *Wa = Mt + CT * TR * R, *T = Wa + R * R;
+        for (int e = t; e < R * R; e += 256) Wa[e] = ata[e];
+        double dres = 0.0;
 #pragma unroll
         for (int i = 0; i < CT; ++i) {
-            const uint64_t tile = blockIdx.x + (uint64_t)i * nblk;
-            const uint64_t r0 = tile * TR;
-            const int nr = tile < ntiles ? (int)((rows - r0) < (uint64_t)TR ? (rows - r0) : TR) : 0;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int rr = rl + q * lanes;
-                ca[i][q] = cd[i][q] = 0.0;
-                if (c < R && rr < nr) {
-                    const uint64_t g = (r0 + rr) * R + c;
-                    Mt[i * TR * R + rr * R + c] = mt[g];
-                    ca[i][q] = aux[g];
-                    cd[i][q] = dual[g];
+            for (int q = 0; q < 4; ++q) ca[i][q] = cd[i][q] = 0.0;
+            const uint64_t tile = blockIdx.x + (uint64_t)i * nblk;
+            if (tile >= ntiles) continue;  // uniform in the workgroup
+            const uint64_t r0 = tile * TR;
+            const int nr = (int)((rows - r0) < (uint64_t)TR ? (rows - r0) : TR);
+            __syncthreads();
+            for (int e = t; e < nr * R; e += 256) T[e] = P[r0 * R + e];
+            __syncthreads();
+            double uu[4] = {0.0, 0.0, 0.0, 0.0};
+            if (c < R)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int rr = rl + q * lanes;
+                    if (rr < nr) {
+                        const uint64_t g = (r0 + rr) * R + c;
+                        double h = 0.0;
+                        for (int k = 0; k < R; ++k) h += T[rr * R + k] * Wa[k * R + c];
+                        const double m = mt[g] + h;
+                        Mt[i * TR * R + rr * R + c] = m;
+                        uu[q] = U[g];
+                        cd[i][q] = dual[g];
+                        T2[rr * R + c] = m + rho * (uu[q] + cd[i][q]);
+                    }
                 }
-            }
+            __syncthreads();
+            if (c < R)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int rr = rl + q * lanes;
+                    if (rr < nr) {
+                        double a = 0.0;
+                        for (int k = 0; k < R; ++k) a += T2[rr * R + k] * Wv[k * R + c];
+                        ca[i][q] = a;
+                        double x = a - cd[i][q];
+                        x = x < 0.0 ? 0.0 : x;
+                        const double dd = x - uu[q];
+                        dres += dd * dd;
+                    }
+                }
         }
+        const double tot = block_sum(dres, red);
+        if (t == 0) store_dev(part0 + blockIdx.x, tot);  // ordered before the first arrival
     }
     for (int it = 0; it < max_iter; ++it) {
         double pn = 0.0, dn = 0.0, pr = 0.0;
@@ -342,7 +402,8 @@ __global__ __launch_bounds__(256) void k_admm_loop(const double *__restrict__ mt
                 double sums[3];
                 sum_partials(part, nblk, sums);
                 const double spn = sums[0], sdn = sums[1], spr = sums[2];
-                const double sdr = it == 0 ? wave_reduce_entry(part0, nblk, 1, 0, 0) : 0.0;
+                const double sdr =
+                    it != 0 ? 0.0 : cached ? sum_part0_dev(part0, nblk) : wave_reduce_entry(part0, nblk, 1, 0, 0);
                 if (t == 0) {
                     const double tol = (double)1e-6f;
                     conv = spr <= tol * spn && sdr <= tol * sdn;
@@ -567,7 +628,8 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
     const TileShape ts = tile_shape(R);
     const size_t lds_first = sizeof(double) * (2 * RR + 2 * (size_t)ts.TR * R);
     const size_t lds_iter = sizeof(double) * (RR + (size_t)ts.TR * R);
-    const size_t lds_iter_c = lds_iter + sizeof(double) * kAdmmCacheTiles * (size_t)ts.TR * R;
+    const size_t lds_iter_c =
+        lds_iter + sizeof(double) * (RR + (kAdmmCacheTiles + 1) * (size_t)ts.TR * R);  // + Mt, ata, P tile
     if (lds_first > 160 * 1024 - 512) {
         set_error("CP-stream supports rank <= 64 (rank %d)", R);
         return ALTO_ERR_UNSUPPORTED;
@@ -746,18 +808,21 @@ extern "C" int alto_hip_cpstream(const alto_hip_coo *coo, int s, int R, int max_
                                    (I + ts.TR - 1) / ts.TR <= (uint64_t)kAdmmCacheTiles * admm_resident_c;
                 const int nba =
                     admm_blocks(I, ts, !nonneg ? admm_resident : cache ? admm_resident_c : admm_resident_s);
-                k_admm_first<<<nba, 256, lds_first, st>>>(buf.as<double>(), P[j].as<double>(),
-                                                         ata.as<double>(), U[j].as<double>(),
-                                                         duals[j].as<double>(), vinv.as<double>(),
-                                                         rho.as<double>(), I, R, ts.RP, nonneg,
-                                                         aux.as<double>(), part0.as<double>());
-                ALTO_LAUNCH_CHECK();
+                if (!cache) {  // the on-chip loop form runs the first pass itself
+                    k_admm_first<<<nba, 256, lds_first, st>>>(buf.as<double>(), P[j].as<double>(),
+                                                             ata.as<double>(), U[j].as<double>(),
+                                                             duals[j].as<double>(), vinv.as<double>(),
+                                                             rho.as<double>(), I, R, ts.RP, nonneg,
+                                                             aux.as<double>(), part0.as<double>());
+                    ALTO_LAUNCH_CHECK();
+                }
                 if (nonneg) {
                     (cache ? k_admm_loop<kAdmmCacheTiles> : k_admm_loop<0>)<<<nba, 256,
                                                                            cache ? lds_iter_c : lds_iter, st>>>(
                         buf.as<double>(), aux.as<double>(), duals[j].as<double>(), vinv.as<double>(),
                         rho.as<double>(), U[j].as<double>(), I, R, ts.RP, part.as<double>(), part0.as<double>(),
-                        bar.as<unsigned>(), flag.as<int>() + 3, bar_limit, kAdmmMaxIter, admm_it.as<double>());
+                        bar.as<unsigned>(), flag.as<int>() + 3, bar_limit, kAdmmMaxIter, admm_it.as<double>(),
+                        P[j].as<double>(), ata.as<double>());
                     ALTO_LAUNCH_CHECK();
                 }
                 ALTO_TRY(gram(j, I));
