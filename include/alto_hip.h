@@ -295,7 +295,9 @@ int alto_hip_cpd_error_noalloc(const alto_hip_coo *coo, int rank, const double *
  * slice-build seconds, MTTKRP seconds, time indices processed.  max_slices >
  * 0 caps the number of batches.  prec: ALTO_PREC_F64 or ALTO_PREC_F32
  * (MTTKRP arithmetic; the solve, ADMM and Grams are FP64 either way).
- * rank <= 64.  Returns the number of batches processed, or a negative
+ * rank <= 64.  ALTO_ADMM_CACHE=0 in the environment makes the non-negative
+ * ADMM stream its row state from memory instead of keeping it on chip (same
+ * results).  Returns the number of batches processed, or a negative
  * status. */
 int alto_hip_cpstream(const alto_hip_coo *coo, int stream_mode, int rank,
                       int max_iters, double epsilon, int nonneg, int max_slices,
