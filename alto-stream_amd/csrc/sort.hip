@@ -14,18 +14,22 @@
 namespace alto {
 
 constexpr int RS_THREADS = 256;
-constexpr int RS_IPT = 16;
-constexpr int RS_TILE = RS_THREADS * RS_IPT;  // 4096 keys per workgroup
+// keys per thread: 16 (4096 keys per workgroup), or 4 below kRsSmall keys,
+// where 4096-key tiles would leave most CUs idle (a 300K-key CP-stream slice
+// schedule made 73 workgroups)
+constexpr uint64_t kRsSmall = 1ULL << 21;
 constexpr int SCAN_CHUNK = 4096;
 
 __device__ __forceinline__ uint32_t digit_of(uint64_t w, int sh) {
     return (uint32_t)((w >> sh) & 0xFFu);
 }
 
+template <int RS_IPT>
 __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t *__restrict__ kw,
                                                         uint64_t n, int sh,
                                                         uint32_t *__restrict__ counts,
                                                         uint32_t ntiles) {
+    constexpr int RS_TILE = RS_THREADS * RS_IPT;
     __shared__ uint32_t hist[256];
     hist[threadIdx.x] = 0;
     __syncthreads();
@@ -116,12 +120,13 @@ int exclusive_scan_u32(uint32_t *a, uint64_t n, DevBuf &tmp, hipStream_t s) {
 }
 
 // ---- stable scatter ----
-template <int KW>
+template <int KW, int RS_IPT>
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
     const uint64_t *__restrict__ k0, const uint64_t *__restrict__ k1,
     const uint32_t *__restrict__ pin, uint64_t *__restrict__ o0, uint64_t *__restrict__ o1,
     uint32_t *__restrict__ pout, uint64_t n, int word, int sh,
     const uint32_t *__restrict__ offs, uint32_t ntiles) {
+    constexpr int RS_TILE = RS_THREADS * RS_IPT;
     __shared__ uint32_t wcnt[4][256];
     __shared__ uint32_t boff[256];
     __shared__ uint32_t goff[256];
@@ -218,7 +223,8 @@ int radix_sort_pairs(uint64_t *k0, uint64_t *k1, uint32_t *pay, uint64_t n, int 
                      SortScratch &ws, hipStream_t s) {
     if (n <= 1 || bits <= 0) return ALTO_OK;
     const int KW = k1 ? 2 : 1;
-    unsigned ntiles = cdiv(n, RS_TILE);
+    const bool small = n < kRsSmall;
+    unsigned ntiles = cdiv(n, RS_THREADS * (small ? 4 : 16));
     ALTO_TRY(ws.t0.alloc(sizeof(uint64_t) * n));
     if (KW == 2) ALTO_TRY(ws.t1.alloc(sizeof(uint64_t) * n));
     ALTO_TRY(ws.tp.alloc(sizeof(uint32_t) * n));
@@ -230,16 +236,16 @@ int radix_sort_pairs(uint64_t *k0, uint64_t *k1, uint32_t *pay, uint64_t n, int 
         int bit = ps * 8;
         int word = bit / 64, sh = bit % 64;
         const uint64_t *src = word == 0 ? a0 : a1;
-        k_rs_hist<<<ntiles, RS_THREADS, 0, s>>>(src, n, sh, ws.counts.as<uint32_t>(), ntiles);
+        (small ? k_rs_hist<4> : k_rs_hist<16>)<<<ntiles, RS_THREADS, 0, s>>>(src, n, sh,
+                                                                          ws.counts.as<uint32_t>(), ntiles);
         ALTO_LAUNCH_CHECK();
         ALTO_TRY(exclusive_scan_u32(ws.counts.as<uint32_t>(), 256ULL * ntiles, ws.scan_tmp, s));
         if (KW == 1)
-            k_rs_scatter<1><<<ntiles, RS_THREADS, 0, s>>>(a0, nullptr, ap, b0, nullptr, bp, n,
-                                                           word, sh, ws.counts.as<uint32_t>(),
-                                                           ntiles);
+            (small ? k_rs_scatter<1, 4> : k_rs_scatter<1, 16>)<<<ntiles, RS_THREADS, 0, s>>>(
+                a0, nullptr, ap, b0, nullptr, bp, n, word, sh, ws.counts.as<uint32_t>(), ntiles);
         else
-            k_rs_scatter<2><<<ntiles, RS_THREADS, 0, s>>>(a0, a1, ap, b0, b1, bp, n, word, sh,
-                                                           ws.counts.as<uint32_t>(), ntiles);
+            (small ? k_rs_scatter<2, 4> : k_rs_scatter<2, 16>)<<<ntiles, RS_THREADS, 0, s>>>(
+                a0, a1, ap, b0, b1, bp, n, word, sh, ws.counts.as<uint32_t>(), ntiles);
         ALTO_LAUNCH_CHECK();
         std::swap(a0, b0);
         std::swap(a1, b1);
