@@ -772,11 +772,17 @@ __global__ __launch_bounds__(256) void k_mttkrp_fib(const uint64_t *__restrict__
                                                                  nnz, F, out, pk, target, fi, per_group, flags, bs);
 }
 
-// elements per lane of the decode kernel: 8 FP32 / BF16, 4 FP64 (<= R)
-template <typename FT, int R> constexpr int dec_vpl() {
-    constexpr int want = sizeof(FT) == 8 ? 4 : 8;
+// elements per lane of the decode-once and fiber kernels (<= R): 4 FP64;
+// FP32 8 from rank 16 and 4 at rank 8 (a rank-8 row is then one gather
+// instruction across 2 lanes instead of two on one lane: half the distinct
+// rows per instruction; 512-tensor batched CP-ALS 0.83 -> 0.81 s with ranks
+// 8 and 16 both at 4, but at rank 16 the changed FP32 rounding moved
+// borderline CP-stream convergence tests, so rank 16 keeps 8); BF16 8
+__host__ __device__ constexpr int dec_vpl_n(int fbytes, int R) {
+    const int want = fbytes == 8 ? 4 : (fbytes == 4 && R <= 8) ? 4 : 8;
     return want < R ? want : R;
 }
+template <typename FT, int R> constexpr int dec_vpl() { return dec_vpl_n((int)sizeof(FT), R); }
 
 // Any rank: one wave per range, lane r covers ranks r, r+64, ...
 template <typename VT, typename FT, typename OT, int PKW>
@@ -999,7 +1005,7 @@ static int launch_cfg(TensorImpl *t, const MttkrpArgs &a0, hipStream_t s) {
     if (!a.out64 && (a.flags & 4) && a.decode_once && (R == 8 || R == 16 || R == 32 || R == 64)) {
         // FP32 output of the decode-once / fiber kernels: the workgroups'
         // boundary rows go through FP64 slots instead of FP32 atomics
-        const int vpl = std::min(a.prec == ALTO_PREC_F64 ? 4 : 8, R);
+        const int vpl = dec_vpl_n(a.prec == ALTO_PREC_F64 ? 8 : a.prec == ALTO_PREC_F32 ? 4 : 2, R);
         const uint64_t groups = (a.nnz + a.per_group - 1) / a.per_group;
         const uint64_t ns = 2 * (uint64_t)cdiv(groups * (R / vpl), 256);
         const uint64_t nl = bnd_level_slots(ns);  // slots of the merge's level buffers
@@ -1312,7 +1318,7 @@ int mttkrp_group_job(TensorImpl *t, int target, int prec, int rank, const void *
     // that fills the device with one tensor.
     job->per_group = 8;
     job->flags = a.flags;
-    const int vpl = prec == ALTO_PREC_F64 ? (rank < 4 ? rank : 4) : (rank < 8 ? rank : 8);
+    const int vpl = dec_vpl_n(prec == ALTO_PREC_F64 ? 8 : prec == ALTO_PREC_F32 ? 4 : 2, rank);
     const uint64_t groups = (a.nnz + job->per_group - 1) / job->per_group;
     job->nblocks = cdiv(groups * (uint64_t)(rank / vpl), 256);
     return ALTO_OK;
