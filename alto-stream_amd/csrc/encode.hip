@@ -394,11 +394,11 @@ int ensure_schedule(TensorImpl *t, int mode, bool need64, hipStream_t s) {
             continue;
         const uint64_t m = o.n;
         ALTO_TRY(sc.pkeys.alloc(sizeof(uint64_t) * pkw * std::max<uint64_t>(m, 1)));
-        ALTO_TRY(sc.vals32.alloc(sizeof(float) * std::max<uint64_t>(m, 1)));
+        ALTO_TRY(sc.vals32.alloc(sizeof(float) * (m + kSchedPad)));
         ALTO_HIP_TRY(hipMemcpyAsync(sc.pkeys.p, o.pkeys.p, sizeof(uint64_t) * pkw * m, hipMemcpyDeviceToDevice, s));
         ALTO_HIP_TRY(hipMemcpyAsync(sc.vals32.p, o.vals32.p, sizeof(float) * m, hipMemcpyDeviceToDevice, s));
         if (need64) {
-            ALTO_TRY(sc.vals64.alloc(sizeof(double) * std::max<uint64_t>(m, 1)));
+            ALTO_TRY(sc.vals64.alloc(sizeof(double) * (m + kSchedPad)));
             ALTO_HIP_TRY(hipMemcpyAsync(sc.vals64.p, o.vals64.p, sizeof(double) * m, hipMemcpyDeviceToDevice, s));
         }
         sc.n = m;
@@ -437,8 +437,8 @@ int ensure_schedule(TensorImpl *t, int mode, bool need64, hipStream_t s) {
     ALTO_HIP_TRY(hipStreamSynchronize(s));
     const uint64_t m = (uint64_t)last_pos + last_head;
     ALTO_TRY(sc.pkeys.alloc(sizeof(uint64_t) * pkw * m));
-    ALTO_TRY(sc.vals32.alloc(sizeof(float) * m));
-    if (need64) ALTO_TRY(sc.vals64.alloc(sizeof(double) * m));
+    ALTO_TRY(sc.vals32.alloc(sizeof(float) * (m + kSchedPad)));
+    if (need64) ALTO_TRY(sc.vals64.alloc(sizeof(double) * (m + kSchedPad)));
     ALTO_TRY(longs.alloc(sizeof(uint64_t) * (n / kShortRun + 2)));
     unsigned long long *nlong = (unsigned long long *)(longs.as<uint64_t>() + n / kShortRun + 1);
     ALTO_HIP_TRY(hipMemsetAsync(nlong, 0, sizeof(unsigned long long), s));

@@ -275,7 +275,13 @@ int ensure_hot_schedule(TensorImpl *t, int target, int rb, hipStream_t s) {
         he.K[m] = h.K[q];
         he.rank[m] = t->hot.rank[m].as<uint32_t>();
     }
-    ALTO_TRY(h.keys.alloc(sizeof(uint64_t) * t->pkw * (sc.n ? sc.n : 1)));
+    ALTO_TRY(h.keys.alloc(sizeof(uint64_t) * t->pkw * (sc.n + kSchedPad)));
+    {
+        const uint64_t words = 2 * (uint64_t)t->pkw * kSchedPad;  // the zero padding, as u32 words
+        k_fill_u32<<<cdiv(words, 256), 256, 0, s>>>(h.keys.as<uint32_t>() + 2 * (uint64_t)t->pkw * sc.n, words,
+                                                     0u);
+        ALTO_LAUNCH_CHECK();
+    }
     if (sc.n) {
         k_hot_encode<<<cdiv(sc.n, 256), 256, 0, s>>>(sc.pkeys.as<uint64_t>(), t->pkw, sc.n, he,
                                                      h.keys.as<uint64_t>());

@@ -93,11 +93,14 @@ struct HotArgs {
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
+// E entries' keys and values from position pos, with no bounds branch: the
+// arrays carry kSchedPad entries of padding after their end (a branch here
+// would leave the loads pending on one path, and the compiler then drains
+// every load before the first gather batch's products).
 template <int PKW, int E, bool NT>
-__device__ __forceinline__ void load_keys(const uint64_t *__restrict__ p, uint64_t pos, uint64_t nall,
-                                          uint64_t *k) {
+__device__ __forceinline__ void load_keys(const uint64_t *__restrict__ p, uint64_t pos, uint64_t *k) {
     constexpr int NW = E * PKW;
-    if (pos + E <= nall && (NW % 2) == 0) {
+    if constexpr ((NW % 2) == 0) {
         const u32x4 *q = (const u32x4 *)(p + pos * PKW);
 #pragma unroll
         for (int i = 0; i < NW / 2; ++i) {
@@ -107,19 +110,14 @@ __device__ __forceinline__ void load_keys(const uint64_t *__restrict__ p, uint64
         }
     } else {
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const uint64_t j = pos + e < nall ? pos + e : nall - 1;
-#pragma unroll
-            for (int w = 0; w < PKW; ++w)
-                k[e * PKW + w] = NT ? __builtin_nontemporal_load(p + j * PKW + w) : p[j * PKW + w];
-        }
+        for (int w = 0; w < NW; ++w) k[w] = NT ? __builtin_nontemporal_load(p + pos * PKW + w) : p[pos * PKW + w];
     }
 }
 
 template <typename VT, int E, bool NT>
-__device__ __forceinline__ void load_vals(const VT *__restrict__ p, uint64_t pos, uint64_t nall, VT *v) {
+__device__ __forceinline__ void load_vals(const VT *__restrict__ p, uint64_t pos, VT *v) {
     constexpr int PER16 = 16 / (int)sizeof(VT);
-    if (pos + E <= nall && (E % PER16) == 0) {
+    if constexpr ((E % PER16) == 0) {
         const u32x4 *q = (const u32x4 *)(p + pos);
 #pragma unroll
         for (int i = 0; i < E / PER16; ++i) {
@@ -133,10 +131,7 @@ __device__ __forceinline__ void load_vals(const VT *__restrict__ p, uint64_t pos
         }
     } else {
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const uint64_t j = pos + e < nall ? pos + e : nall - 1;
-            v[e] = NT ? __builtin_nontemporal_load(p + j) : p[j];
-        }
+        for (int e = 0; e < E; ++e) v[e] = NT ? __builtin_nontemporal_load(p + pos + e) : p[pos + e];
     }
 }
 
@@ -351,8 +346,8 @@ __global__ __launch_bounds__(NT) void k_mttkrp_hot(const uint64_t *__restrict__ 
     uint64_t ck[E * PKW];
     VT cv[E];
     if (start < end) {
-        load_keys<PKW, E, true>(pkeys, start + (uint64_t)lig * E, n, ck);
-        load_vals<VT, E, true>(vals, start + (uint64_t)lig * E, n, cv);
+        load_keys<PKW, E, true>(pkeys, start + (uint64_t)lig * E, ck);
+        load_vals<VT, E, true>(vals, start + (uint64_t)lig * E, cv);
     }
     for (uint64_t base = start; base < end; base += BLK) {
         // decode this lane's E entries
@@ -371,10 +366,6 @@ __global__ __launch_bounds__(NT) void k_mttkrp_hot(const uint64_t *__restrict__ 
             Bits32<VT>::split(cv[e], rval[e]);
         }
         const uint64_t nb = base + BLK;
-        if (nb < end) {  // prefetch the next block
-            load_keys<PKW, E, true>(pkeys, nb + (uint64_t)lig * E, n, ck);
-            load_vals<VT, E, true>(vals, nb + (uint64_t)lig * E, n, cv);
-        }
         const int nh = (end - base) < (uint64_t)BLK ? (int)(end - base) : BLK;
         auto batch = [&](auto e0c) {
             constexpr int e0 = decltype(e0c)::value;
@@ -413,6 +404,17 @@ __global__ __launch_bounds__(NT) void k_mttkrp_hot(const uint64_t *__restrict__ 
                         }
                     }
                 }
+            // The next block's keys and values are loaded after the first
+            // batch's gathers: loads retire in order, so the first batch's
+            // wait does not include them, and they get that batch's time.
+            // Unconditional (a branch would leave them pending on one path):
+            // past the group's last block this reads the next group's
+            // entries or the padding (base < end, so at most 2 BLK - 1 <
+            // kSchedPad entries past the end), which nothing uses.
+            if constexpr (e0 == 0) {
+                load_keys<PKW, E, true>(pkeys, nb + (uint64_t)lig * E, ck);
+                load_vals<VT, E, true>(vals, nb + (uint64_t)lig * E, cv);
+            }
 #pragma unroll
             for (int k = 0; k < CB; ++k) {
                 // entries past the group's end add zero to the current run

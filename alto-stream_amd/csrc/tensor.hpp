@@ -40,6 +40,12 @@ struct HotSched {
     DevBuf keys;
 };
 
+// Entries of padding after a schedule's values and the hot-row kernel's
+// re-encoded keys (zero keys: row 0 of every mode): the hot-row kernel loads
+// whole blocks of keys and values without a bounds branch, and its entries
+// past the end add zero.
+constexpr uint64_t kSchedPad = 64;
+
 // Output-ordered schedule of one mode (algo 2): the nonzeros re-sorted by
 // the mode's index, stably, so ALTO order is kept inside each row run.
 struct ModeSchedule {
