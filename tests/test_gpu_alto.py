@@ -137,6 +137,31 @@ def test_errors_are_status_codes(gpu):
         gpu.create_alto([1 << 40, 1 << 40], [np.zeros(1, np.uint64)] * 2, np.ones(1), key_words=1)
 
 
+def test_c_abi_refuses_a_short_output_buffer(gpu):
+    """alto_hip_mttkrp takes the output's size in bytes and refuses a buffer
+    smaller than the rows the kernel writes (FP64 rows for out_f64, FP32
+    rows otherwise), before anything is launched: the output keeps its bytes."""
+    import ctypes
+    dims, cidx, vals = rand_tensor([40, 30, 20], 800, seed=38)
+    at = gpu.create_alto(dims, cidx, vals)
+    R = 8
+    F = rand_factors(dims, R, seed=39)
+    devF = [gpu.to_prec(f, gpu.PREC_F32) for f in F]
+    out = gpu.DeviceArray.from_numpy(np.full((dims[0], R), 7.0, np.float64))
+    L = gpu.lib()
+    ptrs = (ctypes.c_void_p * 3)(*[f.ptr for f in devF])
+    call = lambda out_f64, nbytes: L.alto_hip_mttkrp(ctypes.c_void_p(at.handle), 0, gpu.PREC_F32, R, ptrs,
+                                                     ctypes.c_void_p(out.ptr), out_f64, ctypes.c_size_t(nbytes),
+                                                     None)
+    assert call(1, dims[0] * R * 8 - 1) == -1  # ALTO_ERR_ARG: FP64 rows need 8 bytes each
+    assert call(0, dims[0] * R * 4 - 4) == -1  # FP32 rows, one element short
+    gpu.synchronize()
+    assert np.array_equal(out.numpy(), np.full((dims[0], R), 7.0))
+    assert call(0, dims[0] * R * 4) == 0  # exactly the FP32 rows: accepted
+    gpu.synchronize()
+    at.close()
+
+
 class _Ref:
     def __init__(self, dims, cidx, vals):
         self.t = O.AltoTensor(dims, cidx, vals, nprtn=8)
